@@ -1,0 +1,341 @@
+"""Pins the CPU oracle with closed forms, invariants and brute force.
+
+These checks are chosen so that a plausible mistake in the restatement (a
+shifted knot, a wrong sign, a dropped term, a transposed dim, a wrong BC
+order) fails at least one of them.  Reference anchors are cited per test.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import CENTER as C, VERTEX as V
+
+
+def _fma(a, b, c):
+    """Correctly rounded a*b+c (exact rational arithmetic, one rounding)."""
+    return float(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def vloc(D, nd):
+    return tuple(V if d == D else C for d in range(nd))
+
+
+# ---------------------------------------------------------------- operators
+def test_partial_of_affine_is_exact():
+    """∂ of an affine field equals its slope at every launcher point whose
+    knots lie in the set region (partial_derivatives.jl:2-5)."""
+    g = O.Grid((-1, -2, 0.5), (2, 3, 1), (6, 5, 4))
+    a = (0.75, -1.25, 2.0)
+    for src_loc in (C, V, (V, C, C), (C, V, V)):
+        f = O.HField(g, src_loc)
+        xs = [g.coords(f.loc, d, first=0, count=f.n[d] + 2) for d in range(3)]
+        X, Y, Z = np.meshgrid(*xs, indexing="ij")
+        f.interior(with_halo=True)[...] = a[0] * X + a[1] * Y + a[2] * Z
+        for D in range(3):
+            out_loc = tuple(1 - l if d == D else l for d, l in enumerate(f.loc))
+            d = O.HField(g, out_loc)
+            O.op("partial", g, d, f, dim=D, b=O.interior_box(d))
+            assert np.allclose(d.interior(), a[D], rtol=1e-12, atol=1e-12)
+
+
+def test_summation_by_parts_1d():
+    """Σ_c P_c ∂V_c = -Σ_v V_v ∂P_v when V vanishes on the boundary vertices:
+    pins the Vertex->Center knots (I, I+1) against the Center->Vertex knots
+    (I-1, I) (GridOperators.jl:26-33)."""
+    rng = np.random.default_rng(3)
+    g = O.Grid((0,), (1,), (9,))
+    P = O.HField(g, C)
+    Vf = O.HField(g, V)
+    P.set_interior(rng.standard_normal(9))
+    v = rng.standard_normal(10)
+    v[0] = v[-1] = 0.0
+    Vf.set_interior(v)
+    dV = O.HField(g, C)
+    dP = O.HField(g, V)
+    O.op("partial", g, dV, Vf, dim=0, b=O.interior_box(dV))
+    O.op("partial", g, dP, P, dim=0, b=O.box([2], [8]))
+    lhs = np.sum(P.interior() * dV.interior())
+    rhs = -np.sum(Vf.interior()[1:-1] * dP.interior()[1:-1])
+    assert np.isclose(lhs, rhs, rtol=1e-12)
+
+
+def test_lerp_bilinear_exact_and_last_dim_outermost():
+    """lerp reproduces bilinear fields exactly; with fma(0.5, b-a, a) nested
+    last-dim-outermost the 4-point value is bitwise the hand-nested form
+    (interpolation.jl:14-25)."""
+    g = O.Grid((0, 0), (1, 1), (4, 3))
+    rng = np.random.default_rng(0)
+    f = O.HField(g, C)
+    f.arr[...] = rng.standard_normal(f.arr.shape)
+    d = O.HField(g, V)
+    O.op("lerp", g, d, f, b=O.interior_box(d))
+    F = f.xyz()
+    for i in range(2, 5):
+        for j in range(2, 4):
+            a00, a10 = F[i - 1 + 1, j - 1 + 1], F[i + 1, j - 1 + 1]
+            a01, a11 = F[i - 1 + 1, j + 1], F[i + 1, j + 1]
+            lo = _fma(0.5, a10 - a00, a00)
+            hi = _fma(0.5, a11 - a01, a01)
+            assert d[i, j] == _fma(0.5, hi - lo, lo)
+    # bilinear field is reproduced (to rounding)
+    xs = g.coords(C, 0, first=0, count=6)
+    ys = g.coords(C, 1, first=0, count=5)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    f.interior(with_halo=True)[...] = 1.5 + 2 * X - 3 * Y + 0.5 * X * Y
+    O.op("lerp", g, d, f, b=O.interior_box(d))
+    xv, yv = g.coords(V, 0), g.coords(V, 1)
+    Xv, Yv = np.meshgrid(xv, yv, indexing="ij")
+    assert np.allclose(d.interior(), 1.5 + 2 * Xv - 3 * Yv + 0.5 * Xv * Yv, atol=1e-13)
+
+
+# ---------------------------------------------------------------- BC ordering
+def test_bc_corner_order_z_y_x():
+    """bc! runs dims N..1, side 1 then 2, over tangential 0..n+2
+    (batch.jl:20-29,159,181): on a 2D Center field with Neumann in y and
+    Dirichlet(5) in x, corner ghosts take the x rule applied last to values
+    the y pass wrote."""
+    g = O.Grid((0, 0), (1, 1), (4, 4))
+    f = O.HField(g, C)
+    f.set_interior(np.arange(16, dtype=float).reshape(4, 4, order="F") + 1)
+    O.bc_apply(g, [(f, {0: O.Dirichlet(5.0), 1: O.Neumann()})])
+    # y pass first: f[i,0] = f[i,1], f[i,5] = f[i,4] for i in 0..6 (incl. padding column 6)
+    # then x pass: f[0,j] = 2*(5 - f[1,j]) + f[1,j] for j in 0..6
+    for j in range(0, 6):
+        assert f[0, j] == math.fsum([2 * (5 - f[1, j]), f[1, j]]) or np.isclose(f[0, j], 10 - f[1, j])
+    assert f[0, 0] == pytest.approx(10 - f[1, 1])  # corner: x rule on the y-copied value
+    assert f[1, 0] == f[1, 1]
+    # tangential range reaches index n+2 = 6 (the padding column): written by the y pass
+    assert f[6, 0] == f[6, 1] == 0.0
+
+
+# ---------------------------------------------------------------- diffusion
+def _diffusion_run(n, steps, C0, T=np.float64):
+    """examples/diffusion_2d.jl:44-48 loop: compute_q!, update_C! + Neumann bc."""
+    g = O.Grid((-1, -1), (2, 2), (n, n))
+    Cf = O.HField(g, C, T)
+    q = [O.HField(g, vloc(D, 2), T) for D in range(2)]
+    Cf.set_interior(C0)
+    O.bc_apply(g, [(Cf, O.Neumann())])
+    dt = min(g.spacing) ** 2 / 1.0 / 4.1
+    for _ in range(steps):
+        O.compute_q(g, q, Cf, 1.0)
+        O.update_C(g, Cf, q, dt)
+        O.bc_apply(g, [(Cf, O.Neumann())])
+    return g, Cf, dt
+
+
+def test_diffusion_cosine_mode_decay():
+    """Cell-centred cosines cos(kπ(i-½)/n) are eigenvectors of the discrete
+    Neumann Laplacian with λ = (4/Δ²)sin²(kπ/2n): after m steps the
+    amplitude is (1 - Δt·χ·(λx+λy))^m."""
+    n, kx, ky, m = 16, 2, 3, 40
+    i = np.arange(1, n + 1)
+    mode = np.cos(kx * np.pi * (i[:, None] - 0.5) / n) * np.cos(ky * np.pi * (i[None, :] - 0.5) / n)
+    g, Cf, dt = _diffusion_run(n, m, mode)
+    dx = g.spacing[0]
+    lam = 4 / dx ** 2 * (np.sin(kx * np.pi / (2 * n)) ** 2 + np.sin(ky * np.pi / (2 * n)) ** 2)
+    assert np.allclose(Cf.interior(), (1 - dt * lam) ** m * mode, atol=1e-12)
+
+
+def test_diffusion_conserves_mass_and_symmetry():
+    """Flux form + Neumann(0): Σ C is conserved; an x<->y symmetric IC stays symmetric."""
+    n = 24
+    rng = np.random.default_rng(1)
+    A = rng.random((n, n))
+    A = A + A.T
+    s0 = A.sum()
+    g, Cf, _ = _diffusion_run(n, 50, A)
+    ci = Cf.interior()
+    assert np.isclose(ci.sum(), s0, rtol=1e-13)
+    assert np.allclose(ci, ci.T, atol=1e-13)
+
+
+# ---------------------------------------------------------------- Stokes
+def _stokes_fields(g, T=np.float64):
+    nd = g.nd
+    tau_locs = [(C, C), (C, C), (V, V)] if nd == 2 else \
+        [(C, C, C)] * 3 + [(V, V, C), (V, C, V), (C, V, V)]
+    F = dict(
+        Pr=O.HField(g, C, T), dV=O.HField(g, C, T),
+        V=[O.HField(g, vloc(D, nd), T) for D in range(nd)],
+        rV=[O.HField(g, vloc(D, nd), T) for D in range(nd)],
+        tau=[O.HField(g, l, T) for l in tau_locs],
+        tauo=[O.HField(g, l, T) for l in tau_locs],
+    )
+    return F
+
+
+def _bc_V(g, F, vals):
+    """Free slip with Dirichlet normal velocity (vals[D] = (left, right)),
+    tangential Neumann (examples/stokes_3d_inc_ve_T.jl:188-190)."""
+    nd = g.nd
+    pairs = []
+    for D in range(nd):
+        spec = {d: O.Neumann() for d in range(nd)}
+        spec[D] = (O.Dirichlet(vals[D][0]), O.Dirichlet(vals[D][1]))
+        pairs.append((F["V"][D], spec))
+    O.bc_apply(g, pairs)
+
+
+def _pure_shear(g, F):
+    """Vx = -2x, Vz(or Vy in 2D) = +2z: ±1 on the unit box walls."""
+    nd = g.nd
+    last = nd - 1
+    for D in range(nd):
+        f = F["V"][D]
+        xs = g.coords(f.loc, D, first=0, count=f.n[D] + 2)
+        sl = [None] * nd
+        sl[D] = slice(None)
+        rate = -2.0 if D == 0 else (2.0 if D == last else 0.0)
+        f.interior(with_halo=True)[...] = rate * xs[tuple(sl)]
+    return {D: ((1.0, -1.0) if D == 0 else ((-1.0, 1.0) if D == last else (0.0, 0.0))) for D in range(nd)}
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_stokes_pure_shear_is_a_fixed_point(nd):
+    """Homogeneous η + pure-shear Dirichlet: linear V, Pr = 0 and
+    τ = 2η_ve·ε̇ (τ_old = 0) make r_τ = 0 and r_V = 0, so one PT iteration
+    leaves the state unchanged (examples/stokes_3d_inc_ve_T.jl:81-115)."""
+    n = 8
+    g = O.Grid((-0.5,) * nd, (1.0,) * nd, (n,) * nd)
+    F = _stokes_fields(g)
+    vals = _pure_shear(g, F)
+    _bc_V(g, F, vals)
+    eta, G, dt = 1.0, 1.0, 0.25
+    eta_ve = 1.0 / (1.0 / eta + 1.0 / (G * dt))
+    last = nd - 1
+    F["tau"][0].arr[...] = 2 * eta_ve * -2.0
+    F["tau"][last].arr[...] = 2 * eta_ve * 2.0
+    p = O.pt_params(g)
+    V0 = [f.interior().copy() for f in F["V"]]
+    O.update_stress(g, F["tau"], F["Pr"], F["dV"], F["V"], F["tauo"], eta, eta_ve, G, dt, p["dtau_Pr"], p["dtau_r"])
+    O.update_velocity(g, F["V"], F["rV"], F["Pr"], F["tau"], None, eta_ve, p["nudtau"])
+    _bc_V(g, F, vals)
+    for D in range(nd):
+        assert np.allclose(F["V"][D].interior(), V0[D], atol=1e-12)
+        assert np.max(np.abs(F["rV"][D].interior()[tuple(slice(1, -1) if d == D else slice(None) for d in range(nd))])) < 1e-9
+    assert np.max(np.abs(F["Pr"].interior())) < 1e-12
+    assert np.allclose(F["tau"][0].interior(), 2 * eta_ve * -2.0, atol=1e-12)
+
+
+def _pt_solve(g, F, vals, eta, G, dt, niter, tol, eta_field=None, eta_ml=None):
+    p = O.pt_params(g)
+    nd = g.nd
+    eta_ve = 1.0 / (1.0 / eta + 1.0 / (G * dt))
+    err = None
+    for it in range(1, niter + 1):
+        O.update_stress(g, F["tau"], F["Pr"], F["dV"], F["V"], F["tauo"], eta_field if eta_field is not None else eta,
+                        eta_ve, G, dt, p["dtau_Pr"], p["dtau_r"])
+        O.update_velocity(g, F["V"], F["rV"], F["Pr"], F["tau"], None, eta_ve, p["nudtau"], eta_ml=eta_ml, G=G, dt=dt)
+        _bc_V(g, F, vals)
+        if it % (2 * g.dims[0]) == 0:
+            O.bc_apply(g, [(F["rV"][D], {D: O.Dirichlet()}) for D in range(nd)])
+            err = max([O.maxabs(F["dV"])] + [O.maxabs(f) for f in F["rV"]])
+            if err < tol:
+                return it, err
+    return niter, err
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_stokes_pt_converges_to_pure_shear(nd):
+    """From V = 0 the PT iteration reaches the exact linear pure-shear state."""
+    n = 12
+    g = O.Grid((-0.5,) * nd, (1.0,) * nd, (n,) * nd)
+    F = _stokes_fields(g)
+    vals = _pure_shear(g, F)
+    Vexact = [f.interior().copy() for f in F["V"]]
+    for f in F["V"]:
+        f.arr[...] = 0
+    _bc_V(g, F, vals)
+    it, err = _pt_solve(g, F, vals, 1.0, 1.0, 0.25, 20000, 1e-10)
+    assert err < 1e-10, (it, err)
+    for D in range(nd):
+        assert np.allclose(F["V"][D].interior(), Vexact[D], atol=1e-8)
+    assert np.max(np.abs(F["Pr"].interior())) < 1e-8
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_stokes_field_viscosity_reduces_to_scalar(nd):
+    """Build extension: a constant η field reproduces the reference scalar
+    form bit for bit (lerp of a constant and max over equal values are exact,
+    η_ve is evaluated with the same expression)."""
+    n = 6
+    g = O.Grid((-0.5,) * nd, (1.0,) * nd, (n,) * nd)
+    rng = np.random.default_rng(7)
+    A = _stokes_fields(g)
+    for f in A["V"] + A["tau"] + A["tauo"] + [A["Pr"]]:
+        f.arr[...] = rng.standard_normal(f.arr.shape)
+    B = {k: ([f.copy() for f in v] if isinstance(v, list) else v.copy()) for k, v in A.items()}
+    eta, G, dt = 0.7, 1.0, 0.3
+    eta_ve = 1.0 / (1.0 / eta + 1.0 / (G * dt))
+    ef = O.HField(g, C)
+    ef.arr[...] = eta
+    p = O.pt_params(g)
+    O.update_stress(g, A["tau"], A["Pr"], A["dV"], A["V"], A["tauo"], eta, eta_ve, G, dt, p["dtau_Pr"], p["dtau_r"])
+    O.update_velocity(g, A["V"], A["rV"], A["Pr"], A["tau"], None, eta_ve, p["nudtau"])
+    O.update_stress(g, B["tau"], B["Pr"], B["dV"], B["V"], B["tauo"], ef, 0.0, G, dt, p["dtau_Pr"], p["dtau_r"])
+    O.update_velocity(g, B["V"], B["rV"], B["Pr"], B["tau"], None, 0.0, p["nudtau"], eta_ml=ef, G=G, dt=dt)
+    for k in ("V", "tau", "rV"):
+        for a, b in zip(A[k], B[k]):
+            assert np.array_equal(a.arr, b.arr)
+    assert np.array_equal(A["Pr"].arr, B["Pr"].arr)
+
+
+def test_stokes_inclusion_mirror_symmetry():
+    """A centred circular weak inclusion under pure shear: Vx is odd in x and
+    even in y, Vy even in x and odd in y, Pr even in both (a transposed or
+    shifted stencil breaks these)."""
+    n = 16
+    g = O.Grid((-0.5, -0.5), (1.0, 1.0), (n, n))
+    F = _stokes_fields(g)
+    vals = _pure_shear(g, F)
+    for f in F["V"]:
+        f.arr[...] = 0
+    _bc_V(g, F, vals)
+    ef = O.HField(g, C)
+    xs = g.coords(C, 0, first=-1, count=n + 4)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    ef.xyz()[...] = np.where(X ** 2 + Y ** 2 < 0.1 ** 2, 1e-2, 1.0)
+    eml = O.HField(g, C)
+    O.maxloc(g, eml, ef)
+    _pt_solve(g, F, vals, 1.0, 1.0, 0.25, 600, 0.0, eta_field=ef, eta_ml=eml)
+    vx, vy, pr = F["V"][0].interior(), F["V"][1].interior(), F["Pr"].interior()
+    assert np.allclose(vx, -vx[::-1, :], atol=1e-12) and np.allclose(vx, vx[:, ::-1], atol=1e-12)
+    assert np.allclose(vy, vy[::-1, :], atol=1e-12) and np.allclose(vy, -vy[:, ::-1], atol=1e-12)
+    assert np.allclose(pr, pr[::-1, :], atol=1e-12) and np.allclose(pr, pr[:, ::-1], atol=1e-12)
+    assert np.max(np.abs(pr)) > 1e-3  # the inclusion does perturb the pressure
+
+
+def test_thermal_pure_diffusion_and_upwind():
+    """update_thermal_flux!/update_thermal! (stokes_3d_inc_ve_T.jl:59-77):
+    with V = 0 the step is the diffusion step (cosine mode decays by
+    1-dt·λ_T·λ); with uniform V>0 the advective flux is V·T[left] (upwind)."""
+    n = 12
+    g = O.Grid((0, 0), (1, 1), (n, n))
+    T = O.HField(g, C)
+    Told = O.HField(g, C)
+    qT = [O.HField(g, vloc(D, 2)) for D in range(2)]
+    Vf = [O.HField(g, vloc(D, 2)) for D in range(2)]
+    i = np.arange(1, n + 1)
+    mode = np.cos(np.pi * (i[:, None] - 0.5) / n) * np.ones((1, n))
+    T.set_interior(mode)
+    O.bc_apply(g, [(T, O.Neumann())])
+    Told.arr[...] = T.arr
+    lam, dt = 0.3, 1e-3
+    O.thermal_flux(g, qT, T, Vf, lam)
+    O.thermal(g, T, Told, qT, dt)
+    dx = g.spacing[0]
+    lx = 4 / dx ** 2 * np.sin(np.pi / (2 * n)) ** 2
+    assert np.allclose(T.interior(), (1 - dt * lam * lx) * mode, atol=1e-13)
+    # upwind: uniform Vx = 2 > 0, Vy = -1 < 0
+    Vf[0].arr[...] = 2.0
+    Vf[1].arr[...] = -1.0
+    R = np.random.default_rng(2).random((n, n))
+    T.set_interior(R)
+    O.thermal_flux(g, qT, T, Vf, 0.0)
+    Ti = T.interior()
+    assert np.allclose(qT[0].interior()[1:-1, :], 2.0 * Ti[:-1, :])   # left_x T at vertex i = T[i-1]
+    assert np.allclose(qT[1].interior()[:, 1:-1], -1.0 * Ti[:, 1:])   # right_y T at vertex j = T[j]
