@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: pseudo-transient 3D viscoelastic Stokes at 255³, FP64, one MI355X.
+
+A step is one PT iteration (examples/stokes_3d_inc_ve_T.jl:222-223): the fused
+stress+velocity kernel over the Launcher box plus the free-slip V boundary
+batch, on fields resident in HBM.  Reported value: T_eff = algorithmic bytes
+per iteration / time per iteration (GB/s), with algorithmic bytes = 27 array
+passes over exact interior counts (updated Vx,Vy,Vz,Pr,τ×6 twice; read
+τ_old×6 and η once).  Setup: [-0.5,0.5]³, background η = 1 with a sphere of
+radius 0.1 at η = 1e-3, times exp(-T) with T = 0.1·exp(-|x|²/0.05²); G = 1;
+dt from the reference's diffusion limit; pure-shear free-slip walls ±1.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "chmy.jl_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_bytes(n, itemsize=8):
+    C = n ** 3
+    Vf = (n + 1) * n * n
+    E = (n + 1) * (n + 1) * n
+    updated = 3 * Vf + 4 * C + 3 * E
+    read = 3 * C + 3 * E + C
+    return (2 * updated + read) * itemsize
+
+
+def reference_bytes(n, itemsize=8):
+    """nIO = 2·10 + 17 arrays × nx·ny·nz (stokes_3d_inc_ve_T_mpi_perf.jl:210-213)."""
+    return 37 * itemsize * n ** 3
+
+
+def setup(n, dtype):
+    import torch
+    import chmy
+    from chmy.stokes import PTStokes, inclusion_setup_3d, pure_shear_initial, pure_shear_values
+    arch = chmy.Arch(chmy.MI355XBackend())
+    g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(n,) * 3)
+    dx = min(chmy.spacing(g))
+    lam = 1e-4 * 1.0 ** 2 / 1.0                  # λ_ρCp = 1e-4 lz²/τsc
+    dt = dx ** 2 / lam / 3 / 2.1                 # dt_diff (stokes_3d_inc_ve_T.jl:215)
+    eta = inclusion_setup_3d(g)
+    S = PTStokes(arch, g, dtype=dtype, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(3))
+    pure_shear_initial(g, S.V)
+    S.apply_bc()
+    S.sync_buffers()
+    torch.cuda.synchronize()
+    return S
+
+
+def cpu_baseline(n, iters):
+    """Time the oracle (C++ restatement of the reference's CPU path: unfused
+    update_stress!, update_velocity!, bc! per iteration, OpenMP) on a bounded
+    sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    from chmy.stokes import inclusion_setup_3d
+    import chmy
+    g = chmy.UniformGrid(origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(n,) * 3)
+    og = O.Grid(g.origin, g.extent, g.dims)
+    C, V = O.CENTER, O.VERTEX
+    tl = [(C, C, C)] * 3 + [(V, V, C), (V, C, V), (C, V, V)]
+    vl = [(V, C, C), (C, V, C), (C, C, V)]
+    Vf = [O.HField(og, l) for l in vl]
+    rV = [O.HField(og, l) for l in vl]
+    tau = [O.HField(og, l) for l in tl]
+    told = [O.HField(og, l) for l in tl]
+    Pr, dV, eta, eml = (O.HField(og, C) for _ in range(4))
+    eta.arr[...] = inclusion_setup_3d(g)
+    O.maxloc(og, eml, eta)
+    p = O.pt_params(og)
+    dx = min(og.spacing)
+    dt = dx ** 2 / 1e-4 / 3 / 2.1
+    spec = []
+    vals = {0: (1.0, -1.0), 1: (0.0, 0.0), 2: (-1.0, 1.0)}
+    for D in range(3):
+        s = {d: O.Neumann() for d in range(3)}
+        s[D] = (O.Dirichlet(vals[D][0]), O.Dirichlet(vals[D][1]))
+        spec.append((Vf[D], s))
+
+    def step():
+        O.update_stress(og, tau, Pr, dV, Vf, told, eta, 0.0, 1.0, dt, p["dtau_Pr"], p["dtau_r"])
+        O.update_velocity(og, Vf, rV, Pr, tau, None, 0.0, p["nudtau"], eta_ml=eml, G=1.0, dt=dt)
+        O.bc_apply(og, spec)
+
+    step()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        step()
+    el = (time.perf_counter() - t0) / iters
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return el, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=255)
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is out of scope)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    isz = 8 if args.dtype == "f64" else 4
+    n = args.n
+    S = setup(n, dtype)
+    print(f"# setup done: {n}^3 {args.dtype}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        S.iteration()
+    torch.cuda.synchronize()
+    K = args.steps
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record()
+    for q in range(K):
+        ev[q][0].record()
+        S._iter_call(False)
+        ev[q][1].record()
+        S.apply_bc(S.V2)
+        S._swap()
+    t_end.record()
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    ms_step = total_ms / K
+    k_ms = sum(a.elapsed_time(b) for a, b in ev) / K
+    A = algorithmic_bytes(n, isz)
+    teff = A / (ms_step * 1e-3) / 1e9
+    achieved = A / (k_ms * 1e-3) / 1e9
+    # residual sanity (not timed)
+    S.iteration(write_res=True)
+    err = S.residual()
+    cpu = None
+    if not args.no_cpu:
+        c_s, cores = cpu_baseline(n, args.cpu_iters)
+        cpu = {"value": round(A / c_s / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_iters} PT iterations at {n}^3 FP64 (oracle: unfused update_stress!, "
+                         f"update_velocity!, bc!; {c_s:.3f} s/iteration)"}
+    out = {
+        "metric": "T_eff GB/s (% HBM peak); time-to-1e-8 FP32+fix vs FP64; adjoint/fwd cost",
+        "value": round(teff, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seedless analytic inclusion setup)",
+        "config": {"workload": f"3D PT viscoelastic Stokes, {n}^3 cells, one PT iteration per step "
+                               "(fused stress+velocity kernel + free-slip V batch)",
+                   "grid": [n, n, n], "teff_pct_of_peak": round(100 * teff / PEAK_HBM_GBS, 2),
+                   "algorithmic_bytes_per_step": A,
+                   "teff_reference_convention_GBs": round(reference_bytes(n, isz) / (ms_step * 1e-3) / 1e9, 2),
+                   "residual_after_timing": err},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "fused3d_kernel<double,true,8>", "kernel_ms": round(k_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

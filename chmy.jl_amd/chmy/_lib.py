@@ -1,0 +1,107 @@
+"""ctypes binding of libchmy_mi355x.so (include/chmy_mi355x.h).
+
+The library is the product: every kernel call goes through it.  Loading
+fails loudly when it is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libchmy_mi355x.so")
+
+F64, F32 = 0, 1
+CENTER, VERTEX = 0, 1
+DIRICHLET, NEUMANN = 0, 1
+VALUE_ZERO, VALUE_SCALAR, VALUE_FIELD = 0, 1, 2
+OP_PARTIAL, OP_LERP, OP_DIVG, OP_VMAG, OP_LEFT, OP_RIGHT = 0, 1, 2, 3, 8, 9
+
+
+class ChmyField(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("nd", C.c_int32), ("dtype", C.c_int32),
+                ("n", C.c_int64 * 3), ("halo", C.c_int32 * 3), ("loc", C.c_int32 * 3)]
+
+
+class ChmyGrid(C.Structure):
+    _fields_ = [("nd", C.c_int32), ("reserved", C.c_int32), ("n", C.c_int64 * 3),
+                ("origin", C.c_double * 3), ("extent", C.c_double * 3),
+                ("spacing", C.c_double * 3), ("inv_spacing", C.c_double * 3)]
+
+
+class ChmyBox(C.Structure):
+    _fields_ = [("lo", C.c_int64 * 3), ("len", C.c_int64 * 3)]
+
+
+class ChmyBC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("vkind", C.c_int32), ("value", C.c_double),
+                ("vfield", C.POINTER(ChmyField))]
+
+
+P = C.POINTER
+PF = P(ChmyField)
+PG = P(ChmyGrid)
+PB = P(ChmyBox)
+VP = C.c_void_p
+D = C.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "chmy_last_error": [],
+    "chmy_init": [C.c_int],
+    "chmy_device_count": [P(C.c_int)],
+    "chmy_malloc_zeros": [P(C.c_void_p), C.c_size_t, VP],
+    "chmy_free": [VP],
+    "chmy_memcpy_h2d": [VP, VP, C.c_size_t, VP],
+    "chmy_memcpy_d2h": [VP, VP, C.c_size_t, VP],
+    "chmy_stream_sync": [VP],
+    "chmy_coords": [PG, C.c_int, C.c_int, C.c_int64, C.c_int64, P(C.c_double)],
+    "chmy_bc_side": [PG, C.c_int, C.c_int, C.c_int, PF, P(ChmyBC), VP],
+    "chmy_bc_batchset": [PG, P(C.c_int32), PF, P(ChmyBC), VP],
+    "chmy_op": [C.c_int, PG, PB, PF, C.c_int, PF, C.c_int, VP],
+    "chmy_diffusion_flux": [PG, PB, PF, PF, D, VP],
+    "chmy_diffusion_update": [PG, PB, PF, PF, D, VP],
+    "chmy_stokes_update_old": [PG, PB, PF, PF, PF, PF, VP],
+    "chmy_stokes_stress": [PG, PB, PF, PF, PF, PF, PF, PF, D, D, D, D, D, D, VP],
+    "chmy_stokes_velocity": [PG, PB, PF, PF, PF, PF, PF, PF, D, D, D, D, VP],
+    "chmy_maxloc": [PG, PB, PF, PF, VP],
+    "chmy_thermal_flux": [PG, PB, PF, PF, PF, D, VP],
+    "chmy_thermal_update": [PG, PB, PF, PF, PF, D, VP],
+    "chmy_stokes_iter": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, C.c_int, D, D, D, D, D, D, D, VP],
+    "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
+    "chmy_maxabs": [PF, PB, P(C.c_double), VP],
+    "chmy_sumsq": [PF, PB, P(C.c_double), VP],
+}
+_RESTYPE = {"chmy_last_error": C.c_char_p}
+
+_lib = None
+
+
+class ChmyError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChmyError(f"{LIB_PATH} not found: build it with `make -C chmy.jl_amd` "
+                            "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    """Call an entry point and raise ChmyError with chmy_last_error() on failure."""
+    L = lib()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        msg = L.chmy_last_error().decode(errors="replace")
+        raise ChmyError(f"{name} failed ({rc}): {msg}")
+    return rc

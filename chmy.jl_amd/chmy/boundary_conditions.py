@@ -1,0 +1,176 @@
+"""bc! batches (src/BoundaryConditions/batch.jl, first_order_boundary_condition.jl).
+
+A per-field condition is a FirstOrderBC (all dims and sides) or a dict
+{"x": bc | (bc_left, bc_right), ...} (the reference's NamedTuple form; missing
+dims are `nothing`).  batch() normalises a list of (field, condition) pairs
+into one FieldBatch per (dim, side), pruning fields without a condition
+(batch.jl:72-157).  bc_() applies the set on the device: dims N..1, side 1 then
+side 2, fields in batch order (batch.jl:20-29).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from .architectures import current_stream
+from .fields import Field
+
+
+class FirstOrderBC:
+    """FirstOrderBC{T,Kind} (first_order_boundary_condition.jl:9-32)."""
+
+    def __init__(self, kind, value=None):
+        self.kind = kind
+        self.value = value
+
+    def __repr__(self):
+        name = "Dirichlet" if self.kind == _lib.DIRICHLET else "Neumann"
+        return f"{name}({0 if self.value is None else self.value})"
+
+    def desc(self, keep):
+        b = _lib.ChmyBC()
+        b.kind = self.kind
+        v = self.value
+        if v is None:
+            b.vkind = _lib.VALUE_ZERO
+        elif isinstance(v, Field):
+            b.vkind = _lib.VALUE_FIELD
+            fd = v.desc()
+            keep.append(fd)
+            b.vfield = C.pointer(fd)
+        else:
+            b.vkind = _lib.VALUE_SCALAR
+            b.value = float(v)
+        return b
+
+
+def Dirichlet(value=None):
+    return FirstOrderBC(_lib.DIRICHLET, value)
+
+
+def Neumann(value=None):
+    return FirstOrderBC(_lib.NEUMANN, value)
+
+
+class EmptyBatch:
+    fields = ()
+    conditions = ()
+
+
+class FieldBatch:
+    """FieldBatch{K} (batch.jl:45-52): K fields with one condition each."""
+
+    def __init__(self, fields, conditions):
+        self.fields = tuple(fields)
+        self.conditions = tuple(conditions)
+
+    def __repr__(self):
+        return f"FieldBatch ({len(self.fields)} fields)"
+
+
+def _regularise(nd, names, spec):
+    """regularise_impl (batch.jl:135-136): per dim, per side condition or None."""
+    if spec is None:
+        return [(None, None)] * nd
+    if isinstance(spec, FirstOrderBC):
+        return [(spec, spec)] * nd
+    if isinstance(spec, dict):
+        out = []
+        for d in range(nd):
+            b = spec.get(names[d], spec.get(d))
+            if isinstance(b, tuple):
+                out.append((b[0], b[1]))
+            else:
+                out.append((b, b))
+        unknown = [k for k in spec if k not in names and k not in range(nd)]
+        if unknown:
+            raise ValueError(f"unknown dims {unknown} in boundary condition")
+        return out
+    raise TypeError(f"unsupported boundary condition {spec!r}")
+
+
+class BatchSet:
+    """NTuple{N, (side-1 batch, side-2 batch)} with cached C descriptors."""
+
+    def __init__(self, grid, per_dim):
+        self.grid = grid
+        self.per_dim = per_dim
+        self._c = None
+
+    def __getitem__(self, d):
+        return self.per_dim[d]
+
+    def _key(self):
+        """Storage pointers the cached descriptors were built from (fields swap
+        storage under double buffering)."""
+        key = []
+        for d in range(self.grid.nd):
+            for s in range(2):
+                b = self.per_dim[d][s]
+                for f, cond in zip(b.fields, b.conditions):
+                    key.append(f.data.data_ptr())
+                    if isinstance(cond.value, Field):
+                        key.append(cond.value.data.data_ptr())
+        return tuple(key)
+
+    def _descriptors(self):
+        key = self._key()
+        if self._c is None or self._c[0] != key:
+            keep = []
+            counts = (C.c_int32 * 6)()
+            fl, bl = [], []
+            for d in range(self.grid.nd):
+                for s in range(2):
+                    b = self.per_dim[d][s]
+                    counts[2 * d + s] = len(b.fields)
+                    for f, cond in zip(b.fields, b.conditions):
+                        fl.append(f.desc())
+                        bl.append(cond.desc(keep))
+            n = max(1, len(fl))
+            fa = (_lib.ChmyField * n)(*fl)
+            ba = (_lib.ChmyBC * n)(*bl)
+            self._c = (key, counts, fa, ba, keep)
+        return self._c[1:]
+
+    def apply(self, stream=None):
+        counts, fa, ba, _keep = self._descriptors()
+        _lib.call("chmy_bc_batchset", C.byref(self.grid.desc()), counts, fa, ba,
+                  current_stream() if stream is None else stream)
+
+    def apply_side(self, d, s, stream=None):
+        """bc!(Side(s), Dim(d), arch, grid, batch[d][s]) — one side of one dim."""
+        b = self.per_dim[d][s - 1]
+        if not b.fields:
+            return
+        keep = []
+        fa = (_lib.ChmyField * len(b.fields))(*[f.desc() for f in b.fields])
+        ba = (_lib.ChmyBC * len(b.fields))(*[c.desc(keep) for c in b.conditions])
+        _lib.call("chmy_bc_side", C.byref(self.grid.desc()), d, s, len(b.fields), fa, ba,
+                  current_stream() if stream is None else stream)
+
+
+def batch(grid, *pairs, exchange=None):
+    """batch(grid, f => bc, ...; exchange) (batch.jl:72-76).  `exchange` names
+    fields to exchange with neighbouring ranks; on a single-device Bounded
+    grid it is a no-op (batch.jl:103-105)."""
+    names = grid.axes_names()
+    regs = [(f, _regularise(grid.nd, names, spec)) for f, spec in pairs]
+    per_dim = []
+    for d in range(grid.nd):
+        sides = []
+        for s in range(2):
+            fs = [f for f, r in regs if r[d][s] is not None]
+            cs = [r[d][s] for f, r in regs if r[d][s] is not None]
+            sides.append(FieldBatch(fs, cs) if fs else EmptyBatch())
+        per_dim.append(tuple(sides))
+    return BatchSet(grid, per_dim)
+
+
+def bc_(arch, grid, *pairs_or_set, exchange=None):
+    """bc!(arch, grid, f => bc, ...) or bc!(arch, grid, batchset) (batch.jl:20-29,157)."""
+    if len(pairs_or_set) == 1 and isinstance(pairs_or_set[0], BatchSet):
+        bs = pairs_or_set[0]
+    else:
+        bs = batch(grid, *pairs_or_set, exchange=exchange)
+    bs.apply()
+    return bs

@@ -1,0 +1,188 @@
+"""Pseudo-transient viscoelastic Stokes solver on the MI355X library.
+
+The time/PT loop of examples/stokes_3d_inc_ve_T.jl:210-239 (2D:
+stokes_2d_inc_ve_T.jl) for one physical time step, driven from the host:
+each PT iteration is one fused stress+velocity kernel (chmy_stokes_iter,
+double-buffered) plus the free-slip V batch; every `ncheck` iterations the
+kernel also writes ∇V and r_V, the boundary residual vertices are zeroed
+(Dirichlet on r_V, :229) and the scaled L∞ norms are reduced on the device
+(:230-237).
+
+Variable viscosity (a build extension; the reference uses a scalar η): η is a
+Center field given on its full padded storage; η_ml (its max over the 3^N
+cell neighbourhood) sets the velocity pseudo-step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .architectures import current_stream
+from .boundary_conditions import Dirichlet, Neumann, batch
+from .fields import Field, TensorField, VectorField, set_storage
+from .grids import Center, UniformGrid, spacing
+from .kernels import _arr, _box, maxloc, HipKernel
+from .kernel_launch import Launcher
+
+
+def pt_params(grid, re_m=2.3 * math.pi, r=0.5):
+    """PT parameters (stokes_3d_inc_ve_T.jl:163-170)."""
+    lt = min(grid.extent) / re_m
+    vdt = min(spacing(grid)) / math.sqrt(grid.nd * 1.1)
+    th = lt * (r + 4 / 3) / vdt
+    return dict(dtau_r=1.0 / (th + 1.0), nudtau=vdt * lt, dtau_Pr=r / th)
+
+
+def free_slip_batch(grid, V, normal_values):
+    """V.x => (x=Dirichlet(vl, vr), y=Neumann(), z=Neumann()), ... (stokes_3d_inc_ve_T.jl:188-190)
+    with per-side normal velocities normal_values[D] = (left, right)."""
+    names = grid.axes_names()
+    pairs = []
+    for D in range(grid.nd):
+        spec = {n: Neumann() for n in names}
+        vl, vr = normal_values[D]
+        spec[names[D]] = (Dirichlet(vl), Dirichlet(vr))
+        pairs.append((V[D], spec))
+    return batch(grid, *pairs)
+
+
+class PTStokes:
+    """Fields and loop of one PT solve.  dtype float64 or float32."""
+
+    def __init__(self, arch, grid, *, dtype=torch.float64, eta=None, eta_field=None, G=1.0, dt=None,
+                 normal_values=None, re_m=2.3 * math.pi, r=0.5, rhog=None):
+        self.arch, self.grid, self.dtype = arch, grid, dtype
+        nd = grid.nd
+        self.nd = nd
+        mk = lambda: (VectorField(arch, grid, dtype), TensorField(arch, grid, dtype), Field(arch, grid, Center(), dtype))
+        self.V, self.tau, self.Pr = mk()
+        self.V2, self.tau2, self.Pr2 = mk()
+        self.tau_old = TensorField(arch, grid, dtype)
+        self.divV = Field(arch, grid, Center(), dtype)
+        self.r_V = VectorField(arch, grid, dtype)
+        self.rhog = rhog
+        self.G = float(G)
+        self.dt = float(dt) if dt is not None else 1.0
+        self.eta_s = float(eta) if eta is not None else 1.0
+        self.eta_ve_s = 1.0 / (1.0 / self.eta_s + 1.0 / (self.G * self.dt))
+        self.eta = None
+        self.eta_ml = None
+        if eta_field is not None:
+            self.eta = Field(arch, grid, Center(), dtype)
+            set_storage(self.eta, eta_field)
+            self.eta_ml = Field(arch, grid, Center(), dtype)
+            Launcher(arch, grid)(arch, grid, (maxloc, (self.eta_ml, self.eta)))
+        self.p = pt_params(grid, re_m, r)
+        nv = normal_values or {D: (0.0, 0.0) for D in range(nd)}
+        self.normal_values = nv
+        self._bcs = {}
+        self._rbc = {}
+        self.iters = 0
+
+    # ------------------------------------------------------------------ state
+    def sync_buffers(self):
+        """Make the output buffers equal to the inputs (also outside the box)."""
+        for a, b in zip(list(self.V) + list(self.tau) + [self.Pr], list(self.V2) + list(self.tau2) + [self.Pr2]):
+            b.data.copy_(a.data)
+
+    def apply_bc(self, V=None):
+        V = V if V is not None else self.V
+        key = V[0].data.data_ptr()
+        bs = self._bcs.get(key)
+        if bs is None:
+            bs = free_slip_batch(self.grid, V, self.normal_values)
+            self._bcs[key] = bs
+        bs.apply()
+
+    def _iter_call(self, write_res):
+        g = self.grid
+        _lib.call("chmy_stokes_iter", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
+                  _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(self.tau_old),
+                  C.byref(self.eta.desc()) if self.eta is not None else None,
+                  C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  C.byref(self.divV.desc()), _arr(self.r_V), int(write_res), self.eta_s, self.eta_ve_s, self.G,
+                  self.dt, self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], current_stream())
+
+    def _swap(self):
+        for a, b in zip(list(self.V) + list(self.tau) + [self.Pr], list(self.V2) + list(self.tau2) + [self.Pr2]):
+            a.swap(b)
+
+    def iteration(self, write_res=False):
+        """One PT iteration: fused kernel into the output buffers, V batch on
+        them, then swap so that (V, tau, Pr) hold the new state."""
+        self._iter_call(write_res)
+        self.apply_bc(self.V2)
+        self._swap()
+        self.iters += 1
+
+    def residual(self, tau_sc=1.0, lz=1.0, psc=1.0):
+        """err = (max|∇V|·τsc, max|r_V.x|·lz/psc, ...) after zeroing boundary
+        residual vertices (stokes_3d_inc_ve_T.jl:229-233).  Requires the last
+        iteration to have written the residual fields."""
+        names = self.grid.axes_names()
+        key = self.r_V[0].data.data_ptr()
+        rb = self._rbc.get(key)
+        if rb is None:
+            rb = batch(self.grid, *[(self.r_V[D], {names[D]: Dirichlet()}) for D in range(self.nd)])
+            self._rbc[key] = rb
+        rb.apply()
+        from .kernels import maxabs
+        errs = [maxabs(self.divV) * tau_sc] + [maxabs(self.r_V[D]) * lz / psc for D in range(self.nd)]
+        return errs
+
+    def solve(self, eps=1e-8, max_iter=100000, ncheck=None, callback=None):
+        """Iterate until all scaled residuals < eps (stokes_3d_inc_ve_T.jl:221-238).
+        Raises on non-finite residuals."""
+        ncheck = ncheck or 2 * self.grid.dims[0]
+        err = None
+        for it in range(1, max_iter + 1):
+            check = it % ncheck == 0
+            self.iteration(write_res=check)
+            if check:
+                err = self.residual()
+                if callback:
+                    callback(it, err)
+                if not all(math.isfinite(e) for e in err):
+                    raise FloatingPointError(f"simulation failed, err = {err}")
+                if all(e < eps for e in err):
+                    return it, err
+        return max_iter, err
+
+
+def inclusion_setup_3d(grid, *, eta0=1.0, eta_incl=1e-3, radius=0.1, T_amp=0.1, T_width=0.05):
+    """Config: 3D VE Stokes+T with a spherical weak inclusion.  η on the full
+    padded storage (logical -1..n+2) from the fma cell-centre coordinates:
+    η = (|x| < r ? η_incl : η0)·exp(-T), T = T_amp·exp(-|x|²/T_width²)."""
+    xs = [grid.coords(Center(), d, first=-1, count=grid.dims[d] + 4) for d in range(3)]
+    X = xs[0][None, None, :]
+    Y = xs[1][None, :, None]
+    Z = xs[2][:, None, None]
+    r2 = X ** 2 + Y ** 2 + Z ** 2
+    T = T_amp * np.exp(-r2 / T_width ** 2)
+    eta = np.where(r2 < radius ** 2, eta_incl, eta0) * np.exp(-T)
+    return eta
+
+
+def pure_shear_values(nd):
+    """Normal boundary velocities ±1: x compressed (Vx = +1 | -1), last dim extended."""
+    vals = {D: (0.0, 0.0) for D in range(nd)}
+    vals[0] = (1.0, -1.0)
+    vals[nd - 1] = (-1.0, 1.0)
+    return vals
+
+
+def pure_shear_initial(grid, V):
+    """V_x = -2x, V_last = +2·x_last on every stored point (linear background)."""
+    nd = grid.nd
+    for D in (0, nd - 1):
+        f = V[D]
+        xs = grid.coords(f.loc, D, first=-1, count=f.dims[D] + 4)
+        rate = -2.0 if D == 0 else 2.0
+        shape = [1] * nd
+        shape[nd - 1 - D] = -1
+        f.data.copy_(torch.as_tensor(np.broadcast_to((rate * xs).reshape(shape), f.data.shape).copy()))

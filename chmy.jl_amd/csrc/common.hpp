@@ -1,0 +1,129 @@
+// Shared device/host helpers of the gfx950 stencil library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/chmy_mi355x.h"
+
+namespace chmy {
+
+// ------------------------------------------------------------------ errors
+int set_error(int code, const char* fmt, ...);
+int hip_status(hipError_t e, const char* what);
+
+#define CHMY_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_status(e_, #call); \
+    } while (0)
+#define CHMY_TRY(expr)               \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_ != 0) return rc_;    \
+    } while (0)
+
+// ------------------------------------------------------------------ views
+// Device view of a Field: p points at logical index (0,0,0), so logical I is
+// p[i + j*sy + k*sz] (storage offset I + 2H - 1 per dim, field.jl:18).
+template <class T>
+struct DF {
+    T* p;
+    int64_t sy, sz;
+    __device__ __forceinline__ T& operator()(int64_t i, int64_t j, int64_t k) const {
+        return p[i + j * sy + k * sz];
+    }
+    __device__ __forceinline__ T& at(int64_t o) const { return p[o]; }
+};
+
+// Same, carrying locations and dimensionality for the generic operators.
+template <class T>
+struct DFL {
+    T* p;
+    int64_t st[3];
+    int32_t loc[3];
+    int32_t nd;
+    __device__ __forceinline__ T& operator()(int64_t i, int64_t j, int64_t k) const {
+        return p[i * st[0] + j * st[1] + k * st[2]];
+    }
+};
+
+inline int64_t pad_of(const chmy_field_t& f, int d) { return f.n[d] + 4 * (int64_t)f.halo[d]; }
+
+template <class T>
+DF<T> view(const chmy_field_t& f) {
+    DF<T> v;
+    int64_t s1 = pad_of(f, 0);
+    int64_t s2 = f.nd >= 2 ? s1 * pad_of(f, 1) : 0;
+    v.sy = f.nd >= 2 ? s1 : 0;
+    v.sz = f.nd >= 3 ? s2 : 0;
+    int64_t off = (2 * (int64_t)f.halo[0] - 1);
+    if (f.nd >= 2) off += (2 * (int64_t)f.halo[1] - 1) * v.sy;
+    if (f.nd >= 3) off += (2 * (int64_t)f.halo[2] - 1) * v.sz;
+    v.p = static_cast<T*>(f.data) + off;
+    return v;
+}
+
+template <class T>
+DFL<T> lview(const chmy_field_t& f) {
+    DFL<T> v;
+    DF<T> b = view<T>(f);
+    v.p = b.p;
+    v.st[0] = 1;
+    v.st[1] = b.sy;
+    v.st[2] = b.sz;
+    for (int d = 0; d < 3; ++d) v.loc[d] = d < f.nd ? f.loc[d] : 0;
+    v.nd = f.nd;
+    return v;
+}
+
+// ------------------------------------------------------------------ validation
+// All descriptors are checked on the host before any launch, so that every
+// access a kernel makes (box ± 1 in each dim) lies inside the field storage.
+int check_grid(const chmy_grid_t* g);
+int check_box(const chmy_grid_t* g, const chmy_box_t* b, chmy_box_t* out);
+// loc: expected location per dim (-1 = any); dtype: expected (-1 = any).
+int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, const int* loc, int dtype);
+int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, int dtype);
+int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype);
+int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype);
+void tensor_loc(int nd, int c, int* loc);
+
+struct Box3 {
+    int64_t lo[3];
+    int64_t len[3];
+};
+inline Box3 box3(const chmy_box_t& b, int nd) {
+    Box3 r;
+    for (int d = 0; d < 3; ++d) {
+        r.lo[d] = d < nd ? b.lo[d] : 0;
+        r.len[d] = d < nd ? b.len[d] : 1;
+    }
+    return r;
+}
+
+// Block of 256 threads as 64 (x, one wave) x 4 (y); grid covers the box.
+inline dim3 grid_for(const Box3& b, dim3 blk) {
+    return dim3((unsigned)((b.len[0] + blk.x - 1) / blk.x), (unsigned)((b.len[1] + blk.y - 1) / blk.y),
+                (unsigned)((b.len[2] + blk.z - 1) / blk.z));
+}
+
+// Julia max/min against zero for floats (NaN propagates, max(-0.0, 0.0) = 0.0).
+template <class T>
+__device__ __forceinline__ T jmax0(T x) { return x != x ? x : (x > T(0) ? x : T(0)); }
+template <class T>
+__device__ __forceinline__ T jmin0(T x) { return x != x ? x : (x <= T(0) ? x : T(0)); }
+template <class T>
+__device__ __forceinline__ T jmax(T a, T b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return a > b ? a : b;
+}
+template <class T>
+__device__ __forceinline__ T visc_ve(T eta, T G, T dt) { return T(1) / (T(1) / eta + T(1) / (G * dt)); }
+
+}  // namespace chmy

@@ -1,0 +1,378 @@
+// Per-point Launcher kernels: generic GridOperators, diffusion, update_old!,
+// thermal flux/update and the viscosity max-neighbourhood.  Each thread owns
+// one logical index I of the box; blocks are one wave64 wide along x (the
+// contiguous dim) so loads and stores coalesce.  These are bandwidth-bound
+// streaming kernels; the fused Stokes iteration lives in stokes.hip.
+#include "common.hpp"
+
+namespace chmy {
+
+constexpr int kBX = 64, kBY = 4;
+
+struct Ix3 {
+    int64_t i, j, k;
+};
+
+__device__ __forceinline__ Ix3 box_index(const Box3& b, bool& ok) {
+    int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t c = blockIdx.y * (int64_t)blockDim.y + threadIdx.y;
+    int64_t e = blockIdx.z * (int64_t)blockDim.z + threadIdx.z;
+    ok = a < b.len[0] && c < b.len[1] && e < b.len[2];
+    return Ix3{b.lo[0] + a, b.lo[1] + c, b.lo[2] + e};
+}
+
+template <class T>
+__device__ __forceinline__ T& g_at(const DFL<T>& f, Ix3 I) { return f(I.i, I.j, I.k); }
+__device__ __forceinline__ Ix3 sh(Ix3 I, int d, int64_t s) {
+    if (d == 0) I.i += s; else if (d == 1) I.j += s; else I.k += s;
+    return I;
+}
+// il/ir of an operator on a field at its own location (GridOperators.jl:23-36)
+template <class T>
+__device__ __forceinline__ T leftv(const DFL<T>& f, int D, Ix3 I) { return f.loc[D] == 0 ? g_at(f, sh(I, D, -1)) : g_at(f, I); }
+template <class T>
+__device__ __forceinline__ T rightv(const DFL<T>& f, int D, Ix3 I) { return f.loc[D] == 0 ? g_at(f, I) : g_at(f, sh(I, D, 1)); }
+template <class T>
+__device__ __forceinline__ T ddv(const DFL<T>& f, T isp, int D, Ix3 I) { return (rightv(f, D, I) - leftv(f, D, I)) * isp; }
+
+// lerp (interpolation.jl:14-94): knots over the differing dims, reduced with
+// fma(0.5, b-a, a) from the first differing dim (innermost) to the last.
+template <class T>
+__device__ T lerpv(const DFL<T>& f, const int32_t* to, Ix3 I) {
+    int dims[3], m = 0;
+    for (int d = 0; d < f.nd; ++d)
+        if (f.loc[d] != to[d]) dims[m++] = d;
+    T v[8];
+    for (int q = 0; q < (1 << m); ++q) {
+        Ix3 J = I;
+        for (int b = 0; b < m; ++b) {
+            int D = dims[b];
+            bool right = (q >> b) & 1;
+            // Center->Vertex knots (I-e, I); Vertex->Center knots (I, I+e)
+            int64_t s = f.loc[D] == 0 ? (right ? 0 : -1) : (right ? 1 : 0);
+            J = sh(J, D, s);
+        }
+        v[q] = g_at(f, J);
+    }
+    for (int b = 0; b < m; ++b) {
+        int half = 1 << (m - b - 1);
+        for (int q = 0; q < half; ++q) {
+            T a = v[2 * q], c = v[2 * q + 1];
+            v[q] = fma(T(0.5), c - a, a);
+        }
+    }
+    return v[0];
+}
+
+struct OpArgs {
+    int op, nd, D, nsrc;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) op_kernel(Box3 b, DFL<T> dst, DFL<T> s0, DFL<T> s1, DFL<T> s2, OpArgs a,
+                                                 T isp0, T isp1, T isp2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const DFL<T>* s[3] = {&s0, &s1, &s2};
+    const T isp[3] = {isp0, isp1, isp2};
+    T v = T(0);
+    switch (a.op) {
+        case CHMY_OP_PARTIAL: v = ddv(s0, isp[a.D], a.D, I); break;
+        case CHMY_OP_LERP: v = lerpv(s0, dst.loc, I); break;
+        case CHMY_OP_DIVG: {
+            v = ddv(*s[0], isp[0], 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + ddv(*s[d], isp[d], d, I);
+        } break;
+        case CHMY_OP_VMAG: {
+            const int32_t c[3] = {0, 0, 0};
+            T x = lerpv(*s[0], c, I);
+            T acc = x * x;
+            for (int d = 1; d < a.nd; ++d) { T y = lerpv(*s[d], c, I); acc = acc + y * y; }
+            v = sqrt(acc);
+        } break;
+        case CHMY_OP_LEFT: v = leftv(s0, a.D, I); break;
+        case CHMY_OP_RIGHT: v = rightv(s0, a.D, I); break;
+    }
+    g_at(dst, I) = v;
+}
+
+// ------------------------------------------------------------------ diffusion
+template <class T>
+__global__ void __launch_bounds__(256) diffusion_flux_kernel(Box3 b, DFL<T> C, DFL<T> q0, DFL<T> q1, DFL<T> q2,
+                                                             int nd, T nchi, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    // q.x[I] = -χ * ∂x(C)[I] (diffusion_2d.jl:11-12): Center C -> (C[I] - C[I-e]) iΔ
+    g_at(q0, I) = nchi * ((g_at(C, I) - g_at(C, sh(I, 0, -1))) * i0);
+    if (nd > 1) g_at(q1, I) = nchi * ((g_at(C, I) - g_at(C, sh(I, 1, -1))) * i1);
+    if (nd > 2) g_at(q2, I) = nchi * ((g_at(C, I) - g_at(C, sh(I, 2, -1))) * i2);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) diffusion_update_kernel(Box3 b, DFL<T> C, DFL<T> q0, DFL<T> q1, DFL<T> q2,
+                                                               int nd, T dt, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    // C[I] -= Δt * divg(q)[I] (diffusion_2d.jl:18); Vertex q_D -> (q[I+e] - q[I]) iΔ
+    T dv = (g_at(q0, sh(I, 0, 1)) - g_at(q0, I)) * i0;
+    if (nd > 1) dv = dv + (g_at(q1, sh(I, 1, 1)) - g_at(q1, I)) * i1;
+    if (nd > 2) dv = dv + (g_at(q2, sh(I, 2, 1)) - g_at(q2, I)) * i2;
+    g_at(C, I) = g_at(C, I) - dt * dv;
+}
+
+// ------------------------------------------------------------------ update_old!
+template <class T>
+struct Copy7 {
+    DFL<T> src[7], dst[7];
+    int n;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) copy_kernel(Box3 b, Copy7<T> c) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    for (int q = 0; q < c.n; ++q) g_at(c.dst[q], I) = g_at(c.src[q], I);
+}
+
+// ------------------------------------------------------------------ thermal
+template <class T>
+__global__ void __launch_bounds__(256) thermal_flux_kernel(Box3 b, DFL<T> Tf, DFL<T> V0, DFL<T> V1, DFL<T> V2,
+                                                           DFL<T> q0, DFL<T> q1, DFL<T> q2, int nd, T nlam, T i0,
+                                                           T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const DFL<T>* V[3] = {&V0, &V1, &V2};
+    const DFL<T>* q[3] = {&q0, &q1, &q2};
+    const T isp[3] = {i0, i1, i2};
+    T tc = g_at(Tf, I);
+    for (int d = 0; d < nd; ++d) {
+        // qT_D = ((-λ ∂_D T) + max(V_D,0) left_D T) + min(V_D,0) right_D T
+        // (stokes_3d_inc_ve_T.jl:120-128); T Center: left = T[I-e], right = T[I]
+        T tl = g_at(Tf, sh(I, d, -1));
+        T v = g_at(*V[d], I);
+        g_at(*q[d], I) = (nlam * ((tc - tl) * isp[d]) + jmax0(v) * tl) + jmin0(v) * tc;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) thermal_update_kernel(Box3 b, DFL<T> Tf, DFL<T> To, DFL<T> q0, DFL<T> q1,
+                                                             DFL<T> q2, int nd, T dt, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    T dv = (g_at(q0, sh(I, 0, 1)) - g_at(q0, I)) * i0;
+    if (nd > 1) dv = dv + (g_at(q1, sh(I, 1, 1)) - g_at(q1, I)) * i1;
+    if (nd > 2) dv = dv + (g_at(q2, sh(I, 2, 1)) - g_at(q2, I)) * i2;
+    g_at(Tf, I) = g_at(To, I) - dt * dv;
+}
+
+// ------------------------------------------------------------------ maxloc
+template <class T>
+__global__ void __launch_bounds__(256) maxloc_kernel(Box3 b, DFL<T> dst, DFL<T> src, int nd) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    T m = g_at(src, I);
+    const int64_t rz = nd > 2 ? 1 : 0, ry = nd > 1 ? 1 : 0;
+    for (int64_t c = -rz; c <= rz; ++c)
+        for (int64_t bb = -ry; bb <= ry; ++bb)
+            for (int64_t a = -1; a <= 1; ++a) m = jmax(m, g_at(src, Ix3{I.i + a, I.j + bb, I.k + c}));
+    g_at(dst, I) = m;
+}
+
+// ------------------------------------------------------------------ host side
+template <class T>
+DFL<T> lv_or(const chmy_field_t* f, const DFL<T>& fallback) {
+    return f ? lview<T>(*f) : fallback;
+}
+
+static dim3 blk2() { return dim3(kBX, kBY, 1); }
+
+template <class T>
+static int diffusion_launch(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* q,
+                            const chmy_field_t* Cf, double scalar, bool flux, hipStream_t s) {
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    Box3 b = box3(bx, g->nd);
+    DFL<T> c = lview<T>(*Cf);
+    DFL<T> q0 = lview<T>(q[0]);
+    DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
+    dim3 grd = grid_for(b, blk2());
+    if (flux)
+        hipLaunchKernelGGL(diffusion_flux_kernel<T>, grd, blk2(), 0, s, b, c, q0, q1, q2, g->nd, -T(scalar),
+                           T(g->inv_spacing[0]), T(g->inv_spacing[1]), T(g->inv_spacing[2]));
+    else
+        hipLaunchKernelGGL(diffusion_update_kernel<T>, grd, blk2(), 0, s, b, c, q0, q1, q2, g->nd, T(scalar),
+                           T(g->inv_spacing[0]), T(g->inv_spacing[1]), T(g->inv_spacing[2]));
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class T>
+static int copy_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_field_t* Tf, const chmy_field_t* tau,
+                       const chmy_field_t* To, const chmy_field_t* tauo, hipStream_t s) {
+    Copy7<T> c;
+    int nt = g->nd == 2 ? 3 : 6;
+    c.n = 0;
+    if (Tf) { c.src[c.n] = lview<T>(*Tf); c.dst[c.n] = lview<T>(*To); ++c.n; }
+    for (int q = 0; q < nt; ++q) { c.src[c.n] = lview<T>(tau[q]); c.dst[c.n] = lview<T>(tauo[q]); ++c.n; }
+    Box3 b = box3(bx, g->nd);
+    hipLaunchKernelGGL(copy_kernel<T>, grid_for(b, blk2()), blk2(), 0, s, b, c);
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class T>
+static int thermal_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_field_t* q, const chmy_field_t* Tf,
+                          const chmy_field_t* other, double scalar, bool flux, hipStream_t s) {
+    Box3 b = box3(bx, g->nd);
+    DFL<T> t = lview<T>(*Tf), o = lview<T>(other[0]);
+    DFL<T> q0 = lview<T>(q[0]);
+    DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
+    dim3 grd = grid_for(b, blk2());
+    if (flux) {
+        DFL<T> v1 = g->nd > 1 ? lview<T>(other[1]) : o, v2 = g->nd > 2 ? lview<T>(other[2]) : o;
+        hipLaunchKernelGGL(thermal_flux_kernel<T>, grd, blk2(), 0, s, b, t, o, v1, v2, q0, q1, q2, g->nd, -T(scalar),
+                           T(g->inv_spacing[0]), T(g->inv_spacing[1]), T(g->inv_spacing[2]));
+    } else {
+        hipLaunchKernelGGL(thermal_update_kernel<T>, grd, blk2(), 0, s, b, t, o, q0, q1, q2, g->nd, T(scalar),
+                           T(g->inv_spacing[0]), T(g->inv_spacing[1]), T(g->inv_spacing[2]));
+    }
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace chmy
+
+using namespace chmy;
+
+extern "C" {
+
+int chmy_op(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, int nsrc,
+            const chmy_field_t* src, int dim, void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_field(g, dst, "dst", nullptr, -1));
+    const int dt = dst->dtype;
+    const bool vec = op == CHMY_OP_DIVG || op == CHMY_OP_VMAG;
+    if (vec && nsrc != g->nd) return set_error(CHMY_E_ARG, "divg/vmag need nd source fields");
+    if (!vec && nsrc != 1) return set_error(CHMY_E_ARG, "operator needs one source field");
+    if (!src) return set_error(CHMY_E_ARG, "src is NULL");
+    if (op != CHMY_OP_PARTIAL && op != CHMY_OP_LERP && op != CHMY_OP_DIVG && op != CHMY_OP_VMAG &&
+        op != CHMY_OP_LEFT && op != CHMY_OP_RIGHT)
+        return set_error(CHMY_E_ARG, "unknown operator %d", op);
+    if ((op == CHMY_OP_PARTIAL || op == CHMY_OP_LEFT || op == CHMY_OP_RIGHT) && (dim < 0 || dim >= g->nd))
+        return set_error(CHMY_E_ARG, "dim %d out of range", dim);
+    for (int q = 0; q < nsrc; ++q) {
+        const int* want = nullptr;
+        int loc[3];
+        if (op == CHMY_OP_DIVG || op == CHMY_OP_VMAG) {
+            for (int d = 0; d < 3; ++d) loc[d] = d == q ? 1 : 0;
+            want = loc;
+        }
+        CHMY_TRY(check_field(g, &src[q], "src", want, dt));
+    }
+    Box3 b = box3(bx, g->nd);
+    OpArgs a{op, g->nd, dim, nsrc};
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grd = grid_for(b, blk2());
+    if (dt == CHMY_F64) {
+        DFL<double> d = lview<double>(*dst), s0 = lview<double>(src[0]);
+        DFL<double> s1 = nsrc > 1 ? lview<double>(src[1]) : s0, s2 = nsrc > 2 ? lview<double>(src[2]) : s0;
+        hipLaunchKernelGGL(op_kernel<double>, grd, blk2(), 0, s, b, d, s0, s1, s2, a, g->inv_spacing[0],
+                           g->inv_spacing[1], g->inv_spacing[2]);
+    } else {
+        DFL<float> d = lview<float>(*dst), s0 = lview<float>(src[0]);
+        DFL<float> s1 = nsrc > 1 ? lview<float>(src[1]) : s0, s2 = nsrc > 2 ? lview<float>(src[2]) : s0;
+        hipLaunchKernelGGL(op_kernel<float>, grd, blk2(), 0, s, b, d, s0, s1, s2, a, (float)g->inv_spacing[0],
+                           (float)g->inv_spacing[1], (float)g->inv_spacing[2]);
+    }
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+int chmy_diffusion_flux(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* q, const chmy_field_t* C,
+                        double chi, void* stream) {
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_center(g, C, "C", -1));
+    CHMY_TRY(check_vector(g, q, "q", C->dtype));
+    return C->dtype == CHMY_F64 ? diffusion_launch<double>(g, box, q, C, chi, true, (hipStream_t)stream)
+                                : diffusion_launch<float>(g, box, q, C, chi, true, (hipStream_t)stream);
+}
+
+int chmy_diffusion_update(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* C,
+                          const chmy_field_t* q, double dt, void* stream) {
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_center(g, C, "C", -1));
+    CHMY_TRY(check_vector(g, q, "q", C->dtype));
+    return C->dtype == CHMY_F64 ? diffusion_launch<double>(g, box, q, C, dt, false, (hipStream_t)stream)
+                                : diffusion_launch<float>(g, box, q, C, dt, false, (hipStream_t)stream);
+}
+
+int chmy_stokes_update_old(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* Tf,
+                           const chmy_field_t* tau, const chmy_field_t* To, const chmy_field_t* tauo, void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_tensor(g, tau, "tau", -1));
+    const int dt = tau[0].dtype;
+    CHMY_TRY(check_tensor(g, tauo, "tau_old", dt));
+    if ((Tf == nullptr) != (To == nullptr)) return set_error(CHMY_E_ARG, "T and T_old must both be given or NULL");
+    if (Tf) {
+        CHMY_TRY(check_center(g, Tf, "T", dt));
+        CHMY_TRY(check_center(g, To, "T_old", dt));
+    }
+    return dt == CHMY_F64 ? copy_launch<double>(g, bx, Tf, tau, To, tauo, (hipStream_t)stream)
+                          : copy_launch<float>(g, bx, Tf, tau, To, tauo, (hipStream_t)stream);
+}
+
+int chmy_thermal_flux(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* qT, const chmy_field_t* Tf,
+                      const chmy_field_t* V, double lam, void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_center(g, Tf, "T", -1));
+    CHMY_TRY(check_vector(g, qT, "qT", Tf->dtype));
+    CHMY_TRY(check_vector(g, V, "V", Tf->dtype));
+    return Tf->dtype == CHMY_F64 ? thermal_launch<double>(g, bx, qT, Tf, V, lam, true, (hipStream_t)stream)
+                                 : thermal_launch<float>(g, bx, qT, Tf, V, lam, true, (hipStream_t)stream);
+}
+
+int chmy_thermal_update(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* Tf,
+                        const chmy_field_t* To, const chmy_field_t* qT, double dt, void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_center(g, Tf, "T", -1));
+    CHMY_TRY(check_center(g, To, "T_old", Tf->dtype));
+    CHMY_TRY(check_vector(g, qT, "qT", Tf->dtype));
+    return Tf->dtype == CHMY_F64 ? thermal_launch<double>(g, bx, qT, Tf, To, dt, false, (hipStream_t)stream)
+                                 : thermal_launch<float>(g, bx, qT, Tf, To, dt, false, (hipStream_t)stream);
+}
+
+int chmy_maxloc(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, const chmy_field_t* src,
+                void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_center(g, src, "src", -1));
+    CHMY_TRY(check_center(g, dst, "dst", src->dtype));
+    Box3 b = box3(bx, g->nd);
+    hipStream_t s = (hipStream_t)stream;
+    if (src->dtype == CHMY_F64)
+        hipLaunchKernelGGL(maxloc_kernel<double>, grid_for(b, blk2()), blk2(), 0, s, b, lview<double>(*dst),
+                           lview<double>(*src), g->nd);
+    else
+        hipLaunchKernelGGL(maxloc_kernel<float>, grid_for(b, blk2()), blk2(), 0, s, b, lview<float>(*dst),
+                           lview<float>(*src), g->nd);
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+// Residual-norm reductions on the GPU: the reference's host-side
+// maximum(abs.(interior(X))) (examples/stokes_3d_inc_ve_T.jl:230-233) and a
+// deterministic f64 sum of squares.  max|x| is exact and order-independent:
+// blocks reduce in registers (wave64 shuffles) and LDS, then combine with one
+// 64-bit atomic max per block on the bit pattern (non-negative doubles order
+// like their unsigned bits; NaN's bits exceed +inf's, so NaN propagates).
+#include <mutex>
+
+#include "common.hpp"
+
+namespace chmy {
+
+constexpr int kRB = 256;           // threads per reduction block
+constexpr int kSumBlocks = 1024;   // fixed partial count: deterministic sum order
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void zero_u64(unsigned long long* p) {
+    if (threadIdx.x == 0) p[0] = 0ull;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned long long* out) {
+    __shared__ double part[kRB / 64];
+    __shared__ int nanflag;
+    if (threadIdx.x == 0) nanflag = 0;
+    __syncthreads();
+    const int64_t nx = b.len[0], ny = b.len[1];
+    const int64_t tot = nx * ny * b.len[2];
+    double m = 0.0;
+    int isnan_ = 0;
+    for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
+        int64_t i = t % nx, r = t / nx;
+        int64_t j = r % ny, k = r / ny;
+        double a = fabs((double)f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k));
+        if (a != a) isnan_ = 1;
+        else m = fmax(m, a);
+    }
+    m = wave_max(m);
+    if (isnan_) nanflag = 1;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < kRB / 64; ++w) v = fmax(v, part[w]);
+        if (nanflag) v = __longlong_as_double(0x7ff8000000000000ll);
+        atomicMax(out, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRB) sumsq_kernel(Box3 b, DF<T> f, double* partial) {
+    __shared__ double part[kRB / 64];
+    const int64_t nx = b.len[0], ny = b.len[1];
+    const int64_t tot = nx * ny * b.len[2];
+    double s = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
+        int64_t i = t % nx, r = t / nx;
+        int64_t j = r % ny, k = r / ny;
+        double a = (double)f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k);
+        s += a * a;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < kRB / 64; ++w) v += part[w];
+        partial[blockIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kRB) sum_partials(const double* partial, int n, double* out) {
+    __shared__ double part[kRB / 64];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += kRB) s += partial[q];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = part[0];
+        for (int w = 1; w < kRB / 64; ++w) v += part[w];
+        out[0] = v;
+    }
+}
+
+// Library-held device scratch for reductions: one buffer per device, grown
+// on demand, kept until process exit.
+static std::mutex g_scratch_mu;
+static void* g_scratch[64] = {nullptr};
+static size_t g_scratch_sz[64] = {0};
+
+static int scratch(size_t bytes, void** p) {
+    int dev = 0;
+    CHMY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(CHMY_E_ARG, "device index %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_sz[dev] < bytes) {
+        if (g_scratch[dev]) CHMY_HIP(hipFree(g_scratch[dev]));
+        g_scratch[dev] = nullptr;
+        g_scratch_sz[dev] = 0;
+        CHMY_HIP(hipMalloc(&g_scratch[dev], bytes));
+        g_scratch_sz[dev] = bytes;
+    }
+    *p = g_scratch[dev];
+    return 0;
+}
+
+static int reduce_box(const chmy_field_t* f, const chmy_box_t* box, Box3* out) {
+    if (!f || !f->data) return set_error(CHMY_E_ARG, "field is NULL");
+    if (f->nd < 1 || f->nd > 3) return set_error(CHMY_E_ARG, "field nd %d", f->nd);
+    chmy_box_t b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = 1; b.len[d] = d < f->nd ? f->n[d] : 1; }
+    if (box) b = *box;
+    for (int d = 0; d < f->nd; ++d) {
+        const int64_t H = f->halo[d];
+        if (b.len[d] < 0 || b.lo[d] < 1 - 2 * H || b.lo[d] + b.len[d] - 1 > f->n[d] + 2 * H)
+            return set_error(CHMY_E_BOX, "reduction box outside the storage of the field along dim %d", d);
+    }
+    *out = box3(b, f->nd);
+    return 0;
+}
+
+static unsigned grid_blocks(const Box3& b) {
+    int64_t tot = b.len[0] * b.len[1] * b.len[2];
+    int64_t nb = (tot + kRB - 1) / kRB;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
+}  // namespace chmy
+
+using namespace chmy;
+
+extern "C" {
+
+int chmy_maxabs_dev(const chmy_field_t* f, const chmy_box_t* box, double* out_dev, void* stream) {
+    Box3 b;
+    CHMY_TRY(reduce_box(f, box, &b));
+    if (!out_dev) return set_error(CHMY_E_ARG, "out_dev is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* o = reinterpret_cast<unsigned long long*>(out_dev);
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, o);
+    if (f->dtype == CHMY_F64)
+        hipLaunchKernelGGL(maxabs_kernel<double>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<double>(*f), o);
+    else
+        hipLaunchKernelGGL(maxabs_kernel<float>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<float>(*f), o);
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+int chmy_maxabs(const chmy_field_t* f, const chmy_box_t* box, double* out_host, void* stream) {
+    if (!out_host) return set_error(CHMY_E_ARG, "out_host is NULL");
+    void* p = nullptr;
+    CHMY_TRY(scratch(kSumBlocks * sizeof(double) + 64, &p));
+    double* d = static_cast<double*>(p) + kSumBlocks;
+    CHMY_TRY(chmy_maxabs_dev(f, box, d, stream));
+    CHMY_HIP(hipMemcpyAsync(out_host, d, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    CHMY_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int chmy_sumsq(const chmy_field_t* f, const chmy_box_t* box, double* out_host, void* stream) {
+    Box3 b;
+    CHMY_TRY(reduce_box(f, box, &b));
+    if (!out_host) return set_error(CHMY_E_ARG, "out_host is NULL");
+    void* p = nullptr;
+    CHMY_TRY(scratch(kSumBlocks * sizeof(double) + 64, &p));
+    double* partial = static_cast<double*>(p);
+    double* res = partial + kSumBlocks;
+    hipStream_t s = (hipStream_t)stream;
+    if (f->dtype == CHMY_F64)
+        hipLaunchKernelGGL(sumsq_kernel<double>, dim3(kSumBlocks), dim3(kRB), 0, s, b, view<double>(*f), partial);
+    else
+        hipLaunchKernelGGL(sumsq_kernel<float>, dim3(kSumBlocks), dim3(kRB), 0, s, b, view<float>(*f), partial);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kRB), 0, s, partial, kSumBlocks, res);
+    CHMY_HIP(hipGetLastError());
+    CHMY_HIP(hipMemcpyAsync(out_host, res, sizeof(double), hipMemcpyDeviceToHost, s));
+    CHMY_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"

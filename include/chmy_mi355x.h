@@ -1,0 +1,200 @@
+/*
+ * chmy_mi355x.h — C ABI of the MI355X (gfx950) staggered-grid stencil library.
+ *
+ * Drop-in boundary for Chmy.jl's hot path: the Launcher-launched kernels that
+ * apply GridOperators (∂x/∂y/∂z, lerp, divg) to Fields in the explicit
+ * diffusion and pseudo-transient (PT) viscoelastic Stokes + temperature
+ * updates, and the bc! boundary batches.  Every entry point takes plain
+ * pointers, POD descriptors and a hipStream_t passed as void*.
+ *
+ * Conventions
+ *  - Layout: Chmy's Field storage (src/Fields/field.jl:6-18,56-62): data has
+ *    size n .+ 4H per dim, x fastest; logical index I (1-based) lives at
+ *    data[I .+ 2H].  chmy_field_t.n is the logical size (size(grid, loc)).
+ *  - Iteration box: chmy_box_t gives logical start and length per dim.  The
+ *    Launcher's box is lo = 0, len = ncells + 2 (src/KernelLaunch.jl:41,
+ *    105-126: worksize n .+ 2 at Offset(-1)).  Boxes must lie in
+ *    [0, ncells+1]; halos must be >= 1 so every ±1 neighbour is in storage.
+ *  - Ownership: the caller owns all device memory; descriptors are read
+ *    during the call only.
+ *  - Status: 0 ok; > 0 a hipError_t; < 0 misuse (CHMY_E_*).  The message of
+ *    the last failure on the calling thread is chmy_last_error().
+ *  - Asynchrony: kernels are enqueued on `stream` and return without
+ *    synchronizing, except the reductions returning host scalars.
+ *  - Arithmetic: IEEE f64/f32 with no contraction; fma only where Chmy
+ *    writes muladd (lerp, first-order BCs), so f64 results match Chmy's CPU
+ *    semantics to rounding of the same expressions.
+ */
+#ifndef CHMY_MI355X_H
+#define CHMY_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { CHMY_F64 = 0, CHMY_F32 = 1 };
+enum { CHMY_CENTER = 0, CHMY_VERTEX = 1 };
+enum { CHMY_DIRICHLET = 0, CHMY_NEUMANN = 1 };
+enum { CHMY_VALUE_ZERO = 0, CHMY_VALUE_SCALAR = 1, CHMY_VALUE_FIELD = 2 };
+enum {
+  CHMY_OK = 0,
+  CHMY_E_ARG = -1,      /* null / malformed argument */
+  CHMY_E_DTYPE = -2,    /* dtype mismatch between fields */
+  CHMY_E_LOC = -3,      /* field location or size does not match the grid */
+  CHMY_E_BOX = -4,      /* iteration box outside [0, ncells+1] */
+  CHMY_E_HALO = -5,     /* halo too small for the stencil */
+  CHMY_E_NODEV = -6,    /* no HIP device */
+  CHMY_E_SIZE = -7      /* size above a stated maximum */
+};
+
+/* Field{T,N,L,H} descriptor (src/Fields/field.jl:6-9). */
+typedef struct {
+  void*   data;      /* device pointer to the padded storage */
+  int32_t nd;        /* 1, 2 or 3 */
+  int32_t dtype;     /* CHMY_F64 / CHMY_F32 */
+  int64_t n[3];      /* logical size per dim = size(grid, loc) */
+  int32_t halo[3];   /* H per dim */
+  int32_t loc[3];    /* CHMY_CENTER / CHMY_VERTEX per dim */
+} chmy_field_t;
+
+/* UniformGrid (src/Grids/structured_grid.jl:27-39, uniform_axis.jl:7-12):
+ * spacing = extent/n, inv_spacing = 1/spacing, as the host computed them. */
+typedef struct {
+  int32_t nd;
+  int32_t reserved;
+  int64_t n[3];            /* cells per dim */
+  double  origin[3];
+  double  extent[3];
+  double  spacing[3];
+  double  inv_spacing[3];
+} chmy_grid_t;
+
+typedef struct { int64_t lo[3]; int64_t len[3]; } chmy_box_t;
+
+/* FirstOrderBC{T,Kind} (first_order_boundary_condition.jl:9-40): value is
+ * nothing (0), a number, or a field indexed by the tangential indices. */
+typedef struct {
+  int32_t kind;                 /* CHMY_DIRICHLET / CHMY_NEUMANN */
+  int32_t vkind;                /* CHMY_VALUE_* */
+  double  value;
+  const chmy_field_t* vfield;   /* device field, vkind == CHMY_VALUE_FIELD */
+} chmy_bc_t;
+
+/* ---------------------------------------------------------------- runtime */
+/* Thread-local message of the last failing call ("" if none). */
+const char* chmy_last_error(void);
+/* Architectures.jl:46-49 Arch(backend; device_id) / set_device!: make `device`
+ * current on this thread (0-based).  Also used by get_device. */
+int chmy_init(int device);
+int chmy_device_count(int* count);
+/* KernelAbstractions.zeros(backend, T, dims) (field.jl:59): zero-filled. */
+int chmy_malloc_zeros(void** ptr, size_t bytes, void* stream);
+int chmy_free(void* ptr);
+int chmy_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int chmy_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+/* KernelAbstractions.synchronize(backend) (src/KernelLaunch.jl:117). */
+int chmy_stream_sync(void* stream);
+
+/* UniformAxis coord (uniform_axis.jl:18-19): out[q] = coord(ax, loc, first+q),
+ * vertex(i) = fma(i-1, Δ, o), center(i) = fma(i-1, Δ, fma(0.5, Δ, o)).  Host. */
+int chmy_coords(const chmy_grid_t* grid, int dim, int loc, int64_t first, int64_t count, double* out);
+
+/* ---------------------------------------------------------------- bc! */
+/* bc_kernel! for one (dim, side) of a FieldBatch of K fields
+ * (src/BoundaryConditions/batch.jl:163-184): tangential indices 0..n_t+2,
+ * fields applied in batch order.  dim is 0-based, side 1 or 2. */
+int chmy_bc_side(const chmy_grid_t* grid, int dim, int side, int K,
+                 const chmy_field_t* fields, const chmy_bc_t* bcs, void* stream);
+/* bc!(arch, grid, batch::BatchSet) (batch.jl:20-29): dims N..1, side 1 then
+ * side 2.  counts[2*dim+side-1] fields per batch, fields/bcs concatenated in
+ * (dim, side) order.  One launch per dim (both sides are independent). */
+int chmy_bc_batchset(const chmy_grid_t* grid, const int32_t* counts,
+                     const chmy_field_t* fields, const chmy_bc_t* bcs, void* stream);
+
+/* ---------------------------------------------------------------- operators
+ * Generic single-output operator over a box, the GPU form of Chmy's
+ * GridOperators as used by the reference's operator tests
+ * (test/test_grid_operators.jl, test_interpolations.jl):
+ *   CHMY_OP_PARTIAL  dst[I] = ∂_dim src[I]           (partial_derivatives.jl:2-5)
+ *   CHMY_OP_LERP     dst[I] = lerp(src, loc(dst))[I]  (interpolation.jl:14-94)
+ *   CHMY_OP_DIVG     dst[I] = divg(src[0..nd-1])[I]  (field_operators.jl:50-59)
+ *   CHMY_OP_VMAG     dst[I] = vmag(src[0..nd-1])[I]  (field_operators.jl:116-125)
+ *   CHMY_OP_LEFT/RIGHT  leftx/rightx... (cartesian_field_operators.jl:17-28) */
+enum { CHMY_OP_PARTIAL = 0, CHMY_OP_LERP = 1, CHMY_OP_DIVG = 2, CHMY_OP_VMAG = 3,
+       CHMY_OP_LEFT = 8, CHMY_OP_RIGHT = 9 };
+int chmy_op(int op, const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* dst,
+            int nsrc, const chmy_field_t* src, int dim, void* stream);
+
+/* ---------------------------------------------------------------- diffusion */
+/* compute_q! (examples/diffusion_2d.jl:8-13): q_D[I] = -χ ∂_D C[I]. */
+int chmy_diffusion_flux(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* q,
+                        const chmy_field_t* C, double chi, void* stream);
+/* update_C! (examples/diffusion_2d.jl:15-19): C[I] -= Δt divg(q)[I]. */
+int chmy_diffusion_update(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* C,
+                          const chmy_field_t* q, double dt, void* stream);
+
+/* ---------------------------------------------------------------- Stokes + T
+ * tau arrays: 2D (xx, yy, xy), 3D (xx, yy, zz, xy, xz, yz) at the locations of
+ * TensorField (field.jl:182-206); V arrays: VectorField (field.jl:148).
+ * Viscosity: eta == NULL is the reference's scalar form (eta_s, eta_ve_s).
+ * eta != NULL (build extension for variable viscosity): eta at cell centres on
+ * the full padded storage; at a tau location eta = lerp(eta); per point
+ * eta_ve = 1/(1/eta + 1/(G dt)); the velocity update uses
+ * eta_ve(max(eta_ml[I-e_D], eta_ml[I])) with eta_ml from chmy_maxloc. */
+/* update_old! (examples/stokes_3d_inc_ve_T.jl:11-21); T/T_old may be NULL. */
+int chmy_stokes_update_old(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* T,
+                           const chmy_field_t* tau, const chmy_field_t* T_old, const chmy_field_t* tau_old,
+                           void* stream);
+/* update_stress! (examples/stokes_3d_inc_ve_T.jl:23-46; 2D stokes_2d_inc_ve_T.jl:20-34). */
+int chmy_stokes_stress(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* tau,
+                       const chmy_field_t* Pr, const chmy_field_t* divV, const chmy_field_t* V,
+                       const chmy_field_t* tau_old, const chmy_field_t* eta, double eta_s, double eta_ve_s,
+                       double G, double dt, double dtau_Pr, double dtau_r, void* stream);
+/* update_velocity! (examples/stokes_3d_inc_ve_T.jl:48-57; 2D :36-43); rhog
+ * (z component in 3D, y in 2D, at the V_last location) may be NULL. */
+int chmy_stokes_velocity(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* V,
+                         const chmy_field_t* r_V, const chmy_field_t* Pr, const chmy_field_t* tau,
+                         const chmy_field_t* rhog, const chmy_field_t* eta_ml, double eta_ve_s, double G,
+                         double dt, double nudtau, void* stream);
+/* dst[I] = max of src over the 3^N cell neighbourhood of I (build extension). */
+int chmy_maxloc(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* dst,
+                const chmy_field_t* src, void* stream);
+/* update_thermal_flux! (examples/stokes_3d_inc_ve_T.jl:59-71). */
+int chmy_thermal_flux(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* qT,
+                      const chmy_field_t* T, const chmy_field_t* V, double lambda_rhoCp, void* stream);
+/* update_thermal! (examples/stokes_3d_inc_ve_T.jl:73-77). */
+int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* T,
+                        const chmy_field_t* T_old, const chmy_field_t* qT, double dt, void* stream);
+
+/* One fused PT iteration = update_stress! then update_velocity! over the
+ * Launcher box (examples/stokes_3d_inc_ve_T.jl:222-223), double-buffered:
+ * reads (V, tau, Pr) and writes (V_out, tau_out, Pr_out), so no block reads a
+ * value another block has already advanced.  The results are those of the two
+ * reference kernels in sequence (tau/Pr outside the box keep their stored
+ * values).  r_V and divV are written only when write_residual != 0 (they are
+ * needed only at convergence checks).  The V boundary batch is applied by the
+ * caller afterwards (chmy_bc_batchset on V_out), as Launcher's bc= does. */
+int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
+                     const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
+                     const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                     const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* divV,
+                     const chmy_field_t* r_V, int write_residual, double eta_s, double eta_ve_s, double G,
+                     double dt, double dtau_Pr, double dtau_r, double nudtau, void* stream);
+
+/* ---------------------------------------------------------------- reductions */
+/* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),
+ * exact (order-independent); result in f64 written to *out_dev (device).
+ * NaN in the box yields NaN. */
+int chmy_maxabs_dev(const chmy_field_t* f, const chmy_box_t* box, double* out_dev, void* stream);
+/* Same, synchronizing and returning the scalar on the host. */
+int chmy_maxabs(const chmy_field_t* f, const chmy_box_t* box, double* out_host, void* stream);
+/* Σ f² over a box in f64, deterministic order (run-to-run identical); host result. */
+int chmy_sumsq(const chmy_field_t* f, const chmy_box_t* box, double* out_host, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHMY_MI355X_H */

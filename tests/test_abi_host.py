@@ -1,0 +1,143 @@
+"""CPU-side checks: the C-ABI library loads and exports every declared entry
+point; descriptor validation rejects misuse without touching a GPU; host
+logic (grid metrics, coordinates, batch regularisation, Launcher boxes)
+matches the reference's definitions."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import chmy
+from chmy import _lib
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "chmy_mi355x.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(chmy_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes table covers all of them
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_validation_rejects_misuse_on_host():
+    """Descriptor checks run before any HIP call: wrong locations, sizes,
+    halos and boxes return negative status with a message."""
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(8, 6))
+    gd = g.desc()
+
+    def fdesc(loc, n, halo=1, dtype=_lib.F64):
+        f = _lib.ChmyField()
+        f.data = 0x1000  # never dereferenced: validation fails first
+        f.nd = 2
+        f.dtype = dtype
+        for d in range(2):
+            f.n[d] = n[d]
+            f.halo[d] = halo
+            f.loc[d] = loc[d]
+        return f
+
+    Cf = fdesc((0, 0), (8, 6))
+    q_bad = (_lib.ChmyField * 2)(fdesc((0, 0), (8, 6)), fdesc((0, 1), (8, 7)))  # q.x at Center: wrong
+    rc = _lib.lib().chmy_diffusion_flux(C.byref(gd), None, q_bad, C.byref(Cf), 1.0, None)
+    assert rc == -3 and b"location" in _lib.lib().chmy_last_error()
+    q_ok = (_lib.ChmyField * 2)(fdesc((1, 0), (9, 6)), fdesc((0, 1), (8, 7)))
+    box = _lib.ChmyBox()
+    box.lo[0], box.lo[1], box.len[0], box.len[1] = 0, 0, 11, 8  # x range [0, 11) > n+2 = 10
+    rc = _lib.lib().chmy_diffusion_flux(C.byref(gd), C.byref(box), q_ok, C.byref(Cf), 1.0, None)
+    assert rc == -4 and b"box" in _lib.lib().chmy_last_error()
+    Ch = fdesc((0, 0), (8, 6), halo=0)
+    rc = _lib.lib().chmy_diffusion_flux(C.byref(gd), None, q_ok, C.byref(Ch), 1.0, None)
+    assert rc == -5
+    Cs = fdesc((0, 0), (8, 5))
+    rc = _lib.lib().chmy_diffusion_flux(C.byref(gd), None, q_ok, C.byref(Cs), 1.0, None)
+    assert rc == -3 and b"size" in _lib.lib().chmy_last_error()
+    q32 = (_lib.ChmyField * 2)(fdesc((1, 0), (9, 6), dtype=_lib.F32), fdesc((0, 1), (8, 7), dtype=_lib.F32))
+    rc = _lib.lib().chmy_diffusion_flux(C.byref(gd), None, q32, C.byref(Cf), 1.0, None)
+    assert rc == -2
+
+
+def test_host_field_refuses_kernel_use():
+    g = chmy.UniformGrid(origin=(0.0,), extent=(1.0,), dims=(4,))
+    f = chmy.Field(None, g, chmy.Center())
+    with pytest.raises(chmy.ChmyError):
+        f.desc()
+
+
+def test_grid_metrics_and_coords_match_oracle():
+    """UniformGrid spacing/inv_spacing and fma coordinates
+    (uniform_axis.jl:7-19; test/test_grids.jl:10-175)."""
+    g = chmy.UniformGrid(origin=(-1, -2), extent=(2, 4), dims=(5, 20))
+    assert chmy.spacing(g) == (0.4, 0.2) and chmy.inv_spacing(g) == (2.5, 5.0)
+    og = O.Grid((-1, -2), (2, 4), (5, 20))
+    for loc in (chmy.Center(), chmy.Vertex()):
+        for d in range(2):
+            a = g.coords(loc, d, first=-1, count=g.dims[d] + 5)
+            b = og.coords(loc.code, d, first=-1, count=g.dims[d] + 5)
+            assert np.array_equal(a, b)
+    assert math.isclose(chmy.coord(g, chmy.Center(), 0, 5), 0.8)
+
+
+def test_field_layout():
+    """Field storage n .+ 4H, x fastest, logical I at I .+ 2H (field.jl:6-18,56-62)."""
+    g = chmy.UniformGrid(origin=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0), dims=(2, 2, 2))
+    f = chmy.Field(None, g, (chmy.Center(), chmy.Vertex(), chmy.Center()), halo=(1, 0, 1))
+    assert f.dims == (2, 3, 2) and f.pad == (6, 3, 6)
+    assert tuple(f.data.shape) == (6, 3, 6)
+    chmy.set_(f, lambda x, y, z: y + 0 * x, g)
+    exp = np.array([0.0, 0.0, 0.5, 0.5, 1.0, 1.0, 0.0, 0.0, 0.5, 0.5, 1.0, 1.0])
+    assert np.array_equal(f.interior().numpy().reshape(-1, order="F"), exp)
+    v = chmy.VectorField(None, g)
+    assert [tuple(l.code for l in c.loc) for c in v] == [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    t = chmy.TensorField(None, g)
+    assert t._fields == ("xx", "yy", "zz", "xy", "xz", "yz")
+    assert tuple(l.code for l in t.xz.loc) == (1, 0, 1)
+
+
+def test_batch_regularisation():
+    """batch(grid, f => bc...) per-dim/per-side FieldBatches with pruning of
+    `nothing` and per-side tuples (batch.jl:72-157)."""
+    g = chmy.UniformGrid(origin=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0), dims=(4, 4, 4))
+    V = chmy.VectorField(None, g)
+    D, N = chmy.Dirichlet, chmy.Neumann
+    bs = chmy.batch(g, (V.x, {"x": D(), "y": N(), "z": N()}), (V.y, {"y": (D(1.0), D(-1.0))}),
+                    (V.z, N()), exchange=(V.x, V.y, V.z))
+    assert [len(bs[0][s].fields) for s in range(2)] == [2, 2]      # V.x, V.z in x
+    assert [len(bs[1][s].fields) for s in range(2)] == [3, 3]
+    assert bs[1][0].conditions[1].value == 1.0 and bs[1][1].conditions[1].value == -1.0
+    assert isinstance(chmy.batch(g, (V.x, None))[0][0], chmy.EmptyBatch)
+    with pytest.raises(ValueError):
+        chmy.batch(g, (V.x, {"w": D()}))
+
+
+def test_launcher_boxes():
+    """worksize n .+ 2 at Offset(-1); inner/outer boxes (KernelLaunch.jl:40-87)."""
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(30, 20))
+    L = chmy.Launcher(None, g, outer_width=(4, 3))
+    assert L.worksize == (32, 22)
+    assert L.inner_worksize() == (24, 16)
+    assert L.outer_worksize(1) == (32, 3) and L.outer_offset(1, 2) == (0, 19)
+    assert L.outer_worksize(0) == (4, 16) and L.outer_offset(0, 1) == (0, 3)
+    # the slabs and the inner box tile the launcher box exactly
+    cover = np.zeros(L.worksize, int)
+    cover[4:28, 3:19] += 1
+    for D in range(2):
+        for S in (1, 2):
+            o, w = L.outer_offset(D, S), L.outer_worksize(D)
+            cover[o[0]:o[0] + w[0], o[1]:o[1] + w[1]] += 1
+    assert np.all(cover == 1)

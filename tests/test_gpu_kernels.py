@@ -1,0 +1,349 @@
+"""GPU parity of every Launcher kernel against the CPU oracle (bitwise).
+
+Inputs fill the whole padded storage with seeded random values so that reads
+of halos and padding are exercised identically on both sides.  The bar is
+bitwise equality in f64 and f32: both sides evaluate the same IEEE
+expressions with no contraction and fma only at the reference's muladd
+sites.
+"""
+import numpy as np
+import pytest
+
+import chmy
+from chmy import Center as Cc, Vertex as Vv
+import oracle as O
+from _helpers import assert_same, dev, ogrid, pair, vloc
+
+pytestmark = pytest.mark.gpu
+
+TYPES = [np.float64, np.float32]
+
+
+@pytest.fixture(scope="module")
+def arch():
+    return chmy.Arch(chmy.MI355XBackend())
+
+
+def grids(nd):
+    if nd == 1:
+        return chmy.UniformGrid(origin=(-1.0,), extent=(2.0,), dims=(37,))
+    if nd == 2:
+        return chmy.UniformGrid(origin=(-1.0, -0.5), extent=(2.0, 1.5), dims=(70, 33))
+    return chmy.UniformGrid(origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.0, 1.2), dims=(67, 13, 9))
+
+
+# ---------------------------------------------------------------- operators
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_partial_lerp_left_right(arch, T, nd):
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(10 + nd)
+    L = chmy.Launcher(arch, g)
+    locs = [tuple(Cc() if (m >> d) & 1 == 0 else Vv() for d in range(nd)) for m in range(1 << nd)]
+    for src_loc in locs:
+        s, hs = pair(arch, g, og, src_loc, T, rng=rng)
+        for D in range(nd):
+            out_loc = tuple((Vv() if isinstance(l, Cc) else Cc()) if d == D else l for d, l in enumerate(src_loc))
+            d_, hd = pair(arch, g, og, out_loc, T, fill=0)
+            L(arch, g, (chmy.kernels.partial, (d_, s, D, g)))
+            O.op("partial", og, hd, hs, dim=D)
+            assert_same(d_, hd, f"∂ dim {D} src {src_loc}")
+            for name, kern in (("left", chmy.kernels.left), ("right", chmy.kernels.right)):
+                L(arch, g, (kern, (d_, s, D, g)))
+                O.op(name, og, hd, hs, dim=D)
+                assert_same(d_, hd, f"{name} dim {D}")
+        for to in locs:
+            d_, hd = pair(arch, g, og, to, T, fill=0)
+            L(arch, g, (chmy.kernels.lerp, (d_, s, g)))
+            O.op("lerp", og, hd, hs)
+            assert_same(d_, hd, f"lerp {src_loc}->{to}")
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [2, 3])
+def test_divg_vmag(arch, T, nd):
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(20 + nd)
+    L = chmy.Launcher(arch, g)
+    V = [pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)]
+    out, hout = pair(arch, g, og, Cc(), T, fill=0)
+    L(arch, g, (chmy.kernels.divg, (out, [v for v, _ in V], g)))
+    O.op("divg", og, hout, [h for _, h in V])
+    assert_same(out, hout, "divg")
+    L(arch, g, (chmy.kernels.vmag, (out, [v for v, _ in V], g)))
+    O.op("vmag", og, hout, [h for _, h in V])
+    assert_same(out, hout, "vmag")
+
+
+def test_reference_operator_identities(arch):
+    """test/test_grid_operators.jl:14-42,114-132 through the GPU Launcher:
+    divg(V) == ∂xVx + ∂yVy + ∂zVz exactly; vmag of a constant 2-vector ≈ 3.4641."""
+    g = chmy.UniformGrid(arch, origin=(-5.0, -5.0, -5.0), extent=(10.0, 10.0, 10.0), dims=(12, 10, 8))
+    L = chmy.Launcher(arch, g)
+    Ci = chmy.Field(arch, g, Cc())
+    chmy.set_(Ci, lambda x, y, z: np.exp(-x ** 2 - y ** 2 - z ** 2), g)
+    V = chmy.VectorField(arch, g)
+    for D in range(3):
+        L(arch, g, (chmy.kernels.partial, (V[D], Ci, D, g)))
+    parts = [chmy.Field(arch, g, Cc()) for _ in range(3)]
+    for D in range(3):
+        L(arch, g, (chmy.kernels.partial, (parts[D], V[D], D, g)))
+    C2 = chmy.Field(arch, g, Cc())
+    L(arch, g, (chmy.kernels.divg, (C2, list(V), g)))
+    C1 = (parts[0].interior() + parts[1].interior()) + parts[2].interior()
+    assert bool((C2.interior() == C1).all())
+    for D in range(3):
+        chmy.set_(V[D], 2.0)
+    L(arch, g, (chmy.kernels.vmag, (C2, list(V), g)))
+    v = C2.interior().cpu().numpy()
+    assert np.all(np.round(v, 4) == 3.4641)
+
+
+# ---------------------------------------------------------------- boundary conditions
+def _bc_specs(nd, value_fields):
+    D, N = chmy.Dirichlet, chmy.Neumann
+    names = "xyz"[:nd]
+    specs = [D(), N(), D(2.0), N(-1.5)]
+    mixed = {names[0]: (D(1.0), N(0.5))}
+    if nd > 1:
+        mixed[names[1]] = N()
+    if nd > 2:
+        mixed[names[2]] = (N(), D(-3.0))
+    specs.append(mixed)
+    if value_fields is not None:
+        specs.append({names[0]: D(value_fields[0]), names[-1]: (N(value_fields[1]), D())})
+    return specs
+
+
+def _ospec(spec, nd, vmap):
+    names = "xyz"[:nd]
+    def conv(b):
+        v = b.value
+        if isinstance(v, chmy.Field):
+            v = vmap[id(v)]
+        return O.BC(b.kind, v)
+    if isinstance(spec, chmy.FirstOrderBC):
+        return conv(spec)
+    out = {}
+    for k, b in spec.items():
+        d = names.index(k)
+        out[d] = tuple(conv(x) for x in b) if isinstance(b, tuple) else conv(b)
+    return out
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_bc_batches(arch, T, nd):
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(30 + nd)
+    vmap = {}
+    vfields = None
+    if nd >= 2:
+        # value fields on the (nd-1)-D tangential grid (Center sizes, halo 1)
+        tg = chmy.UniformGrid(origin=g.origin[1:], extent=g.extent[1:], dims=g.dims[1:])
+        tg0 = chmy.UniformGrid(origin=g.origin[:-1], extent=g.extent[:-1], dims=g.dims[:-1])
+        otg, otg0 = ogrid(tg), ogrid(tg0)
+        a, ha = pair(arch, tg, otg, Cc(), T, rng=rng)
+        b, hb = pair(arch, tg0, otg0, Cc(), T, rng=rng)
+        vfields = (a, b)
+        vmap = {id(a): ha, id(b): hb}
+    locs = [tuple(Cc() if (m >> d) & 1 == 0 else Vv() for d in range(nd)) for m in range(1 << nd)]
+    for spec in _bc_specs(nd, vfields):
+        fs = [pair(arch, g, og, l, T, rng=rng) for l in locs]
+        chmy.bc_(arch, g, *[(f, spec) for f, _ in fs])
+        O.bc_apply(og, [(h, _ospec(spec, nd, vmap)) for _, h in fs])
+        for (f, h), l in zip(fs, locs):
+            assert_same(f, h, f"bc {spec} loc {l}")
+
+
+def test_reference_bc_ghost_relations_3d(arch):
+    """test/test_boundary_conditions.jl:140-205 on the GPU: 8x8x6, loc (C,V,C)."""
+    import math
+    g = chmy.UniformGrid(arch, origin=(-math.pi,) * 3, extent=(2 * math.pi,) * 3, dims=(8, 8, 6))
+    f = chmy.Field(arch, g, (Cc(), Vv(), Cc()))
+    s = slice(1, -1)
+    chmy.set_(f, 1.0)
+    chmy.bc_(arch, g, (f, chmy.Dirichlet(2.0)))
+    fi = f.interior(with_halo=True).cpu().numpy()
+    assert np.allclose(fi[0, s, s], -fi[1, s, s] + 4) and np.allclose(fi[-1, s, s], -fi[-2, s, s] + 4)
+    assert np.all(fi[s, 1, s] == 2) and np.all(fi[s, -2, s] == 2)
+    assert np.allclose(fi[s, s, 0], -fi[s, s, 1] + 4) and np.allclose(fi[s, s, -1], -fi[s, s, -2] + 4)
+    chmy.set_(f, 1.0)
+    chmy.bc_(arch, g, (f, chmy.Neumann(2.0)))
+    fi = f.interior(with_halo=True).cpu().numpy()
+    dx, dy, dz = chmy.spacing(g)
+    assert np.allclose((fi[1, s, s] - fi[0, s, s]) / dx, 2.0)
+    assert np.allclose((fi[s, -1, s] - fi[s, -2, s]) / dy, 2.0)
+    assert np.allclose((fi[s, s, 1] - fi[s, s, 0]) / dz, 2.0)
+
+
+# ---------------------------------------------------------------- diffusion (config: 256² x 1000 steps)
+def test_diffusion_2d_256_1000_steps(arch):
+    """examples/diffusion_2d.jl loop at 256² on [-1,1]², C0 = exp(-(x²+y²)/0.1) +
+    1e-3·noise, χ = 1, Δt = min(Δ)²/4.1, Neumann(0), 1000 steps: every stored
+    value (interior + post-BC halos) bitwise equal to the oracle."""
+    n, steps = 256, 1000
+    g = chmy.UniformGrid(arch, origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(n, n))
+    og = ogrid(g)
+    L = chmy.Launcher(arch, g, sync=False)
+    rng = np.random.default_rng(1234)
+    xs = og.coords(O.CENTER, 0)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    C0 = np.exp(-(X ** 2 + Y ** 2) / 0.1) + 1e-3 * rng.random((n, n))
+    Cf = chmy.Field(arch, g, Cc())
+    q = chmy.VectorField(arch, g)
+    hC = O.HField(og, O.CENTER)
+    hq = [O.HField(og, (O.VERTEX, O.CENTER)), O.HField(og, (O.CENTER, O.VERTEX))]
+    chmy.set_(Cf, C0)
+    hC.set_interior(C0)
+    bcs = chmy.batch(g, (Cf, chmy.Neumann()), exchange=Cf)
+    chmy.bc_(arch, g, bcs)
+    O.bc_apply(og, [(hC, O.Neumann())])
+    dt = min(chmy.spacing(g)) ** 2 / 1.0 / 4.1
+    for _ in range(steps):
+        L(arch, g, (chmy.kernels.compute_q, (q, Cf, 1.0, g)))
+        L(arch, g, (chmy.kernels.update_C, (Cf, q, dt, g)), bc=bcs)
+        O.compute_q(og, hq, hC, 1.0)
+        O.update_C(og, hC, hq, dt)
+        O.bc_apply(og, [(hC, O.Neumann())])
+    chmy.synchronize()
+    assert_same(Cf, hC, "C after 1000 steps")
+    assert_same(q.x, hq[0], "qx")
+
+
+@pytest.mark.parametrize("ow", [(16, 8), (3, 5)])
+def test_launcher_outer_width_matches(arch, ow):
+    """Launcher(outer_width): inner box + 2N slabs with per-side bc
+    (KernelLaunch.jl:152-183) give the same fields as the plain launch."""
+    g = chmy.UniformGrid(arch, origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(62, 40))
+    rng = np.random.default_rng(5)
+    res = []
+    for width in (None, ow):
+        L = chmy.Launcher(arch, g, outer_width=width)
+        Cf = chmy.Field(arch, g, Cc())
+        q = chmy.VectorField(arch, g)
+        chmy.set_(Cf, rng.random((62, 40)) if not res else res[0][1])
+        c0 = Cf.interior().cpu().numpy().copy()
+        bcs = chmy.batch(g, (Cf, chmy.Neumann()))
+        chmy.bc_(arch, g, bcs)
+        for _ in range(20):
+            L(arch, g, (chmy.kernels.compute_q, (q, Cf, 1.0, g)))
+            L(arch, g, (chmy.kernels.update_C, (Cf, q, 1e-4, g)), bc=bcs)
+        res.append((dev(Cf), c0))
+    assert np.array_equal(res[0][0], res[1][0])
+
+
+# ---------------------------------------------------------------- Stokes kernels
+def _stokes_pairs(arch, g, og, T, rng):
+    nd = g.nd
+    tl = list(chmy.tensor_locations(nd).values())
+    F = dict(
+        Pr=pair(arch, g, og, Cc(), T, rng=rng), dV=pair(arch, g, og, Cc(), T, rng=rng),
+        V=[pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)],
+        rV=[pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)],
+        tau=[pair(arch, g, og, l, T, rng=rng) for l in tl],
+        tauo=[pair(arch, g, og, l, T, rng=rng) for l in tl],
+        eta=pair(arch, g, og, Cc(), T, rng=rng, lo=1e-3, hi=3.0),
+        rg=pair(arch, g, og, vloc(nd - 1, nd), T, rng=rng),
+    )
+    return F
+
+
+def _d(F, k):
+    v = F[k]
+    return [a for a, _ in v] if isinstance(v, list) else v[0]
+
+
+def _h(F, k):
+    v = F[k]
+    return [b for _, b in v] if isinstance(v, list) else v[1]
+
+
+def _check_all(F, what):
+    for k, v in F.items():
+        for i, (a, b) in enumerate(v if isinstance(v, list) else [v]):
+            assert_same(a, b, f"{what}: {k}[{i}]")
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("visc", ["scalar", "field"])
+def test_stress_velocity(arch, T, nd, visc):
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(40 + nd)
+    F = _stokes_pairs(arch, g, og, T, rng)
+    eml, heml = pair(arch, g, og, Cc(), T, fill=0)
+    L = chmy.Launcher(arch, g)
+    G, dt, eta = 1.0, 0.37, 0.8
+    eta_ve = 1.0 / (1.0 / eta + 1.0 / (G * dt))
+    p = O.pt_params(og)
+    if visc == "field":
+        L(arch, g, (chmy.kernels.maxloc, (eml, _d(F, "eta"))))
+        O.maxloc(og, heml, _h(F, "eta"))
+        assert_same(eml, heml, "maxloc")
+        # the Launcher box touches eta_ml at I-e_D: fill the rest of its storage identically
+        heml.arr[...] = dev(eml)
+    ed = _d(F, "eta") if visc == "field" else eta
+    eh = _h(F, "eta") if visc == "field" else eta
+    L(arch, g, (chmy.kernels.update_stress, (_d(F, "tau"), _d(F, "Pr"), _d(F, "dV"), _d(F, "V"), _d(F, "tauo"),
+                                             ed, eta_ve, G, dt, p["dtau_Pr"], p["dtau_r"], g)))
+    O.update_stress(og, _h(F, "tau"), _h(F, "Pr"), _h(F, "dV"), _h(F, "V"), _h(F, "tauo"), eh, eta_ve, G, dt,
+                    p["dtau_Pr"], p["dtau_r"])
+    _check_all(F, "after update_stress")
+    if visc == "field":
+        L(arch, g, (chmy.kernels.update_velocity_visc, (_d(F, "V"), _d(F, "rV"), _d(F, "Pr"), _d(F, "tau"),
+                                                        _d(F, "rg"), eml, G, dt, p["nudtau"], g)))
+        O.update_velocity(og, _h(F, "V"), _h(F, "rV"), _h(F, "Pr"), _h(F, "tau"), _h(F, "rg"), 0.0, p["nudtau"],
+                          eta_ml=heml, G=G, dt=dt)
+    else:
+        L(arch, g, (chmy.kernels.update_velocity, (_d(F, "V"), _d(F, "rV"), _d(F, "Pr"), _d(F, "tau"),
+                                                   _d(F, "rg"), eta_ve, p["nudtau"], g)))
+        O.update_velocity(og, _h(F, "V"), _h(F, "rV"), _h(F, "Pr"), _h(F, "tau"), _h(F, "rg"), eta_ve, p["nudtau"])
+    _check_all(F, "after update_velocity")
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [2, 3])
+def test_update_old_and_thermal(arch, T, nd):
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(50 + nd)
+    L = chmy.Launcher(arch, g)
+    tl = list(chmy.tensor_locations(nd).values())
+    tau = [pair(arch, g, og, l, T, rng=rng) for l in tl]
+    tauo = [pair(arch, g, og, l, T, rng=rng) for l in tl]
+    Tf, hT = pair(arch, g, og, Cc(), T, rng=rng)
+    To, hTo = pair(arch, g, og, Cc(), T, rng=rng)
+    L(arch, g, (chmy.kernels.update_old, (Tf, [a for a, _ in tau], To, [a for a, _ in tauo])))
+    O.update_old(og, hT, [b for _, b in tau], hTo, [b for _, b in tauo])
+    for (a, b) in tauo + [(To, hTo)]:
+        assert_same(a, b, "update_old")
+    V = [pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)]
+    qT = [pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)]
+    L(arch, g, (chmy.kernels.update_thermal_flux, ([a for a, _ in qT], Tf, [a for a, _ in V], 0.37, g)))
+    O.thermal_flux(og, [b for _, b in qT], hT, [b for _, b in V], 0.37)
+    for a, b in qT:
+        assert_same(a, b, "thermal flux")
+    L(arch, g, (chmy.kernels.update_thermal, (Tf, To, [a for a, _ in qT], 1e-3, g)))
+    O.thermal(og, hT, hTo, [b for _, b in qT], 1e-3)
+    assert_same(Tf, hT, "thermal update")
+
+
+# ---------------------------------------------------------------- reductions
+@pytest.mark.parametrize("T", TYPES)
+def test_reductions(arch, T):
+    g = grids(3)
+    og = ogrid(g)
+    rng = np.random.default_rng(60)
+    f, h = pair(arch, g, og, (Vv(), Cc(), Vv()), T, rng=rng, lo=-5, hi=5)
+    assert chmy.kernels.maxabs(f) == O.maxabs(h)
+    s = chmy.kernels.sumsq(f)
+    assert np.isclose(s, O.sumsq(h), rtol=1e-12 if T == np.float64 else 1e-6)
+    assert s == chmy.kernels.sumsq(f)  # deterministic order
+    # NaN propagates
+    f.interior()[3, 2, 1] = float("nan")
+    assert np.isnan(chmy.kernels.maxabs(f))
