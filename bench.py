@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="fused", choices=["fused", "unfused"],
+                    help="fused: one-pass double-buffered iteration; unfused: the reference's "
+                         "update_stress! + update_velocity! kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -121,8 +124,9 @@ def main():
     n = args.n
     S = setup(n, dtype)
     print(f"# setup done: {n}^3 {args.dtype}", file=sys.stderr, flush=True)
+    fused = args.mode == "fused"
     for _ in range(args.warmup):
-        S.iteration()
+        S.iteration() if fused else S.iteration_unfused()
     torch.cuda.synchronize()
     K = args.steps
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
@@ -131,11 +135,18 @@ def main():
     torch.cuda.synchronize()
     t_start.record()
     for q in range(K):
-        ev[q][0].record()
-        S._iter_call(False)
-        ev[q][1].record()
-        S.apply_bc(S.V2)
-        S._swap()
+        if fused:
+            ev[q][0].record()
+            S._iter_call(False)
+            ev[q][1].record()
+            S.apply_bc(S.V2)
+            S._swap()
+        else:
+            # kernel time here = both reference kernels (stress + velocity); the BC batch follows
+            ev[q][0].record()
+            S.iteration_unfused_kernels()
+            ev[q][1].record()
+            S.apply_bc(S.V)
     t_end.record()
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
@@ -145,7 +156,10 @@ def main():
     teff = A / (ms_step * 1e-3) / 1e9
     achieved = A / (k_ms * 1e-3) / 1e9
     # residual sanity (not timed)
-    S.iteration(write_res=True)
+    if fused:
+        S.iteration(write_res=True)
+    else:
+        S.iteration_unfused()
     err = S.residual()
     cpu = None
     if not args.no_cpu:
@@ -174,7 +188,8 @@ def main():
                    "residual_after_timing": err},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "fused3d_kernel<double,true,8>", "kernel_ms": round(k_ms, 4)},
+                     "kernel": "fused3d_kernel" if fused else "stress_kernel + velocity_kernel",
+                     "kernel_ms": round(k_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

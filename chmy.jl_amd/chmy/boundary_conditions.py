@@ -115,7 +115,12 @@ class BatchSet:
 
     def _descriptors(self):
         key = self._key()
-        if self._c is None or self._c[0] != key:
+        if self._c is None:
+            self._c = {}
+        hit = self._c.get(key)
+        if hit is None:
+            if len(self._c) >= 4:          # double buffering alternates two keys
+                self._c.clear()
             keep = []
             counts = (C.c_int32 * 6)()
             fl, bl = [], []
@@ -129,8 +134,9 @@ class BatchSet:
             n = max(1, len(fl))
             fa = (_lib.ChmyField * n)(*fl)
             ba = (_lib.ChmyBC * n)(*bl)
-            self._c = (key, counts, fa, ba, keep)
-        return self._c[1:]
+            hit = (counts, fa, ba, keep)
+            self._c[key] = hit
+        return hit
 
     def apply(self, stream=None):
         counts, fa, ba, _keep = self._descriptors()

@@ -29,6 +29,9 @@ from .kernels import _arr, _box, maxloc, HipKernel
 from .kernel_launch import Launcher
 
 
+ITER_RESIDUAL, ITER_EXACT = 1, 2
+
+
 def pt_params(grid, re_m=2.3 * math.pi, r=0.5):
     """PT parameters (stokes_3d_inc_ve_T.jl:163-170)."""
     lt = min(grid.extent) / re_m
@@ -54,7 +57,7 @@ class PTStokes:
     """Fields and loop of one PT solve.  dtype float64 or float32."""
 
     def __init__(self, arch, grid, *, dtype=torch.float64, eta=None, eta_field=None, G=1.0, dt=None,
-                 normal_values=None, re_m=2.3 * math.pi, r=0.5, rhog=None):
+                 normal_values=None, re_m=2.3 * math.pi, r=0.5, rhog=None, exact=False):
         self.arch, self.grid, self.dtype = arch, grid, dtype
         nd = grid.nd
         self.nd = nd
@@ -82,6 +85,9 @@ class PTStokes:
         self._bcs = {}
         self._rbc = {}
         self.iters = 0
+        # exact=True: reference expression order (bitwise equal to the two
+        # reference kernels); False: the algebraically equal fast form
+        self.exact = bool(exact)
 
     # ------------------------------------------------------------------ state
     def sync_buffers(self):
@@ -90,8 +96,10 @@ class PTStokes:
             b.data.copy_(a.data)
 
     def apply_bc(self, V=None):
+        """Free-slip V batch on the given vector field (default: the current state).
+        Batch sets are cached per Field object; they track storage swaps."""
         V = V if V is not None else self.V
-        key = V[0].data.data_ptr()
+        key = id(V[0])
         bs = self._bcs.get(key)
         if bs is None:
             bs = free_slip_batch(self.grid, V, self.normal_values)
@@ -105,7 +113,8 @@ class PTStokes:
                   C.byref(self.eta.desc()) if self.eta is not None else None,
                   C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
                   C.byref(self.rhog.desc()) if self.rhog is not None else None,
-                  C.byref(self.divV.desc()), _arr(self.r_V), int(write_res), self.eta_s, self.eta_ve_s, self.G,
+                  C.byref(self.divV.desc()), _arr(self.r_V),
+                  (ITER_RESIDUAL if write_res else 0) | (ITER_EXACT if self.exact else 0), self.eta_s, self.eta_ve_s, self.G,
                   self.dt, self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], current_stream())
 
     def _swap(self):
@@ -120,12 +129,34 @@ class PTStokes:
         self._swap()
         self.iters += 1
 
+    def iteration_unfused(self):
+        """The reference's sequence: update_stress!, update_velocity! (per-point
+        kernels, single buffers, r_V and ∇V written), then the V batch."""
+        self.iteration_unfused_kernels()
+        self.apply_bc(self.V)
+        self.iters += 1
+
+    def iteration_unfused_kernels(self):
+        from .kernels import _box, update_stress, update_velocity, update_velocity_visc
+        g = self.grid
+        box = _box((0,) * self.nd, tuple(n + 2 for n in g.dims))
+        p = self.p
+        eta = self.eta if self.eta is not None else self.eta_s
+        update_stress._fn(g, box, self.tau, self.Pr, self.divV, self.V, self.tau_old, eta, self.eta_ve_s, self.G,
+                          self.dt, p["dtau_Pr"], p["dtau_r"], g)
+        if self.eta_ml is not None:
+            update_velocity_visc._fn(g, box, self.V, self.r_V, self.Pr, self.tau, self.rhog, self.eta_ml, self.G,
+                                     self.dt, p["nudtau"], g)
+        else:
+            update_velocity._fn(g, box, self.V, self.r_V, self.Pr, self.tau, self.rhog, self.eta_ve_s,
+                                p["nudtau"], g)
+
     def residual(self, tau_sc=1.0, lz=1.0, psc=1.0):
         """err = (max|∇V|·τsc, max|r_V.x|·lz/psc, ...) after zeroing boundary
         residual vertices (stokes_3d_inc_ve_T.jl:229-233).  Requires the last
         iteration to have written the residual fields."""
         names = self.grid.axes_names()
-        key = self.r_V[0].data.data_ptr()
+        key = id(self.r_V[0])
         rb = self._rbc.get(key)
         if rb is None:
             rb = batch(self.grid, *[(self.r_V[D], {names[D]: Dirichlet()}) for D in range(self.nd)])

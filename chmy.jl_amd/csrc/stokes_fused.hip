@@ -2,50 +2,123 @@
 // update_stress! followed by update_velocity! (examples/stokes_3d_inc_ve_T.jl:
 // 222-223), fused and double-buffered.
 //
-// Layout of the work (3D): a block is 64 (x, one wave) x BY (y) threads; thread
+// Work layout (3D): a block is 64 (x, one wave) x BY (y) threads; thread
 // (tx, ty) owns the logical column (i, j) = (i0-1+tx, j0-1+ty) and marches z
 // over a chunk [kb, ke).  The tile's outputs are tx in [1, 62], ty in
-// [1, BY-2]; the one-cell ring around them recomputes the stresses the
-// velocity stencil needs (Center values at I-e, edge values at I+e), so no
-// stress is ever read back from HBM after being advanced.  At z-step k a
-// thread computes the Center stresses and Pr at plane k, tau_xy at plane k
-// and tau_xz, tau_yz at z-vertex plane k+1, then V at plane k.  x/y
-// neighbours come from LDS planes; z neighbours are carried in registers.
+// [1, BY-2]; the one-cell ring around them recomputes the stresses that the
+// velocity stencil needs (Center values at I-e, edge values at I+e), so an
+// advanced stress is never read back from HBM.  At z-step k a thread computes
+// the Center stresses and Pr at plane k, tau_xy at plane k and tau_xz, tau_yz
+// at z-vertex plane k+1, then V at plane k.  x/y neighbours come from LDS
+// planes; z neighbours are carried in registers.  Two barriers per z-step:
+// LDS planes whose readers and next writers are not separated by a barrier
+// are double-buffered by step parity.
+//
+// Addressing: the 7 staggered locations (Center, Vx, Vy, Vz, xy, xz, yz edges)
+// each have their own padded strides; a thread keeps one 32-bit byte offset per
+// location (advanced by the z stride each step) and every array is a uniform
+// base pointer, so loads are SGPR-base + VGPR-offset.  The host checks that
+// every offset fits in 32 bits.
 //
 // Inputs (V, tau, Pr) and outputs (V_out, tau_out, Pr_out) are distinct
-// buffers, so a block never reads a value another block has advanced.  Points
-// of the ring outside the Launcher box keep their stored (input) value, as in
-// the two-kernel sequence where update_stress! does not touch them.  Every
-// arithmetic expression is the one of the per-point kernels (stokes.hip), so
-// the results are bitwise those of update_stress! + update_velocity!.
+// buffers, so a block never reads a value another block has advanced.  Ring
+// points outside the Launcher box keep their stored (input) value, as in the
+// two-kernel sequence where update_stress! does not touch them.  With
+// CHMY_ITER_EXACT every arithmetic expression is the one of the per-point
+// kernels (stokes.hip), so the results are bitwise those of update_stress! +
+// update_velocity!; the default fast form (Arith<T, false>) is algebraically
+// equal and agrees to a few ulp per iteration.
 #include "common.hpp"
 
 namespace chmy {
 
 constexpr int kFX = 64;
+enum { LC = 0, LVX = 1, LVY = 2, LVZ = 3, LXY = 4, LXZ = 5, LYZ = 6, NLOC = 7 };
 
+// Scalars shared by both arithmetic forms.
 template <class T>
-struct FusedArgs {
-    DF<T> V[3], tau[6], Pr;            // inputs
-    DF<T> Vo[3], tauo_[6], Pro;        // outputs
-    DF<T> told[6], eta, eml, rg;
-    DF<T> dV, rV[3];
-    int64_t n[3];                      // grid cells
-    T ix, iy, iz;
-    T eta_s, eve_s, Gs, dts, Gdt, dPr, dr, nu;
-    int has_rg, write_res, kchunk;
+struct Kc {
+    T Gs, dts, Gdt, dPr, dr, nu, eta_s, eve_s;   // reference inputs
+    T iGdt, omdr, third, a_s, b_s, cv_s;         // fast-form constants (host computed)
+};
+
+struct LocStride {
+    uint32_t sy, sz;     // element strides
+    uint32_t o0;         // element offset of logical (0, 0, 0)
 };
 
 template <class T>
-__device__ __forceinline__ T f_diag(T t, T to, T e, T dV, T eta, T eve, T Gdt, T dr) {
-    T r = ((-(t - to)) / Gdt - t / eta) + T(2) * (e - dV / T(3));
-    return t + (r * eve) * dr;
+struct Fused3 {
+    const T *Vx, *Vy, *Vz, *Pr, *txx, *tyy, *tzz, *txy, *txz, *tyz;
+    T *oVx, *oVy, *oVz, *oPr, *otxx, *otyy, *otzz, *otxy, *otxz, *otyz;
+    const T *Oxx, *Oyy, *Ozz, *Oxy, *Oxz, *Oyz;    // tau_old
+    const T *eta, *eml, *rg;
+    T *dV, *rVx, *rVy, *rVz;
+    LocStride L[NLOC];
+    int32_t n[3];
+    int32_t ntx, nty, nchunks, kchunk;
+    int32_t has_rg, write_res;
+    T ix, iy, iz;
+    Kc<T> k;
+};
+
+template <class T>
+__device__ __forceinline__ T ldo(const T* p, uint32_t boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
 }
 template <class T>
-__device__ __forceinline__ T f_off(T t, T to, T e, T eta, T eve, T Gdt, T dr) {
-    T r = ((-(t - to)) / Gdt - t / eta) + T(2) * e;
-    return t + (r * eve) * dr;
+__device__ __forceinline__ void sto(T* p, uint32_t boff, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + boff) = v;
 }
+
+// EXACT: the reference's expressions (stokes_3d_inc_ve_T.jl:91-114), bitwise.
+// FAST: the same update with eta_ve*(1/(G dt) + 1/eta) = 1 used to cancel
+// tau/eta: tau_new = tau(1 - dtau_r) + eta_ve dtau_r (tau_old/(G dt) + 2 eps'),
+// eta_ve = eta/(1 + eta/(G dt)), Pr_new = Pr - divV eta_ve dtau_Pr and
+// V_new = V + r nudtau (1/eta_V + 1/(G dt)): 7 divisions per point instead
+// of 31, results within a few ulp per iteration of the exact form.
+template <class T, bool EXACT>
+struct Arith;
+
+template <class T>
+struct Arith<T, true> {
+    struct Loc { T eta, eve; };
+    __device__ static Loc field(T eta, const Kc<T>& k) { return Loc{eta, visc_ve(eta, k.Gs, k.dts)}; }
+    __device__ static Loc scalar(const Kc<T>& k) { return Loc{k.eta_s, k.eve_s}; }
+    __device__ static T diag(T t, T to, T e, T dV, Loc L, const Kc<T>& k) {
+        T r = ((-(t - to)) / k.Gdt - t / L.eta) + T(2) * (e - dV / T(3));
+        return t + (r * L.eve) * k.dr;
+    }
+    __device__ static T off(T t, T to, T e, Loc L, const Kc<T>& k) {
+        T r = ((-(t - to)) / k.Gdt - t / L.eta) + T(2) * e;
+        return t + (r * L.eve) * k.dr;
+    }
+    __device__ static T pr(T P, T dv, Loc L, const Kc<T>& k) { return P - (dv * L.eve) * k.dPr; }
+    __device__ static T vcoef_field(T em, const Kc<T>& k) { return visc_ve(em, k.Gs, k.dts); }
+    __device__ static T vcoef_scalar(const Kc<T>& k) { return k.eve_s; }
+    __device__ static T vel(T v, T r, T c, const Kc<T>& k) { return v + (r * k.nu) / c; }
+};
+
+template <class T>
+struct Arith<T, false> {
+    struct Loc { T a, b; };   // a = eta_ve dtau_r, b = eta_ve dtau_Pr
+    __device__ static Loc field(T eta, const Kc<T>& k) {
+        const T eve = eta / fma(eta, k.iGdt, T(1));
+        return Loc{eve * k.dr, eve * k.dPr};
+    }
+    __device__ static Loc scalar(const Kc<T>& k) { return Loc{k.a_s, k.b_s}; }
+    __device__ static T diag(T t, T to, T e, T dV, Loc L, const Kc<T>& k) {
+        return fma(t, k.omdr, L.a * fma(to, k.iGdt, T(2) * fma(-dV, k.third, e)));
+    }
+    __device__ static T off(T t, T to, T e, Loc L, const Kc<T>& k) {
+        return fma(t, k.omdr, L.a * fma(to, k.iGdt, T(2) * e));
+    }
+    __device__ static T pr(T P, T dv, Loc L, const Kc<T>& k) { return fma(-dv, L.b, P); }
+    __device__ static T vcoef_field(T em, const Kc<T>& k) { return k.nu * (T(1) / em + k.iGdt); }
+    __device__ static T vcoef_scalar(const Kc<T>& k) { return k.cv_s; }
+    __device__ static T vel(T v, T r, T c, const Kc<T>& k) { return fma(r, c, v); }
+};
+
 template <class T>
 __device__ __forceinline__ T lerp4(T a0, T a1, T b0, T b1) {
     // knots (lo,lo),(hi,lo),(lo,hi),(hi,hi); first dim innermost (interpolation.jl:37-56)
@@ -54,139 +127,170 @@ __device__ __forceinline__ T lerp4(T a0, T a1, T b0, T b1) {
     return fma(T(0.5), B - A, A);
 }
 
-template <class T, bool FVISC, int BY>
-__global__ void __launch_bounds__(kFX * BY) fused3d_kernel(FusedArgs<T> a) {
+// Block id -> tile, so that the blocks an XCD receives (ids congruent mod 8)
+// form one contiguous run of tiles and share halo rows in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int per = nb / 8;
+    if (bid >= 8 * per) return bid;
+    return (bid % 8) * per + bid / 8;
+}
+
+template <class T, bool FVISC, bool EXACT, int BY>
+__global__ void __launch_bounds__(kFX * BY) fused3d_kernel(Fused3<T> a) {
+    using AR = Arith<T, EXACT>;
+    const Kc<T>& K = a.k;
     __shared__ T s_vx[BY][kFX], s_vy[BY][kFX], s_vz1[BY][kFX];
-    __shared__ T s_eta[FVISC ? BY : 1][kFX], s_eta1[FVISC ? BY : 1][kFX], s_eml[FVISC ? BY : 1][kFX];
+    __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? BY : 1][kFX], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][kFX];
     __shared__ T s_P[BY][kFX], s_txx[BY][kFX], s_tyy[BY][kFX], s_txy[BY][kFX];
     __shared__ T s_txz[2][BY][kFX], s_tyz[2][BY][kFX];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int64_t i = (int64_t)blockIdx.x * (kFX - 2) - 1 + tx;
-    const int64_t j = (int64_t)blockIdx.y * (BY - 2) - 1 + ty;
-    const int64_t nb0 = a.n[0] + 1, nb1 = a.n[1] + 1, nb2 = a.n[2] + 1;   // box is [0, n+1]
-    const int64_t kb = (int64_t)blockIdx.z * a.kchunk;
-    const int64_t ke = min(kb + (int64_t)a.kchunk, nb2 + 1);
+    const int nb = a.ntx * a.nty * a.nchunks;
+    const int lt = xcd_remap(blockIdx.x, nb);
+    const int bx = lt % a.ntx, by = (lt / a.ntx) % a.nty, bz = lt / (a.ntx * a.nty);
+    const int i = bx * (kFX - 2) - 1 + tx;
+    const int j = by * (BY - 2) - 1 + ty;
+    const int nb0 = a.n[0] + 1, nb1 = a.n[1] + 1, nb2 = a.n[2] + 1;   // box is [0, n+1]
+    const int kb = bz * a.kchunk;
+    const int ke = min(kb + a.kchunk, nb2 + 1);
     // storage holds logical [-1, n+2] (Center) / [-1, n+3] (Vertex) per dim
     const bool valid = i <= a.n[0] + 2 && j <= a.n[1] + 2;
     const bool inxy = i >= 0 && i <= nb0 && j >= 0 && j <= nb1;
     const bool own = tx >= 1 && tx <= kFX - 2 && ty >= 1 && ty <= BY - 2 && inxy;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
+    constexpr uint32_t E = sizeof(T);
 
-    auto O = [i, j](DF<T> f, int64_t k) { return i + j * f.sy + k * f.sz; };
-    auto ld = [i, j, valid](DF<T> f, int64_t k) -> T { return valid ? f.p[i + j * f.sy + k * f.sz] : T(0); };
+    // byte offsets at plane k = kb - 1 and the per-step z increments
+    uint32_t o[NLOC], dz[NLOC];
+#pragma unroll
+    for (int c = 0; c < NLOC; ++c) {
+        const LocStride s = a.L[c];
+        o[c] = valid ? (s.o0 + (uint32_t)i + (uint32_t)j * s.sy + (uint32_t)(kb - 1) * s.sz) * E : 0u;
+        dz[c] = s.sz * E;
+    }
+    auto ld = [valid](const T* p, uint32_t off) -> T { return valid ? ldo(p, off) : T(0); };
 
-    // registers carried along z (own column)
-    T vx_k = ld(a.V[0], kb - 1), vy_k = ld(a.V[1], kb - 1), vz_k = ld(a.V[2], kb - 1);
-    T eta_k = FVISC ? ld(a.eta, kb - 1) : T(0);
-    T eml_km1 = FVISC ? ld(a.eml, kb - 1) : T(0);
+    T vx_k = ld(a.Vx, o[LVX]), vy_k = ld(a.Vy, o[LVY]), vz_k = ld(a.Vz, o[LVZ]);
+    T eta_k = T(0), eml_km1 = T(0);
+    if constexpr (FVISC) {
+        eta_k = ld(a.eta, o[LC]);
+        s_eta[0][ty][tx] = eta_k;
+    }
     T P_km1 = T(0), tzz_km1 = T(0), txz_k = T(0), tyz_k = T(0);
-    int par = 0;
+    int p = 0;
 
-    for (int64_t k = kb - 1; k < ke; ++k) {
-        const bool pro = k < kb;          // prologue: plane kb-1 feeds the first step
-        // ---------------- A: loads of plane k+1, V/eta planes to LDS
-        const T vx_k1 = ld(a.V[0], k + 1), vy_k1 = ld(a.V[1], k + 1), vz_k1 = ld(a.V[2], k + 1);
+    for (int k = kb - 1; k < ke; ++k) {
+        const bool pro = k < kb;
+        // ---------------- A: plane k+1 loads; V and eta planes to LDS
+        const T vx_k1 = ld(a.Vx, o[LVX] + dz[LVX]);
+        const T vy_k1 = ld(a.Vy, o[LVY] + dz[LVY]);
+        const T vz_k1 = ld(a.Vz, o[LVZ] + dz[LVZ]);
         T eta_k1 = T(0), eml_k = T(0);
         s_vx[ty][tx] = vx_k;
         s_vy[ty][tx] = vy_k;
         s_vz1[ty][tx] = vz_k1;
         if constexpr (FVISC) {
-            eta_k1 = ld(a.eta, k + 1);
-            eml_k = ld(a.eml, k);
-            s_eta[ty][tx] = eta_k;
-            s_eta1[ty][tx] = eta_k1;
-            s_eml[ty][tx] = eml_k;
+            eta_k1 = ld(a.eta, o[LC] + dz[LC]);
+            eml_k = ld(a.eml, o[LC]);
+            s_eta[p ^ 1][ty][tx] = eta_k1;
+            s_eml[p][ty][tx] = eml_k;
+        }
+        // stress-state loads for this step (independent of LDS)
+        T P = ld(a.Pr, o[LC]), txx = ld(a.txx, o[LC]), tyy = ld(a.tyy, o[LC]), tzz = ld(a.tzz, o[LC]);
+        T txy = ld(a.txy, o[LXY]);
+        T txz1 = ld(a.txz, o[LXZ] + dz[LXZ]), tyz1 = ld(a.tyz, o[LYZ] + dz[LYZ]);
+        const bool in_k = inxy && k >= 0;
+        const bool in_k1 = inxy && k + 1 <= nb2;
+        T oxx = T(0), oyy = T(0), ozz = T(0), oxy = T(0), oxz = T(0), oyz = T(0);
+        if (in_k) {
+            oxx = ldo(a.Oxx, o[LC]);
+            oyy = ldo(a.Oyy, o[LC]);
+            ozz = ldo(a.Ozz, o[LC]);
+            oxy = ldo(a.Oxy, o[LXY]);
+        }
+        if (in_k1) {
+            oxz = ldo(a.Oxz, o[LXZ] + dz[LXZ]);
+            oyz = ldo(a.Oyz, o[LYZ] + dz[LYZ]);
         }
         __syncthreads();
 
         // ---------------- B: stresses
-        const bool in_k = inxy && k >= 0 && k <= nb2;
-        const bool in_k1 = inxy && k + 1 <= nb2;
         const int tx1 = tx < kFX - 1 ? tx + 1 : tx, ty1 = ty < BY - 1 ? ty + 1 : ty;
         const int txm = tx > 0 ? tx - 1 : tx, tym = ty > 0 ? ty - 1 : ty;
-        // Center plane k
-        T exx = (s_vx[ty][tx1] - vx_k) * ix;
-        T eyy = (s_vy[ty1][tx] - vy_k) * iy;
-        T ezz = (vz_k1 - vz_k) * iz;
-        T dv = (exx + eyy) + ezz;
-        T P = ld(a.Pr, k), txx = ld(a.tau[0], k), tyy = ld(a.tau[1], k), tzz = ld(a.tau[2], k);
-        T txy = ld(a.tau[3], k);
-        T txz1 = ld(a.tau[4], k + 1), tyz1 = ld(a.tau[5], k + 1);
-        T eta_c = a.eta_s, eve_c = a.eve_s;
-        if constexpr (FVISC) {
-            eta_c = eta_k;
-            eve_c = visc_ve(eta_c, a.Gs, a.dts);
-        }
+        const T exx = (s_vx[ty][tx1] - vx_k) * ix;
+        const T eyy = (s_vy[ty1][tx] - vy_k) * iy;
+        const T ezz = (vz_k1 - vz_k) * iz;
+        const T dv = (exx + eyy) + ezz;
         if (in_k) {
-            P = P - (dv * eve_c) * a.dPr;
-            txx = f_diag(txx, ld(a.told[0], k), exx, dv, eta_c, eve_c, a.Gdt, a.dr);
-            tyy = f_diag(tyy, ld(a.told[1], k), eyy, dv, eta_c, eve_c, a.Gdt, a.dr);
-            tzz = f_diag(tzz, ld(a.told[2], k), ezz, dv, eta_c, eve_c, a.Gdt, a.dr);
-            T exy = T(0.5) * ((vx_k - s_vx[tym][tx]) * iy + (vy_k - s_vy[ty][txm]) * ix);
-            T et = a.eta_s, ev = a.eve_s;
+            typename AR::Loc Lc, Le;
             if constexpr (FVISC) {
-                et = lerp4(s_eta[tym][txm], s_eta[tym][tx], s_eta[ty][txm], eta_k);
-                ev = visc_ve(et, a.Gs, a.dts);
+                Lc = AR::field(eta_k, K);
+                Le = AR::field(lerp4(s_eta[p][tym][txm], s_eta[p][tym][tx], s_eta[p][ty][txm], eta_k), K);
+            } else {
+                Lc = Le = AR::scalar(K);
             }
-            txy = f_off(txy, ld(a.told[3], k), exy, et, ev, a.Gdt, a.dr);
+            P = AR::pr(P, dv, Lc, K);
+            txx = AR::diag(txx, oxx, exx, dv, Lc, K);
+            tyy = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+            tzz = AR::diag(tzz, ozz, ezz, dv, Lc, K);
+            const T exy = T(0.5) * ((vx_k - s_vx[tym][tx]) * iy + (vy_k - s_vy[ty][txm]) * ix);
+            txy = AR::off(txy, oxy, exy, Le, K);
         }
         if (in_k1) {
-            T exz = T(0.5) * ((vx_k1 - vx_k) * iz + (vz_k1 - s_vz1[ty][txm]) * ix);
-            T eyz = T(0.5) * ((vy_k1 - vy_k) * iz + (vz_k1 - s_vz1[tym][tx]) * iy);
-            T et4 = a.eta_s, ev4 = a.eve_s, et5 = a.eta_s, ev5 = a.eve_s;
+            const T exz = T(0.5) * ((vx_k1 - vx_k) * iz + (vz_k1 - s_vz1[ty][txm]) * ix);
+            const T eyz = T(0.5) * ((vy_k1 - vy_k) * iz + (vz_k1 - s_vz1[tym][tx]) * iy);
+            typename AR::Loc L4, L5;
             if constexpr (FVISC) {
-                et4 = lerp4(s_eta[ty][txm], eta_k, s_eta1[ty][txm], eta_k1);
-                ev4 = visc_ve(et4, a.Gs, a.dts);
-                et5 = lerp4(s_eta[tym][tx], eta_k, s_eta1[tym][tx], eta_k1);
-                ev5 = visc_ve(et5, a.Gs, a.dts);
+                L4 = AR::field(lerp4(s_eta[p][ty][txm], eta_k, s_eta[p ^ 1][ty][txm], eta_k1), K);
+                L5 = AR::field(lerp4(s_eta[p][tym][tx], eta_k, s_eta[p ^ 1][tym][tx], eta_k1), K);
+            } else {
+                L4 = L5 = AR::scalar(K);
             }
-            txz1 = f_off(txz1, ld(a.told[4], k + 1), exz, et4, ev4, a.Gdt, a.dr);
-            tyz1 = f_off(tyz1, ld(a.told[5], k + 1), eyz, et5, ev5, a.Gdt, a.dr);
+            txz1 = AR::off(txz1, oxz, exz, L4, K);
+            tyz1 = AR::off(tyz1, oyz, eyz, L5, K);
         }
         if (!pro && own && in_k) {
-            const int64_t oc = O(a.Pro, k);
-            a.Pro.at(oc) = P;
-            a.tauo_[0].at(O(a.tauo_[0], k)) = txx;
-            a.tauo_[1].at(O(a.tauo_[1], k)) = tyy;
-            a.tauo_[2].at(O(a.tauo_[2], k)) = tzz;
-            a.tauo_[3].at(O(a.tauo_[3], k)) = txy;
-            a.tauo_[4].at(O(a.tauo_[4], k)) = txz_k;
-            a.tauo_[5].at(O(a.tauo_[5], k)) = tyz_k;
-            if (a.write_res) a.dV.at(O(a.dV, k)) = dv;
+            sto(a.oPr, o[LC], P);
+            sto(a.otxx, o[LC], txx);
+            sto(a.otyy, o[LC], tyy);
+            sto(a.otzz, o[LC], tzz);
+            sto(a.otxy, o[LXY], txy);
+            sto(a.otxz, o[LXZ], txz_k);
+            sto(a.otyz, o[LYZ], tyz_k);
+            if (a.write_res) sto(a.dV, o[LC], dv);
         }
         s_P[ty][tx] = P;
         s_txx[ty][tx] = txx;
         s_tyy[ty][tx] = tyy;
         s_txy[ty][tx] = txy;
-        s_txz[par][ty][tx] = txz1;
-        s_tyz[par][ty][tx] = tyz1;
+        s_txz[p][ty][tx] = txz1;
+        s_tyz[p][ty][tx] = tyz1;
         __syncthreads();
 
         // ---------------- C: velocity at plane k
-        if (!pro) {
+        if (!pro && own && in_k) {
             const T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) +
                           (s_txy[ty1][tx] - txy) * iy) + (txz1 - txz_k) * iz;
             const T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) +
                           (s_txy[ty][tx1] - txy) * ix) + (tyz1 - tyz_k) * iz;
-            T rz = ((-((P - P_km1) * iz) + (tzz - tzz_km1) * iz) + (s_txz[par ^ 1][ty][tx1] - txz_k) * ix) +
-                   (s_tyz[par ^ 1][ty1][tx] - tyz_k) * iy;
-            if (own && in_k) {
-                if (a.has_rg) rz = rz - a.rg.at(O(a.rg, k));
-                T ex = a.eve_s, ey = a.eve_s, ez = a.eve_s;
-                if constexpr (FVISC) {
-                    ex = visc_ve(jmax(s_eml[ty][txm], eml_k), a.Gs, a.dts);
-                    ey = visc_ve(jmax(s_eml[tym][tx], eml_k), a.Gs, a.dts);
-                    ez = visc_ve(jmax(eml_km1, eml_k), a.Gs, a.dts);
-                }
-                a.Vo[0].at(O(a.Vo[0], k)) = vx_k + (rx * a.nu) / ex;
-                a.Vo[1].at(O(a.Vo[1], k)) = vy_k + (ry * a.nu) / ey;
-                a.Vo[2].at(O(a.Vo[2], k)) = vz_k + (rz * a.nu) / ez;
-                if (a.write_res) {
-                    a.rV[0].at(O(a.rV[0], k)) = rx;
-                    a.rV[1].at(O(a.rV[1], k)) = ry;
-                    a.rV[2].at(O(a.rV[2], k)) = rz;
-                }
+            T rz = ((-((P - P_km1) * iz) + (tzz - tzz_km1) * iz) + (s_txz[p ^ 1][ty][tx1] - txz_k) * ix) +
+                   (s_tyz[p ^ 1][ty1][tx] - tyz_k) * iy;
+            if (a.has_rg) rz = rz - ldo(a.rg, o[LVZ]);
+            T ex, ey, ez;
+            if constexpr (FVISC) {
+                ex = AR::vcoef_field(jmax(s_eml[p][ty][txm], eml_k), K);
+                ey = AR::vcoef_field(jmax(s_eml[p][tym][tx], eml_k), K);
+                ez = AR::vcoef_field(jmax(eml_km1, eml_k), K);
+            } else {
+                ex = ey = ez = AR::vcoef_scalar(K);
+            }
+            sto(a.oVx, o[LVX], AR::vel(vx_k, rx, ex, K));
+            sto(a.oVy, o[LVY], AR::vel(vy_k, ry, ey, K));
+            sto(a.oVz, o[LVZ], AR::vel(vz_k, rz, ez, K));
+            if (a.write_res) {
+                sto(a.rVx, o[LVX], rx);
+                sto(a.rVy, o[LVY], ry);
+                sto(a.rVz, o[LVZ], rz);
             }
         }
         // ---------------- rotate the z pipeline
@@ -199,14 +303,32 @@ __global__ void __launch_bounds__(kFX * BY) fused3d_kernel(FusedArgs<T> a) {
         vz_k = vz_k1;
         eta_k = eta_k1;
         eml_km1 = eml_k;
-        par ^= 1;
-        __syncthreads();
+        p ^= 1;
+#pragma unroll
+        for (int c = 0; c < NLOC; ++c) o[c] += dz[c];
     }
 }
 
 // 2D: one tile per block, no z pipeline.
-template <class T, bool FVISC, int BY>
-__global__ void __launch_bounds__(kFX * BY) fused2d_kernel(FusedArgs<T> a) {
+template <class T>
+struct DF2 {
+    T* p;
+    int64_t sy;
+};
+
+template <class T>
+struct Fused2 {
+    DF2<T> V[2], tau[3], Pr, Vo[2], tauo[3], Pro, told[3], eta, eml, rg, dV, rV[2];
+    int64_t n[2];
+    int32_t has_rg, write_res;
+    T ix, iy;
+    Kc<T> k;
+};
+
+template <class T, bool FVISC, bool EXACT, int BY>
+__global__ void __launch_bounds__(kFX * BY) fused2d_kernel(Fused2<T> a) {
+    using AR = Arith<T, EXACT>;
+    const Kc<T>& K = a.k;
     __shared__ T s_vx[BY][kFX], s_vy[BY][kFX];
     __shared__ T s_eta[FVISC ? BY : 1][kFX], s_eml[FVISC ? BY : 1][kFX];
     __shared__ T s_P[BY][kFX], s_txx[BY][kFX], s_tyy[BY][kFX], s_txy[BY][kFX];
@@ -218,8 +340,8 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(FusedArgs<T> a) {
     const bool inxy = i >= 0 && i <= nb0 && j >= 0 && j <= nb1;
     const bool own = tx >= 1 && tx <= kFX - 2 && ty >= 1 && ty <= BY - 2 && inxy;
     const T ix = a.ix, iy = a.iy;
-    auto O = [i, j](DF<T> f) { return i + j * f.sy; };
-    auto ld = [i, j, valid](DF<T> f) -> T { return valid ? f.p[i + j * f.sy] : T(0); };
+    auto O = [i, j](DF2<T> f) { return i + j * f.sy; };
+    auto ld = [i, j, valid](DF2<T> f) -> T { return valid ? f.p[i + j * f.sy] : T(0); };
 
     const T vx = ld(a.V[0]), vy = ld(a.V[1]);
     T eta = T(0), eml = T(0);
@@ -231,33 +353,33 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(FusedArgs<T> a) {
         s_eta[ty][tx] = eta;
         s_eml[ty][tx] = eml;
     }
+    T P = ld(a.Pr), txx = ld(a.tau[0]), tyy = ld(a.tau[1]), txy = ld(a.tau[2]);
     __syncthreads();
     const int tx1 = tx < kFX - 1 ? tx + 1 : tx, ty1 = ty < BY - 1 ? ty + 1 : ty;
     const int txm = tx > 0 ? tx - 1 : tx, tym = ty > 0 ? ty - 1 : ty;
-    T exx = (s_vx[ty][tx1] - vx) * ix;
-    T eyy = (s_vy[ty1][tx] - vy) * iy;
-    T dv = exx + eyy;
-    T P = ld(a.Pr), txx = ld(a.tau[0]), tyy = ld(a.tau[1]), txy = ld(a.tau[2]);
+    const T exx = (s_vx[ty][tx1] - vx) * ix;
+    const T eyy = (s_vy[ty1][tx] - vy) * iy;
+    const T dv = exx + eyy;
     if (inxy) {
-        T eta_c = a.eta_s, eve_c = a.eve_s, et = a.eta_s, ev = a.eve_s;
+        typename AR::Loc Lc, Le;
         if constexpr (FVISC) {
-            eta_c = eta;
-            eve_c = visc_ve(eta_c, a.Gs, a.dts);
-            et = lerp4(s_eta[tym][txm], s_eta[tym][tx], s_eta[ty][txm], eta);
-            ev = visc_ve(et, a.Gs, a.dts);
+            Lc = AR::field(eta, K);
+            Le = AR::field(lerp4(s_eta[tym][txm], s_eta[tym][tx], s_eta[ty][txm], eta), K);
+        } else {
+            Lc = Le = AR::scalar(K);
         }
-        P = P - (dv * eve_c) * a.dPr;
-        txx = f_diag(txx, ld(a.told[0]), exx, dv, eta_c, eve_c, a.Gdt, a.dr);
-        tyy = f_diag(tyy, ld(a.told[1]), eyy, dv, eta_c, eve_c, a.Gdt, a.dr);
-        T exy = T(0.5) * ((vx - s_vx[tym][tx]) * iy + (vy - s_vy[ty][txm]) * ix);
-        txy = f_off(txy, ld(a.told[2]), exy, et, ev, a.Gdt, a.dr);
+        P = AR::pr(P, dv, Lc, K);
+        txx = AR::diag(txx, ld(a.told[0]), exx, dv, Lc, K);
+        tyy = AR::diag(tyy, ld(a.told[1]), eyy, dv, Lc, K);
+        const T exy = T(0.5) * ((vx - s_vx[tym][tx]) * iy + (vy - s_vy[ty][txm]) * ix);
+        txy = AR::off(txy, ld(a.told[2]), exy, Le, K);
     }
     if (own) {
-        a.Pro.at(O(a.Pro)) = P;
-        a.tauo_[0].at(O(a.tauo_[0])) = txx;
-        a.tauo_[1].at(O(a.tauo_[1])) = tyy;
-        a.tauo_[2].at(O(a.tauo_[2])) = txy;
-        if (a.write_res) a.dV.at(O(a.dV)) = dv;
+        a.Pro.p[O(a.Pro)] = P;
+        a.tauo[0].p[O(a.tauo[0])] = txx;
+        a.tauo[1].p[O(a.tauo[1])] = tyy;
+        a.tauo[2].p[O(a.tauo[2])] = txy;
+        if (a.write_res) a.dV.p[O(a.dV)] = dv;
     }
     s_P[ty][tx] = P;
     s_txx[ty][tx] = txx;
@@ -267,17 +389,19 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(FusedArgs<T> a) {
     if (own) {
         const T rx = (-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) + (s_txy[ty1][tx] - txy) * iy;
         T ry = (-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[ty][tx1] - txy) * ix;
-        if (a.has_rg) ry = ry - a.rg.at(O(a.rg));
-        T ex = a.eve_s, ey = a.eve_s;
+        if (a.has_rg) ry = ry - a.rg.p[O(a.rg)];
+        T ex, ey;
         if constexpr (FVISC) {
-            ex = visc_ve(jmax(s_eml[ty][txm], eml), a.Gs, a.dts);
-            ey = visc_ve(jmax(s_eml[tym][tx], eml), a.Gs, a.dts);
+            ex = AR::vcoef_field(jmax(s_eml[ty][txm], eml), K);
+            ey = AR::vcoef_field(jmax(s_eml[tym][tx], eml), K);
+        } else {
+            ex = ey = AR::vcoef_scalar(K);
         }
-        a.Vo[0].at(O(a.Vo[0])) = vx + (rx * a.nu) / ex;
-        a.Vo[1].at(O(a.Vo[1])) = vy + (ry * a.nu) / ey;
+        a.Vo[0].p[O(a.Vo[0])] = AR::vel(vx, rx, ex, K);
+        a.Vo[1].p[O(a.Vo[1])] = AR::vel(vy, ry, ey, K);
         if (a.write_res) {
-            a.rV[0].at(O(a.rV[0])) = rx;
-            a.rV[1].at(O(a.rV[1])) = ry;
+            a.rV[0].p[O(a.rV[0])] = rx;
+            a.rV[1].p[O(a.rV[1])] = ry;
         }
     }
 }
@@ -285,60 +409,131 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(FusedArgs<T> a) {
 constexpr int kBY3 = 8, kBY2 = 8, kZChunk = 32;
 
 template <class T>
+static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
+    Kc<T> k;
+    k.Gs = T(G);
+    k.dts = T(dt);
+    k.Gdt = T(G) * T(dt);
+    k.dPr = T(dPr);
+    k.dr = T(dr);
+    k.nu = T(nu);
+    k.eta_s = T(eta_s);
+    k.eve_s = T(eve_s);
+    k.iGdt = T(1.0 / (G * dt));
+    k.omdr = T(1.0 - dr);
+    k.third = T(1.0 / 3.0);
+    k.a_s = T(eve_s * dr);
+    k.b_s = T(eve_s * dPr);
+    k.cv_s = T(nu / eve_s);
+    return k;
+}
+
+
+template <class T>
+static DF2<T> v2(const chmy_field_t& f) {
+    DF<T> d = view<T>(f);
+    return DF2<T>{d.p, d.sy};
+}
+
+static LocStride loc_stride(const chmy_field_t& f) {
+    const int64_t p0 = pad_of(f, 0), p1 = pad_of(f, 1);
+    LocStride s;
+    s.sy = (uint32_t)p0;
+    s.sz = (uint32_t)(p0 * p1);
+    s.o0 = (uint32_t)((2 * f.halo[0] - 1) + (2 * f.halo[1] - 1) * p0 + (2 * f.halo[2] - 1) * p0 * p1);
+    return s;
+}
+
+template <class T>
 static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
                         const chmy_field_t* Vo, const chmy_field_t* tauo, const chmy_field_t* Pro,
                         const chmy_field_t* told, const chmy_field_t* eta, const chmy_field_t* eml,
-                        const chmy_field_t* rg, const chmy_field_t* dV, const chmy_field_t* rV, int wres,
+                        const chmy_field_t* rg, const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
                         hipStream_t s) {
-    FusedArgs<T> a;
-    std::memset(&a, 0, sizeof a);
-    const int nd = g->nd, nt = nd == 2 ? 3 : 6;
-    for (int d = 0; d < nd; ++d) {
-        a.V[d] = view<T>(V[d]);
-        a.Vo[d] = view<T>(Vo[d]);
-        a.rV[d] = wres ? view<T>(rV[d]) : a.Vo[d];
-    }
-    for (int c = 0; c < nt; ++c) {
-        a.tau[c] = view<T>(tau[c]);
-        a.tauo_[c] = view<T>(tauo[c]);
-        a.told[c] = view<T>(told[c]);
-    }
-    a.Pr = view<T>(*Pr);
-    a.Pro = view<T>(*Pro);
-    a.eta = eta ? view<T>(*eta) : a.Pr;
-    a.eml = eml ? view<T>(*eml) : a.Pr;
-    a.rg = rg ? view<T>(*rg) : a.Pr;
-    a.dV = wres ? view<T>(*dV) : a.Pro;
-    for (int d = 0; d < 3; ++d) a.n[d] = d < nd ? g->n[d] : 0;
-    a.ix = T(g->inv_spacing[0]);
-    a.iy = T(g->inv_spacing[1]);
-    a.iz = T(g->inv_spacing[2]);
-    a.eta_s = T(eta_s);
-    a.eve_s = T(eve_s);
-    a.Gs = T(G);
-    a.dts = T(dt);
-    a.Gdt = T(G) * T(dt);
-    a.dPr = T(dPr);
-    a.dr = T(dr);
-    a.nu = T(nu);
-    a.has_rg = rg != nullptr;
-    a.write_res = wres;
     const bool fv = eta != nullptr;
-    if (nd == 3) {
+    if (g->nd == 3) {
+        // every byte offset a thread forms must fit in 32 bits
+        const chmy_field_t* all[] = {&V[0], &V[1], &V[2], &tau[3], &tau[4], &tau[5], Pr};
+        for (const chmy_field_t* f : all) {
+            int64_t el = pad_of(*f, 0) * pad_of(*f, 1) * pad_of(*f, 2);
+            if (el * (int64_t)sizeof(T) >= (int64_t)UINT32_MAX)
+                return set_error(CHMY_E_SIZE, "fused iteration: a field exceeds 4 GiB (32-bit offsets)");
+        }
+        Fused3<T> a;
+        std::memset(&a, 0, sizeof a);
+        auto P = [](const chmy_field_t& f) { return static_cast<T*>(f.data); };
+        a.Vx = P(V[0]); a.Vy = P(V[1]); a.Vz = P(V[2]); a.Pr = P(*Pr);
+        a.txx = P(tau[0]); a.tyy = P(tau[1]); a.tzz = P(tau[2]); a.txy = P(tau[3]); a.txz = P(tau[4]); a.tyz = P(tau[5]);
+        a.oVx = P(Vo[0]); a.oVy = P(Vo[1]); a.oVz = P(Vo[2]); a.oPr = P(*Pro);
+        a.otxx = P(tauo[0]); a.otyy = P(tauo[1]); a.otzz = P(tauo[2]);
+        a.otxy = P(tauo[3]); a.otxz = P(tauo[4]); a.otyz = P(tauo[5]);
+        a.Oxx = P(told[0]); a.Oyy = P(told[1]); a.Ozz = P(told[2]);
+        a.Oxy = P(told[3]); a.Oxz = P(told[4]); a.Oyz = P(told[5]);
+        a.eta = fv ? P(*eta) : a.Pr;
+        a.eml = fv ? P(*eml) : a.Pr;
+        a.rg = rg ? P(*rg) : a.Pr;
+        a.dV = wres ? P(*dV) : a.oPr;
+        a.rVx = wres ? P(rV[0]) : a.oVx;
+        a.rVy = wres ? P(rV[1]) : a.oVy;
+        a.rVz = wres ? P(rV[2]) : a.oVz;
+        a.L[LC] = loc_stride(*Pr);
+        a.L[LVX] = loc_stride(V[0]);
+        a.L[LVY] = loc_stride(V[1]);
+        a.L[LVZ] = loc_stride(V[2]);
+        a.L[LXY] = loc_stride(tau[3]);
+        a.L[LXZ] = loc_stride(tau[4]);
+        a.L[LYZ] = loc_stride(tau[5]);
+        for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
         a.kchunk = kZChunk;
-        const int64_t nbx = g->n[0] + 2, nby = g->n[1] + 2, nbz = g->n[2] + 2;
-        dim3 blk(kFX, kBY3, 1);
-        dim3 grd((unsigned)((nbx + kFX - 3) / (kFX - 2)), (unsigned)((nby + kBY3 - 3) / (kBY3 - 2)),
-                 (unsigned)((nbz + kZChunk - 1) / kZChunk));
-        if (fv) hipLaunchKernelGGL((fused3d_kernel<T, true, kBY3>), grd, blk, 0, s, a);
-        else hipLaunchKernelGGL((fused3d_kernel<T, false, kBY3>), grd, blk, 0, s, a);
+        a.ntx = (int32_t)((g->n[0] + 2 + kFX - 3) / (kFX - 2));
+        a.nty = (int32_t)((g->n[1] + 2 + kBY3 - 3) / (kBY3 - 2));
+        a.nchunks = (int32_t)((g->n[2] + 2 + kZChunk - 1) / kZChunk);
+        a.has_rg = rg != nullptr;
+        a.write_res = wres;
+        a.ix = T(g->inv_spacing[0]);
+        a.iy = T(g->inv_spacing[1]);
+        a.iz = T(g->inv_spacing[2]);
+        a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
+        const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
+        dim3 blk(kFX, kBY3, 1), grd(nb, 1, 1);
+        if (fv && exact) hipLaunchKernelGGL((fused3d_kernel<T, true, true, kBY3>), grd, blk, 0, s, a);
+        else if (fv) hipLaunchKernelGGL((fused3d_kernel<T, true, false, kBY3>), grd, blk, 0, s, a);
+        else if (exact) hipLaunchKernelGGL((fused3d_kernel<T, false, true, kBY3>), grd, blk, 0, s, a);
+        else hipLaunchKernelGGL((fused3d_kernel<T, false, false, kBY3>), grd, blk, 0, s, a);
     } else {
+        Fused2<T> a;
+        std::memset(&a, 0, sizeof a);
+        for (int d = 0; d < 2; ++d) {
+            a.V[d] = v2<T>(V[d]);
+            a.Vo[d] = v2<T>(Vo[d]);
+            a.rV[d] = wres ? v2<T>(rV[d]) : a.Vo[d];
+        }
+        for (int c = 0; c < 3; ++c) {
+            a.tau[c] = v2<T>(tau[c]);
+            a.tauo[c] = v2<T>(tauo[c]);
+            a.told[c] = v2<T>(told[c]);
+        }
+        a.Pr = v2<T>(*Pr);
+        a.Pro = v2<T>(*Pro);
+        a.eta = fv ? v2<T>(*eta) : a.Pr;
+        a.eml = fv ? v2<T>(*eml) : a.Pr;
+        a.rg = rg ? v2<T>(*rg) : a.Pr;
+        a.dV = wres ? v2<T>(*dV) : a.Pro;
+        a.n[0] = g->n[0];
+        a.n[1] = g->n[1];
+        a.has_rg = rg != nullptr;
+        a.write_res = wres;
+        a.ix = T(g->inv_spacing[0]);
+        a.iy = T(g->inv_spacing[1]);
+        a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const int64_t nbx = g->n[0] + 2, nby = g->n[1] + 2;
         dim3 blk(kFX, kBY2, 1);
         dim3 grd((unsigned)((nbx + kFX - 3) / (kFX - 2)), (unsigned)((nby + kBY2 - 3) / (kBY2 - 2)), 1);
-        if (fv) hipLaunchKernelGGL((fused2d_kernel<T, true, kBY2>), grd, blk, 0, s, a);
-        else hipLaunchKernelGGL((fused2d_kernel<T, false, kBY2>), grd, blk, 0, s, a);
+        if (fv && exact) hipLaunchKernelGGL((fused2d_kernel<T, true, true, kBY2>), grd, blk, 0, s, a);
+        else if (fv) hipLaunchKernelGGL((fused2d_kernel<T, true, false, kBY2>), grd, blk, 0, s, a);
+        else if (exact) hipLaunchKernelGGL((fused2d_kernel<T, false, true, kBY2>), grd, blk, 0, s, a);
+        else hipLaunchKernelGGL((fused2d_kernel<T, false, false, kBY2>), grd, blk, 0, s, a);
     }
     CHMY_HIP(hipGetLastError());
     return 0;
@@ -353,11 +548,14 @@ extern "C" {
 int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
                      const chmy_field_t* V_out, const chmy_field_t* tau_out, const chmy_field_t* Pr_out,
                      const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* eta_ml,
-                     const chmy_field_t* rhog, const chmy_field_t* divV, const chmy_field_t* r_V, int write_residual,
+                     const chmy_field_t* rhog, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                      double nudtau, void* stream) {
     CHMY_TRY(check_grid(g));
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
+    if (flags & ~(CHMY_ITER_RESIDUAL | CHMY_ITER_EXACT)) return set_error(CHMY_E_ARG, "unknown flags %d", flags);
+    const int write_residual = (flags & CHMY_ITER_RESIDUAL) != 0;
+    const bool exact = (flags & CHMY_ITER_EXACT) != 0;
     CHMY_TRY(check_center(g, Pr, "Pr", -1));
     const int dt_ = Pr->dtype;
     CHMY_TRY(check_center(g, Pr_out, "Pr_out", dt_));
@@ -380,17 +578,32 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
         CHMY_TRY(check_center(g, divV, "divV", dt_));
         CHMY_TRY(check_vector(g, r_V, "r_V", dt_));
     }
-    for (int d = 0; d < g->nd; ++d)
-        for (const chmy_field_t* f : {Pr, Pr_out, eta, eta_ml})
-            if (f && f->halo[d] != 1) return set_error(CHMY_E_HALO, "fused iteration needs halo 1");
-    if (V[0].data == V_out[0].data || Pr->data == Pr_out->data || tau[0].data == tau_out[0].data)
-        return set_error(CHMY_E_ARG, "fused iteration needs distinct input and output buffers");
+    // one stride set per staggered location: every field must have halo 1
+    const int nd = g->nd, nt = nd == 2 ? 3 : 6;
+    auto halo1 = [nd](const chmy_field_t* f) {
+        for (int d = 0; d < nd; ++d)
+            if (f->halo[d] != 1) return false;
+        return true;
+    };
+    bool ok = halo1(Pr) && halo1(Pr_out) && (!eta || (halo1(eta) && halo1(eta_ml))) && (!rhog || halo1(rhog)) &&
+              (!write_residual || halo1(divV));
+    for (int d = 0; d < nd; ++d) {
+        ok = ok && halo1(&V[d]) && halo1(&V_out[d]);
+        if (write_residual) ok = ok && halo1(&r_V[d]);
+    }
+    for (int c = 0; c < nt; ++c) ok = ok && halo1(&tau[c]) && halo1(&tau_out[c]) && halo1(&tau_old[c]);
+    if (!ok) return set_error(CHMY_E_HALO, "fused iteration needs halo 1 on every field");
+    for (int d = 0; d < nd; ++d)
+        if (V[d].data == V_out[d].data) return set_error(CHMY_E_ARG, "V and V_out must be distinct buffers");
+    for (int c = 0; c < nt; ++c)
+        if (tau[c].data == tau_out[c].data) return set_error(CHMY_E_ARG, "tau and tau_out must be distinct buffers");
+    if (Pr->data == Pr_out->data) return set_error(CHMY_E_ARG, "Pr and Pr_out must be distinct buffers");
     return dt_ == CHMY_F64
                ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, divV, r_V,
-                                      write_residual, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
+                                      write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
                                       (hipStream_t)stream)
                : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, divV, r_V,
-                                     write_residual, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
+                                     write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
                                      (hipStream_t)stream);
 }
 

@@ -172,16 +172,21 @@ int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const ch
 /* One fused PT iteration = update_stress! then update_velocity! over the
  * Launcher box (examples/stokes_3d_inc_ve_T.jl:222-223), double-buffered:
  * reads (V, tau, Pr) and writes (V_out, tau_out, Pr_out), so no block reads a
- * value another block has already advanced.  The results are those of the two
- * reference kernels in sequence (tau/Pr outside the box keep their stored
- * values).  r_V and divV are written only when write_residual != 0 (they are
- * needed only at convergence checks).  The V boundary batch is applied by the
- * caller afterwards (chmy_bc_batchset on V_out), as Launcher's bc= does. */
+ * value another block has already advanced.  tau/Pr/V outside the box keep
+ * their stored values, so V_out/tau_out/Pr_out must equal the inputs there
+ * (e.g. copies made once before the first iteration).  Every field has halo 1.
+ * flags: CHMY_ITER_RESIDUAL also writes r_V and divV (needed only at
+ * convergence checks); CHMY_ITER_EXACT evaluates the reference's expressions
+ * (results bitwise those of chmy_stokes_stress + chmy_stokes_velocity);
+ * without it the algebraically equal form with 7 instead of 31 divisions per
+ * point is used (a few ulp per iteration).  The V boundary batch is applied
+ * by the caller afterwards (chmy_bc_batchset on V_out), as Launcher's bc= does. */
+enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2 };
 int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                      const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                      const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
                      const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* divV,
-                     const chmy_field_t* r_V, int write_residual, double eta_s, double eta_ve_s, double G,
+                     const chmy_field_t* r_V, int flags, double eta_s, double eta_ve_s, double G,
                      double dt, double dtau_Pr, double dtau_r, double nudtau, void* stream);
 
 /* ---------------------------------------------------------------- reductions */
