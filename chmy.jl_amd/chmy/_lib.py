@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libchmy_mi355x.so")
+LIB_PATH = os.environ.get("CHMY_MI355X_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libchmy_mi355x.so")
 
 F64, F32 = 0, 1
 CENTER, VERTEX = 0, 1

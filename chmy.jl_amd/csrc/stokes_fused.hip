@@ -135,8 +135,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     return (bid % 8) * per + bid / 8;
 }
 
+// Tuning knobs fixed at build time: rows per block and the minimum waves per
+// SIMD the register allocation must allow (0 = no bound).
+#ifndef CHMY_FUSED_BY
+#define CHMY_FUSED_BY 8
+#endif
+#ifndef CHMY_FUSED_WPE
+#define CHMY_FUSED_WPE 0
+#endif
+
 template <class T, bool FVISC, bool EXACT, int BY>
-__global__ void __launch_bounds__(kFX * BY) fused3d_kernel(Fused3<T> a) {
+__global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
     __shared__ T s_vx[BY][kFX], s_vy[BY][kFX], s_vz1[BY][kFX];
@@ -406,7 +415,7 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(Fused2<T> a) {
     }
 }
 
-constexpr int kBY3 = 8, kBY2 = 8, kZChunk = 32;
+constexpr int kBY3 = CHMY_FUSED_BY, kBY2 = 8, kZChunk = 32;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
