@@ -40,13 +40,10 @@ def reference_bytes(n, itemsize=8):
 def setup(n, dtype):
     import torch
     import chmy
-    from chmy.stokes import PTStokes, inclusion_setup_3d, pure_shear_initial, pure_shear_values
+    from chmy import configs
+    from chmy.stokes import PTStokes, pure_shear_initial, pure_shear_values
     arch = chmy.Arch(chmy.MI355XBackend())
-    g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(n,) * 3)
-    dx = min(chmy.spacing(g))
-    lam = 1e-4 * 1.0 ** 2 / 1.0                  # λ_ρCp = 1e-4 lz²/τsc
-    dt = dx ** 2 / lam / 3 / 2.1                 # dt_diff (stokes_3d_inc_ve_T.jl:215)
-    eta = inclusion_setup_3d(g)
+    g, eta, dt = configs.stokes3d_inclusion(arch, n)   # dt_diff (stokes_3d_inc_ve_T.jl:215), λ_ρCp = 1e-4
     S = PTStokes(arch, g, dtype=dtype, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(3))
     pure_shear_initial(g, S.V)
     S.apply_bc()
@@ -100,6 +97,66 @@ def cpu_baseline(n, iters):
     return el, cores
 
 
+def time_to_converge(args):
+    """Time to an f64 residual below eps: plain f64 PT (fused kernel) versus
+    f32 PT sweeps with f64 defect correction every `inner` iterations, on the
+    same GPU and setup; also the per-field error of the mixed solution
+    against the f64 one (max and L2, relative)."""
+    import numpy as np
+    import torch
+    import chmy
+    from chmy import configs
+    from chmy.stokes import MixedPTStokes, PTStokes, pure_shear_initial, pure_shear_values
+    arch = chmy.Arch(chmy.MI355XBackend())
+    if args.ttc == "2d":
+        g, eta, dt = configs.stokes2d_inclusions(arch, args.n)
+    else:
+        g, eta, dt = configs.stokes3d_inclusion(arch, args.n)
+    nd = g.nd
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    res = {}
+    S = PTStokes(arch, g, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(nd))
+    pure_shear_initial(g, S.V)
+    S.apply_bc()
+    S.sync_buffers()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it64, err64 = S.solve(eps=args.eps, max_iter=args.max_iter, ncheck=args.ncheck,
+                          callback=lambda it, e: log("f64", it, [f"{x:.2e}" for x in e]) if it % (20 * args.ncheck) == 0 else None)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    res["f64"] = {"seconds": round(t64, 3), "iterations": it64, "residual": err64,
+                  "true_residual_incl_tau": S.true_residual()}
+    log("f64 done", it64, t64, err64)
+    M = MixedPTStokes(arch, g, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(nd), inner=args.inner)
+    pure_shear_initial(g, M.x.V)
+    M.x.apply_bc()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sw, errm = M.solve(eps=args.eps, max_cycles=args.max_iter // args.inner + 1,
+                       callback=lambda s, e: log("mixed", s, [f"{x:.2e}" for x in e]) if (s // args.inner) % 20 == 0 else None)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    res["f32_fp64dc"] = {"seconds": round(tm, 3), "f32_iterations": sw, "cycles": M.cycles, "residual": errm}
+    log("mixed done", sw, tm, errm)
+    errs = {}
+    names = list(g.axes_names())
+    pairs = [("V" + names[D], S.V[D], M.x.V[D]) for D in range(nd)] + [("Pr", S.Pr, M.x.Pr)]
+    pairs += [("tau_" + k, a, b) for k, a, b in zip(S.tau._fields, S.tau, M.x.tau)]
+    for name, a, b in pairs:
+        ai = a.interior().double().cpu().numpy()
+        bi = b.interior().double().cpu().numpy()
+        d = ai - bi
+        errs[name] = {"max_rel": float(np.max(np.abs(d)) / max(np.max(np.abs(ai)), 1e-300)),
+                      "l2_rel": float(np.linalg.norm(d) / max(np.linalg.norm(ai), 1e-300))}
+    out = {"metric": "time-to-1e-8 FP32+fix vs FP64", "value": round(t64 / tm, 3), "unit": "x speedup",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f32 sweeps + f64 residual",
+           "config": {"workload": f"{nd}D VE Stokes PT, {args.n}^{nd} cells, inclusions, eps {args.eps}",
+                      "inner": args.inner, "ncheck": args.ncheck},
+           "runs": res, "field_errors_vs_f64": errs}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,11 +169,20 @@ def main():
     ap.add_argument("--mode", default="fused", choices=["fused", "unfused"],
                     help="fused: one-pass double-buffered iteration; unfused: the reference's "
                          "update_stress! + update_velocity! kernels")
+    ap.add_argument("--ttc", choices=["2d", "3d"], default=None,
+                    help="time-to-convergence comparison instead of the T_eff line")
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--inner", type=int, default=200)
+    ap.add_argument("--ncheck", type=int, default=200)
+    ap.add_argument("--max-iter", type=int, default=400000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is out of scope)", file=sys.stderr)
         sys.exit(2)
+    if args.ttc:
+        time_to_converge(args)
+        return
 
     import torch
     dtype = torch.float64 if args.dtype == "f64" else torch.float32

@@ -67,7 +67,10 @@ SIGNATURES = {
     "chmy_maxloc": [PG, PB, PF, PF, VP],
     "chmy_thermal_flux": [PG, PB, PF, PF, PF, D, VP],
     "chmy_thermal_update": [PG, PB, PF, PF, PF, D, VP],
-    "chmy_stokes_iter": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, C.c_int, D, D, D, D, D, D, D, VP],
+    "chmy_stokes_iter": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, C.c_int, D, D, D, D, D, D, D,
+                         VP],
+    "chmy_stokes_residual": [PG, PF, PF, PF, PF, PF, PF, D, D, D, PF, PF, PF, PF, PF, VP],
+    "chmy_stokes_accumulate": [PG, PF, PF, PF, PF, PF, PF, VP],
     "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
     "chmy_maxabs": [PF, PB, P(C.c_double), VP],
     "chmy_sumsq": [PF, PB, P(C.c_double), VP],

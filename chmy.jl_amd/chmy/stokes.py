@@ -68,6 +68,8 @@ class PTStokes:
         self.divV = Field(arch, grid, Center(), dtype)
         self.r_V = VectorField(arch, grid, dtype)
         self.rhog = rhog
+        self.f_V = None          # defect-correction forcing (set by MixedPTStokes)
+        self.f_P = None
         self.G = float(G)
         self.dt = float(dt) if dt is not None else 1.0
         self.eta_s = float(eta) if eta is not None else 1.0
@@ -113,6 +115,8 @@ class PTStokes:
                   C.byref(self.eta.desc()) if self.eta is not None else None,
                   C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
                   C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  _arr(self.f_V) if self.f_V is not None else None,
+                  C.byref(self.f_P.desc()) if self.f_P is not None else None,
                   C.byref(self.divV.desc()), _arr(self.r_V),
                   (ITER_RESIDUAL if write_res else 0) | (ITER_EXACT if self.exact else 0), self.eta_s, self.eta_ve_s, self.G,
                   self.dt, self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], current_stream())
@@ -166,6 +170,26 @@ class PTStokes:
         errs = [maxabs(self.divV) * tau_sc] + [maxabs(self.r_V[D]) * lz / psc for D in range(self.nd)]
         return errs
 
+    def true_residual(self):
+        """Residual norms of the current f64 state evaluated from scratch:
+        [max|∇·V|, max|R_V| per component, max|R_τ|] (chmy_stokes_residual)."""
+        if self.dtype != torch.float64:
+            raise ValueError("true_residual needs an f64 state")
+        if getattr(self, "_scr", None) is None:
+            g = self.grid
+            self._scr = (TensorField(self.arch, g, torch.float64), VectorField(self.arch, g, torch.float64),
+                         Field(self.arch, g, Center(), torch.float64))
+        toc, fV, fP = self._scr
+        g = self.grid
+        _lib.call("chmy_stokes_residual", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
+                  _arr(self.tau_old), C.byref(self.eta.desc()) if self.eta is not None else None,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None, self.eta_s, self.G, self.dt,
+                  _arr(toc), _arr(fV), C.byref(fP.desc()), C.byref(self.divV.desc()), _arr(self.r_V),
+                  current_stream())
+        from .kernels import maxabs
+        err_tau = max(maxabs(t) for t in toc) / (self.G * self.dt)
+        return self.residual() + [err_tau]
+
     def solve(self, eps=1e-8, max_iter=100000, ncheck=None, callback=None):
         """Iterate until all scaled residuals < eps (stokes_3d_inc_ve_T.jl:221-238).
         Raises on non-finite residuals."""
@@ -217,3 +241,81 @@ def pure_shear_initial(grid, V):
         shape = [1] * nd
         shape[nd - 1 - D] = -1
         f.data.copy_(torch.as_tensor(np.broadcast_to((rate * xs).reshape(shape), f.data.shape).copy()))
+
+
+class MixedPTStokes:
+    """FP32 PT sweeps with FP64 defect correction (build extension).
+
+    The f64 master state x = (V, τ, Pr) lives in `self.x` (a PTStokes in f64;
+    only its storage and BC batch are used).  Every `inner` iterations the f64
+    residual R(x) is evaluated (chmy_stokes_residual) and becomes the forcing
+    of an f32 PT solve for a correction d (`self.d`, homogeneous BCs), which
+    starts from 0 and runs `inner` fused f32 iterations; then x += d
+    (chmy_stokes_accumulate) and the real V batch is applied to x.  The
+    convergence test uses the f64 residual of x: max|∇·V| and max|R_V| with
+    boundary residual vertices zeroed (stokes_3d_inc_ve_T.jl:229-233).
+    """
+
+    def __init__(self, arch, grid, *, eta_field=None, eta=None, G=1.0, dt=1.0, normal_values=None,
+                 re_m=2.3 * math.pi, r=0.5, rhog=None, inner=200, exact=False):
+        self.grid = grid
+        self.inner = int(inner)
+        self.x = PTStokes(arch, grid, dtype=torch.float64, eta_field=eta_field, eta=eta, G=G, dt=dt,
+                          normal_values=normal_values, re_m=re_m, r=r, rhog=rhog, exact=exact)
+        self.d = PTStokes(arch, grid, dtype=torch.float32, eta_field=eta_field, eta=eta, G=G, dt=dt,
+                          normal_values={D: (0.0, 0.0) for D in range(grid.nd)}, re_m=re_m, r=r, exact=exact)
+        self.d.f_V = VectorField(arch, grid, torch.float32)
+        self.d.f_P = Field(arch, grid, Center(), torch.float32)
+        self.cycles = 0
+        self.sweeps = 0
+
+    def refresh(self):
+        """f64 residual of x -> f32 forcing; returns the residual norms
+        [max|∇·V|, max|R_V| per component, max|R_τ|] (τsc = lz = psc = 1).
+        The stress residual is included because a state with τ = 0, Pr = 0 and
+        a divergence-free linear V has ∇·V = 0 and R_V = 0 exactly."""
+        x, d, g = self.x, self.d, self.grid
+        _lib.call("chmy_stokes_residual", C.byref(g.desc()), _arr(x.V), _arr(x.tau), C.byref(x.Pr.desc()),
+                  _arr(x.tau_old), C.byref(x.eta.desc()) if x.eta is not None else None,
+                  C.byref(x.rhog.desc()) if x.rhog is not None else None, x.eta_s, x.G, x.dt,
+                  _arr(d.tau_old), _arr(d.f_V), C.byref(d.f_P.desc()), C.byref(x.divV.desc()), _arr(x.r_V),
+                  current_stream())
+        from .kernels import maxabs
+        err_tau = max(maxabs(t) for t in d.tau_old) / (x.G * x.dt)
+        return x.residual() + [err_tau]
+
+    def reset_correction(self):
+        d = self.d
+        for f in list(d.V) + list(d.tau) + [d.Pr] + list(d.V2) + list(d.tau2) + [d.Pr2]:
+            f.data.zero_()
+
+    def accumulate(self):
+        x, d, g = self.x, self.d, self.grid
+        _lib.call("chmy_stokes_accumulate", C.byref(g.desc()), _arr(x.V), _arr(x.tau), C.byref(x.Pr.desc()),
+                  _arr(d.V), _arr(d.tau), C.byref(d.Pr.desc()), current_stream())
+        x.apply_bc(x.V)
+
+    def cycle(self):
+        self.reset_correction()
+        for _ in range(self.inner):
+            self.d.iteration()
+        self.accumulate()
+        self.cycles += 1
+        self.sweeps += self.inner
+
+    def solve(self, eps=1e-8, max_cycles=10000, callback=None):
+        """Defect-correction cycles until the f64 residual of x is below eps."""
+        err = self.refresh()
+        for c in range(max_cycles):
+            if callback:
+                callback(self.sweeps, err)
+            if not all(math.isfinite(e) for e in err):
+                raise FloatingPointError(f"simulation failed, err = {err}")
+            if all(e < eps for e in err):
+                return self.sweeps, err
+            self.cycle()
+            err = self.refresh()
+        return self.sweeps, err
+
+    def true_residual(self):
+        return self.refresh()

@@ -53,11 +53,12 @@ struct Fused3 {
     T *oVx, *oVy, *oVz, *oPr, *otxx, *otyy, *otzz, *otxy, *otxz, *otyz;
     const T *Oxx, *Oyy, *Ozz, *Oxy, *Oxz, *Oyz;    // tau_old
     const T *eta, *eml, *rg;
+    const T *fVx, *fVy, *fVz, *fP;     // correction forcing (defect correction), optional
     T *dV, *rVx, *rVy, *rVz;
     LocStride L[NLOC];
     int32_t n[3];
     int32_t ntx, nty, nchunks, kchunk;
-    int32_t has_rg, write_res;
+    int32_t has_rg, write_res, has_f;
     T ix, iy, iz;
     Kc<T> k;
 };
@@ -238,7 +239,7 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
             } else {
                 Lc = Le = AR::scalar(K);
             }
-            P = AR::pr(P, dv, Lc, K);
+            P = AR::pr(P, a.has_f ? dv - ldo(a.fP, o[LC]) : dv, Lc, K);
             txx = AR::diag(txx, oxx, exx, dv, Lc, K);
             tyy = AR::diag(tyy, oyy, eyy, dv, Lc, K);
             tzz = AR::diag(tzz, ozz, ezz, dv, Lc, K);
@@ -278,13 +279,18 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
 
         // ---------------- C: velocity at plane k
         if (!pro && own && in_k) {
-            const T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) +
-                          (s_txy[ty1][tx] - txy) * iy) + (txz1 - txz_k) * iz;
-            const T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) +
-                          (s_txy[ty][tx1] - txy) * ix) + (tyz1 - tyz_k) * iz;
+            T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) +
+                    (s_txy[ty1][tx] - txy) * iy) + (txz1 - txz_k) * iz;
+            T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) +
+                    (s_txy[ty][tx1] - txy) * ix) + (tyz1 - tyz_k) * iz;
             T rz = ((-((P - P_km1) * iz) + (tzz - tzz_km1) * iz) + (s_txz[p ^ 1][ty][tx1] - txz_k) * ix) +
                    (s_tyz[p ^ 1][ty1][tx] - tyz_k) * iy;
             if (a.has_rg) rz = rz - ldo(a.rg, o[LVZ]);
+            if (a.has_f) {
+                rx = rx + ldo(a.fVx, o[LVX]);
+                ry = ry + ldo(a.fVy, o[LVY]);
+                rz = rz + ldo(a.fVz, o[LVZ]);
+            }
             T ex, ey, ez;
             if constexpr (FVISC) {
                 ex = AR::vcoef_field(jmax(s_eml[p][ty][txm], eml_k), K);
@@ -327,9 +333,9 @@ struct DF2 {
 
 template <class T>
 struct Fused2 {
-    DF2<T> V[2], tau[3], Pr, Vo[2], tauo[3], Pro, told[3], eta, eml, rg, dV, rV[2];
+    DF2<T> V[2], tau[3], Pr, Vo[2], tauo[3], Pro, told[3], eta, eml, rg, dV, rV[2], fV[2], fP;
     int64_t n[2];
-    int32_t has_rg, write_res;
+    int32_t has_rg, write_res, has_f;
     T ix, iy;
     Kc<T> k;
 };
@@ -377,7 +383,7 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(Fused2<T> a) {
         } else {
             Lc = Le = AR::scalar(K);
         }
-        P = AR::pr(P, dv, Lc, K);
+        P = AR::pr(P, a.has_f ? dv - a.fP.p[O(a.fP)] : dv, Lc, K);
         txx = AR::diag(txx, ld(a.told[0]), exx, dv, Lc, K);
         tyy = AR::diag(tyy, ld(a.told[1]), eyy, dv, Lc, K);
         const T exy = T(0.5) * ((vx - s_vx[tym][tx]) * iy + (vy - s_vy[ty][txm]) * ix);
@@ -396,9 +402,13 @@ __global__ void __launch_bounds__(kFX * BY) fused2d_kernel(Fused2<T> a) {
     s_txy[ty][tx] = txy;
     __syncthreads();
     if (own) {
-        const T rx = (-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) + (s_txy[ty1][tx] - txy) * iy;
+        T rx = (-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) + (s_txy[ty1][tx] - txy) * iy;
         T ry = (-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[ty][tx1] - txy) * ix;
         if (a.has_rg) ry = ry - a.rg.p[O(a.rg)];
+        if (a.has_f) {
+            rx = rx + a.fV[0].p[O(a.fV[0])];
+            ry = ry + a.fV[1].p[O(a.fV[1])];
+        }
         T ex, ey;
         if constexpr (FVISC) {
             ex = AR::vcoef_field(jmax(s_eml[ty][txm], eml), K);
@@ -457,7 +467,8 @@ template <class T>
 static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
                         const chmy_field_t* Vo, const chmy_field_t* tauo, const chmy_field_t* Pro,
                         const chmy_field_t* told, const chmy_field_t* eta, const chmy_field_t* eml,
-                        const chmy_field_t* rg, const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
+                        const chmy_field_t* rg, const chmy_field_t* fV, const chmy_field_t* fP,
+                        const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
                         hipStream_t s) {
     const bool fv = eta != nullptr;
@@ -486,6 +497,11 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.rVx = wres ? P(rV[0]) : a.oVx;
         a.rVy = wres ? P(rV[1]) : a.oVy;
         a.rVz = wres ? P(rV[2]) : a.oVz;
+        a.has_f = fV != nullptr;
+        a.fVx = fV ? P(fV[0]) : a.Pr;
+        a.fVy = fV ? P(fV[1]) : a.Pr;
+        a.fVz = fV ? P(fV[2]) : a.Pr;
+        a.fP = fV ? P(*fP) : a.Pr;
         a.L[LC] = loc_stride(*Pr);
         a.L[LVX] = loc_stride(V[0]);
         a.L[LVY] = loc_stride(V[1]);
@@ -529,6 +545,9 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.eml = fv ? v2<T>(*eml) : a.Pr;
         a.rg = rg ? v2<T>(*rg) : a.Pr;
         a.dV = wres ? v2<T>(*dV) : a.Pro;
+        a.has_f = fV != nullptr;
+        for (int d = 0; d < 2; ++d) a.fV[d] = fV ? v2<T>(fV[d]) : a.Pr;
+        a.fP = fV ? v2<T>(*fP) : a.Pr;
         a.n[0] = g->n[0];
         a.n[1] = g->n[1];
         a.has_rg = rg != nullptr;
@@ -557,7 +576,8 @@ extern "C" {
 int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
                      const chmy_field_t* V_out, const chmy_field_t* tau_out, const chmy_field_t* Pr_out,
                      const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* eta_ml,
-                     const chmy_field_t* rhog, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
+                     const chmy_field_t* rhog, const chmy_field_t* f_V, const chmy_field_t* f_P,
+                     const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                      double nudtau, void* stream) {
     CHMY_TRY(check_grid(g));
@@ -587,6 +607,11 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
         CHMY_TRY(check_center(g, divV, "divV", dt_));
         CHMY_TRY(check_vector(g, r_V, "r_V", dt_));
     }
+    if ((f_V == nullptr) != (f_P == nullptr)) return set_error(CHMY_E_ARG, "f_V and f_P go together");
+    if (f_V) {
+        CHMY_TRY(check_vector(g, f_V, "f_V", dt_));
+        CHMY_TRY(check_center(g, f_P, "f_P", dt_));
+    }
     // one stride set per staggered location: every field must have halo 1
     const int nd = g->nd, nt = nd == 2 ? 3 : 6;
     auto halo1 = [nd](const chmy_field_t* f) {
@@ -595,10 +620,11 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
         return true;
     };
     bool ok = halo1(Pr) && halo1(Pr_out) && (!eta || (halo1(eta) && halo1(eta_ml))) && (!rhog || halo1(rhog)) &&
-              (!write_residual || halo1(divV));
+              (!write_residual || halo1(divV)) && (!f_P || halo1(f_P));
     for (int d = 0; d < nd; ++d) {
         ok = ok && halo1(&V[d]) && halo1(&V_out[d]);
         if (write_residual) ok = ok && halo1(&r_V[d]);
+        if (f_V) ok = ok && halo1(&f_V[d]);
     }
     for (int c = 0; c < nt; ++c) ok = ok && halo1(&tau[c]) && halo1(&tau_out[c]) && halo1(&tau_old[c]);
     if (!ok) return set_error(CHMY_E_HALO, "fused iteration needs halo 1 on every field");
@@ -608,10 +634,10 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
         if (tau[c].data == tau_out[c].data) return set_error(CHMY_E_ARG, "tau and tau_out must be distinct buffers");
     if (Pr->data == Pr_out->data) return set_error(CHMY_E_ARG, "Pr and Pr_out must be distinct buffers");
     return dt_ == CHMY_F64
-               ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, divV, r_V,
+               ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                       write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
                                       (hipStream_t)stream)
-               : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, divV, r_V,
+               : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                      write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
                                      (hipStream_t)stream);
 }

@@ -180,14 +180,37 @@ int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const ch
  * (results bitwise those of chmy_stokes_stress + chmy_stokes_velocity);
  * without it the algebraically equal form with 7 instead of 31 divisions per
  * point is used (a few ulp per iteration).  The V boundary batch is applied
- * by the caller afterwards (chmy_bc_batchset on V_out), as Launcher's bc= does. */
+ * by the caller afterwards (chmy_bc_batchset on V_out), as Launcher's bc= does.
+ * Defect correction (build extension): with f_V/f_P (vector / Center fields)
+ * the iteration solves for a correction δ driven by a residual R of an f64
+ * state: r_V += f_V, and Pr uses (divV - f_P); the stress forcing enters as
+ * tau_old = G dt R_tau (see chmy_stokes_residual).  NULL f_V/f_P: plain PT. */
 enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2 };
 int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                      const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                      const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
-                     const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* divV,
-                     const chmy_field_t* r_V, int flags, double eta_s, double eta_ve_s, double G,
-                     double dt, double dtau_Pr, double dtau_r, double nudtau, void* stream);
+                     const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
+                     const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
+                     double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                     double nudtau, void* stream);
+
+/* Defect correction (build extension; the reference has no mixed-precision
+ * path).  Residuals of an f64 state in the reference's expression order
+ * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:84-111):
+ * R_tau = -(tau - tau_old)/(G dt) - tau/eta + 2 dev(eps(V)), R_P = -divg(V),
+ * R_V = -grad(Pr) + divg(tau) - rhog, over the Launcher box.  Writes the
+ * correction forcing in the dtype of the output fields: tau_old_c = G dt R_tau,
+ * f_P = R_P, f_V = R_V; divV/r_V (f64, optional) receive divg(V) and R_V for
+ * the residual norms.  Viscosity as in chmy_stokes_stress. */
+int chmy_stokes_residual(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
+                         const chmy_field_t* Pr, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                         const chmy_field_t* rhog, double eta_s, double G, double dt,
+                         const chmy_field_t* tau_old_c, const chmy_field_t* f_V, const chmy_field_t* f_P,
+                         const chmy_field_t* divV, const chmy_field_t* r_V, void* stream);
+/* x += d over the Launcher box for (V, tau, Pr) in f64, d in f32 or f64. */
+int chmy_stokes_accumulate(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
+                           const chmy_field_t* Pr, const chmy_field_t* V_c, const chmy_field_t* tau_c,
+                           const chmy_field_t* Pr_c, void* stream);
 
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),

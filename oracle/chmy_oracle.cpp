@@ -442,6 +442,67 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
     });
 }
 
+// Residuals of a PT state (defect correction): the bracket of update_stress!
+// (r_tau, :92-97), -divg(V) and update_velocity!'s r_V (:109-111), f64.
+template <class T> void residual_t(const orc_grid* gg, const orc_box* b, const orc_field* V_, const orc_field* tau,
+                                   const orc_field* Pr_, const orc_field* tauo, const orc_field* eta_, const orc_field* rg_,
+                                   double eta_s, double G_, double dt_, const orc_field* Rt, const orc_field* RV,
+                                   const orc_field* RP) {
+    G<T> g(gg);
+    const int nd = g.nd, nt = nd == 2 ? 3 : 6;
+    alignas(FV<T>) unsigned char b1[6 * sizeof(FV<T>)], b2[6 * sizeof(FV<T>)], b3[3 * sizeof(FV<T>)],
+        b4[6 * sizeof(FV<T>)], b5[3 * sizeof(FV<T>)];
+    FV<T>* t = reinterpret_cast<FV<T>*>(b1);
+    FV<T>* to = reinterpret_cast<FV<T>*>(b2);
+    FV<T>* V = reinterpret_cast<FV<T>*>(b3);
+    FV<T>* rt = reinterpret_cast<FV<T>*>(b4);
+    FV<T>* rv = reinterpret_cast<FV<T>*>(b5);
+    views(tau, nt, t);
+    views(tauo, nt, to);
+    views(V_, nd, V);
+    views(Rt, nt, rt);
+    views(RV, nd, rv);
+    FV<T> Pr(Pr_), rp(RP);
+    const bool fe = eta_ != nullptr, hr = rg_ != nullptr;
+    FV<T> eta(fe ? eta_ : Pr_), rg(hr ? rg_ : Pr_);
+    const T Gdt = T(G_) * T(dt_);
+    static const int L2[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 1, 0}};
+    static const int L3[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    for_box(b, nd, [&](const Ix& I) {
+        T e[6];
+        if (nd == 2) {
+            e[0] = dd(V[0], g, 0, I); e[1] = dd(V[1], g, 1, I);
+            e[2] = T(0.5) * (dd(V[0], g, 1, I) + dd(V[1], g, 0, I));
+        } else {
+            e[0] = dd(V[0], g, 0, I); e[1] = dd(V[1], g, 1, I); e[2] = dd(V[2], g, 2, I);
+            e[3] = T(0.5) * (dd(V[0], g, 1, I) + dd(V[1], g, 0, I));
+            e[4] = T(0.5) * (dd(V[0], g, 2, I) + dd(V[2], g, 0, I));
+            e[5] = T(0.5) * (dd(V[1], g, 2, I) + dd(V[2], g, 1, I));
+        }
+        const T dv = divg(V, g, I);
+        for (int c = 0; c < nt; ++c) {
+            T et = T(eta_s);
+            if (fe) et = c < nd ? eta(I) : lerp(eta, nd == 2 ? L2[c] : L3[c], I);
+            const T tt = t[c](I), tto = to[c](I);
+            rt[c](I) = c < nd ? ((-(tt - tto)) / Gdt - tt / et) + T(2) * (e[c] - dv / T(3))
+                              : ((-(tt - tto)) / Gdt - tt / et) + T(2) * e[c];
+        }
+        rp(I) = -dv;
+        T r[3];
+        if (nd == 2) {
+            r[0] = (-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[2], g, 1, I);
+            r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[2], g, 0, I));
+            if (hr) r[1] = r[1] - rg(I);
+        } else {
+            r[0] = ((-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[3], g, 1, I)) + dd(t[4], g, 2, I);
+            r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[3], g, 0, I)) + dd(t[5], g, 2, I);
+            r[2] = ((-dd(Pr, g, 2, I) + dd(t[2], g, 2, I)) + dd(t[4], g, 0, I)) + dd(t[5], g, 1, I);
+            if (hr) r[2] = r[2] - rg(I);
+        }
+        for (int d = 0; d < nd; ++d) rv[d](I) = r[d];
+    });
+}
+
 // examples/stokes_3d_inc_ve_T.jl:11-21 (2D: stokes_2d_inc_ve_T.jl:11-18)
 template <class T> void update_old_t(const orc_box* b, int nd, const orc_field* Tf, const orc_field* tau,
                                      const orc_field* Told, const orc_field* tauo) {
@@ -718,6 +779,13 @@ int orc_update_velocity(const orc_grid* g, const orc_box* b, const orc_field* V,
     Visc vs{nullptr, eta_ml, 0.0, eta_ve_s, G, dt};
     if (Pr->dtype == 0) velocity_t<double>(g, b, V, rV, Pr, tau, rhog, vs, nudtau);
     else velocity_t<float>(g, b, V, rV, Pr, tau, rhog, vs, nudtau);
+    return g_err;
+}
+int orc_stokes_residual(const orc_grid* g, const orc_box* b, const orc_field* V, const orc_field* tau,
+                        const orc_field* Pr, const orc_field* tauo, const orc_field* eta, const orc_field* rg,
+                        double eta_s, double G, double dt, const orc_field* Rt, const orc_field* RV,
+                        const orc_field* RP) {
+    residual_t<double>(g, b, V, tau, Pr, tauo, eta, rg, eta_s, G, dt, Rt, RV, RP);
     return g_err;
 }
 int orc_thermal_flux(const orc_grid* g, const orc_box* b, const orc_field* qT, const orc_field* T,

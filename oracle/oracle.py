@@ -75,6 +75,9 @@ def _declare(L):
                                     P(OField)] + [C.c_double] * 6
     L.orc_update_velocity.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
                                       P(OField)] + [C.c_double] * 4
+    L.orc_stokes_residual.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
+                                      P(OField), C.c_double, C.c_double, C.c_double, P(OField), P(OField),
+                                      P(OField)]
     L.orc_thermal_flux.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), C.c_double]
     L.orc_thermal.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), C.c_double]
     L.orc_maxloc.argtypes = [P(OBox), P(OField), P(OField)]
@@ -328,6 +331,16 @@ def update_velocity(grid, V, rV, Pr, tau, rhog, eta_ve, nudtau, eta_ml=None, G=1
     _check(lib().orc_update_velocity(C.byref(grid.desc()), C.byref(b), _fields(V), _fields(rV),
                                      C.byref(Pr.desc()), _fields(tau), _opt(rhog), _opt(eta_ml),
                                      float(eta_ve), G, dt, nudtau))
+
+
+def stokes_residual(grid, V, tau, Pr, tauo, eta, rhog, G, dt, Rtau, RV, RP, b=None):
+    """R_tau, R_V, R_P of a PT state (defect correction); eta scalar or HField."""
+    b = b or grid.launcher_box()
+    ef = eta if isinstance(eta, HField) else None
+    es = 0.0 if ef is not None else float(eta)
+    _check(lib().orc_stokes_residual(C.byref(grid.desc()), C.byref(b), _fields(V), _fields(tau), C.byref(Pr.desc()),
+                                     _fields(tauo), _opt(ef), _opt(rhog), es, G, dt, _fields(Rtau), _fields(RV),
+                                     C.byref(RP.desc())))
 
 
 def thermal_flux(grid, qT, T, V, lam, b=None):

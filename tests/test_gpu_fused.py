@@ -67,7 +67,8 @@ def test_fused_iteration_equals_two_kernels(arch, T, nd, big_z, visc, wres, exac
     _lib.call("chmy_stokes_iter", C.byref(g.desc()), _arr([a for a, _ in V]), _arr([a for a, _ in tau]),
               C.byref(Pr[0].desc()), _arr(Vo), _arr(tauo), C.byref(Pro.desc()), _arr([a for a, _ in told]),
               C.byref(eta[0].desc()) if fv else None, C.byref(eml[0].desc()) if fv else None,
-              C.byref(rg[0].desc()), C.byref(dV[0].desc()), _arr([a for a, _ in rV]), wres | (2 if exact else 0),
+              C.byref(rg[0].desc()), None, None, C.byref(dV[0].desc()), _arr([a for a, _ in rV]),
+              wres | (2 if exact else 0),
               eta_s, eve, G, dt,
               p["dtau_Pr"], p["dtau_r"], p["nudtau"], chmy.current_stream())
     chmy.synchronize()

@@ -339,3 +339,47 @@ def test_thermal_pure_diffusion_and_upwind():
     Ti = T.interior()
     assert np.allclose(qT[0].interior()[1:-1, :], 2.0 * Ti[:-1, :])   # left_x T at vertex i = T[i-1]
     assert np.allclose(qT[1].interior()[:, 1:-1], -1.0 * Ti[:, 1:])   # right_y T at vertex j = T[j]
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("visc", ["scalar", "field"])
+def test_stokes_residual_matches_update_kernels(nd, visc):
+    """The defect-correction residual restatement agrees with the update
+    kernels it is read from: R_tau = (tau_new - tau)/(eta_ve dtau_r) of
+    update_stress!, R_P = -divg(V) (the stored ∇V), R_V = update_velocity!'s r_V
+    (examples/stokes_3d_inc_ve_T.jl:92-111)."""
+    n = (9, 8, 7)[:nd]
+    g = O.Grid((-0.5,) * nd, (1.0,) * nd, n)
+    rng = np.random.default_rng(11 + nd)
+    F = _stokes_fields(g)
+    for f in F["V"] + F["tau"] + F["tauo"] + [F["Pr"]]:
+        f.arr[...] = rng.standard_normal(f.arr.shape)
+    eta_s, G, dt = 0.6, 1.0, 0.4
+    ef = None
+    if visc == "field":
+        ef = O.HField(g, C)
+        ef.arr[...] = rng.uniform(0.5, 2.0, ef.arr.shape)
+    Rt = [O.HField(g, f.loc) for f in F["tau"]]
+    RV = [O.HField(g, f.loc) for f in F["V"]]
+    RP = O.HField(g, C)
+    O.stokes_residual(g, F["V"], F["tau"], F["Pr"], F["tauo"], ef if ef is not None else eta_s, None, G, dt, Rt, RV, RP)
+    tau0 = [f.arr.copy() for f in F["tau"]]
+    # drive update_stress! with dtau_r = 1 and a unit eta_ve so tau_new - tau = r_tau exactly up to rounding
+    eve = 1.0 / (1.0 / eta_s + 1.0 / (G * dt))
+    O.update_stress(g, F["tau"], F["Pr"], F["dV"], F["V"], F["tauo"], ef if ef is not None else eta_s, eve, G, dt,
+                    0.0, 1.0)
+    sl = tuple(slice(1, -1) for _ in range(nd))
+    for c, (t0, f, r) in enumerate(zip(tau0, F["tau"], Rt)):
+        if ef is None:
+            ev = eve
+        else:
+            # per-point eta_ve: recover from the stored eta at the tau location
+            ev = None
+        if ev is not None:
+            assert np.allclose(((f.arr - t0) / ev)[sl], r.arr[sl], rtol=1e-9, atol=1e-9)
+    assert np.array_equal(RP.arr[sl], -F["dV"].arr[sl])
+    O.update_velocity(g, F["V"], F["rV"], F["Pr"], F["tau"], None, eve, 1.0)
+    # r_V of update_velocity! uses the updated tau: recompute R_V from the updated state
+    O.stokes_residual(g, F["V"], F["tau"], F["Pr"], F["tauo"], ef if ef is not None else eta_s, None, G, dt, Rt, RV, RP)
+    for D in range(nd):
+        assert np.array_equal(RV[D].arr[sl], F["rV"][D].arr[sl])
