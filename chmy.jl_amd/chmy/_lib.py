@@ -69,8 +69,16 @@ SIGNATURES = {
     "chmy_thermal_update": [PG, PB, PF, PF, PF, D, VP],
     "chmy_stokes_iter": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, C.c_int, D, D, D, D, D, D, D,
                          VP],
-    "chmy_stokes_residual": [PG, PF, PF, PF, PF, PF, PF, D, D, D, PF, PF, PF, PF, PF, VP],
+    "chmy_stokes_iterate": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32),
+                            P(C.c_int32), P(ChmyBC), C.c_int, C.c_int, C.c_int, D, D, D, D, D, D, D, P(C.c_int),
+                            VP],
+    "chmy_stokes_residual":[PG, PF, PF, PF, PF, PF, PF, D, D, D, PF, PF, PF, PF, PF, VP],
     "chmy_stokes_accumulate": [PG, PF, PF, PF, PF, PF, PF, VP],
+    "chmy_sia_flux": [PG, PF, PF, PF, PF, D, C.c_int, VP],
+    "chmy_sia_update": [PG, PF, PF, PF, PF, D, D, D, D, VP],
+    "chmy_sia_diffusivity": [PG, PF, PF, PF, PF, D, C.c_int, VP],
+    "chmy_sia_step": [PG, PF, PF, PF, PF, D, C.c_int, D, D, D, D, VP],
+    "chmy_sia_step_adjoint": [PG, PF, PF, PF, PF, PF, PF, D, C.c_int, D, D, D, D, PF, VP],
     "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
     "chmy_maxabs": [PF, PB, P(C.c_double), VP],
     "chmy_sumsq": [PF, PB, P(C.c_double), VP],

@@ -133,6 +133,47 @@ class PTStokes:
         self._swap()
         self.iters += 1
 
+    def _vbatch(self):
+        """The free-slip V batch as (counts, component index, condition) arrays
+        for chmy_stokes_iterate, built once."""
+        if getattr(self, "_vb", None) is None:
+            bs = free_slip_batch(self.grid, self.V, self.normal_values)
+            counts = (C.c_int32 * 6)()
+            comp, conds, keep = [], [], []
+            for d in range(self.nd):
+                for s in range(2):
+                    b = bs[d][s]
+                    counts[2 * d + s] = len(b.fields)
+                    for f, cond in zip(b.fields, b.conditions):
+                        comp.append(next(c for c in range(self.nd) if self.V[c] is f))
+                        conds.append(cond.desc(keep))
+            n = max(1, len(comp))
+            self._vb = (counts, (C.c_int32 * n)(*comp), (_lib.ChmyBC * n)(*conds), keep)
+        return self._vb
+
+    def iterate(self, niter, residual_last=False):
+        """niter PT iterations in one native call (fused kernel + V batch,
+        buffers alternating); (V, tau, Pr) hold the result afterwards."""
+        if niter <= 0:
+            return
+        g = self.grid
+        counts, comp, conds, _keep = self._vbatch()
+        base = ITER_EXACT if self.exact else 0
+        par = C.c_int(0)
+        _lib.call("chmy_stokes_iterate", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
+                  _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(self.tau_old),
+                  C.byref(self.eta.desc()) if self.eta is not None else None,
+                  C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  _arr(self.f_V) if self.f_V is not None else None,
+                  C.byref(self.f_P.desc()) if self.f_P is not None else None,
+                  C.byref(self.divV.desc()), _arr(self.r_V), counts, comp, conds, int(niter), base,
+                  base | (ITER_RESIDUAL if residual_last else 0), self.eta_s, self.eta_ve_s, self.G, self.dt,
+                  self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], C.byref(par), current_stream())
+        if par.value == 1:
+            self._swap()
+        self.iters += niter
+
     def iteration_unfused(self):
         """The reference's sequence: update_stress!, update_velocity! (per-point
         kernels, single buffers, r_V and ∇V written), then the V batch."""
@@ -195,17 +236,18 @@ class PTStokes:
         Raises on non-finite residuals."""
         ncheck = ncheck or 2 * self.grid.dims[0]
         err = None
-        for it in range(1, max_iter + 1):
-            check = it % ncheck == 0
-            self.iteration(write_res=check)
-            if check:
-                err = self.residual()
-                if callback:
-                    callback(it, err)
-                if not all(math.isfinite(e) for e in err):
-                    raise FloatingPointError(f"simulation failed, err = {err}")
-                if all(e < eps for e in err):
-                    return it, err
+        it = 0
+        while it < max_iter:
+            n = min(ncheck, max_iter - it)
+            self.iterate(n, residual_last=True)
+            it += n
+            err = self.residual()
+            if callback:
+                callback(it, err)
+            if not all(math.isfinite(e) for e in err):
+                raise FloatingPointError(f"simulation failed, err = {err}")
+            if all(e < eps for e in err):
+                return it, err
         return max_iter, err
 
 
@@ -297,8 +339,7 @@ class MixedPTStokes:
 
     def cycle(self):
         self.reset_correction()
-        for _ in range(self.inner):
-            self.d.iteration()
+        self.d.iterate(self.inner)
         self.accumulate()
         self.cycles += 1
         self.sweeps += self.inner

@@ -642,4 +642,53 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
                                      (hipStream_t)stream);
 }
 
+// Several PT iterations in one call: each is chmy_stokes_iter from buffer
+// set p into set 1-p followed by the V boundary batch on set 1-p (the
+// Launcher's `bc=` of update_velocity!, examples/stokes_3d_inc_ve_T.jl:223),
+// so the host pays one call per `niter` iterations instead of ~4 launches and
+// descriptor packing per iteration.  The V batch is given once: bc_counts[2D+s]
+// entries per (dim, side), each naming a V component (bc_comp) and its
+// condition (bcs), in batch order; it is applied to whichever set was written.
+// flags apply to every iteration, flags_last to the final one (e.g. to write
+// the residual fields).  *parity_inout is the set holding the current state
+// on entry and receives it on return.
+int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy_field_t* tau0,
+                        const chmy_field_t* Pr0, const chmy_field_t* V1, const chmy_field_t* tau1,
+                        const chmy_field_t* Pr1, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                        const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
+                        const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V,
+                        const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int niter,
+                        int flags, int flags_last, double eta_s, double eta_ve_s, double G, double dt,
+                        double dtau_Pr, double dtau_r, double nudtau, int* parity_inout, void* stream) {
+    CHMY_TRY(check_grid(g));
+    if (!parity_inout || (*parity_inout != 0 && *parity_inout != 1)) return set_error(CHMY_E_ARG, "parity must be 0/1");
+    if (niter < 0) return set_error(CHMY_E_ARG, "niter < 0");
+    if (!bc_counts || !bc_comp || !bcs) return set_error(CHMY_E_ARG, "V batch is NULL");
+    const int nd = g->nd;
+    int total = 0;
+    for (int q = 0; q < 2 * nd; ++q) {
+        if (bc_counts[q] < 0) return set_error(CHMY_E_ARG, "negative batch count");
+        total += bc_counts[q];
+    }
+    if (total > 64) return set_error(CHMY_E_SIZE, "V batch larger than 64 entries");
+    for (int q = 0; q < total; ++q)
+        if (bc_comp[q] < 0 || bc_comp[q] >= nd) return set_error(CHMY_E_ARG, "bc_comp out of range");
+    chmy_field_t fset[2][64];
+    const chmy_field_t* Vs[2] = {V0, V1};
+    for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < total; ++q) fset[p][q] = Vs[p][bc_comp[q]];
+    const chmy_field_t* T[2] = {tau0, tau1};
+    const chmy_field_t* P[2] = {Pr0, Pr1};
+    int p = *parity_inout;
+    for (int it = 0; it < niter; ++it) {
+        const int fl = it == niter - 1 ? flags_last : flags;
+        CHMY_TRY(chmy_stokes_iter(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml, rhog,
+                                  f_V, f_P, divV, r_V, fl, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream));
+        CHMY_TRY(chmy_bc_batchset(g, bc_counts, fset[1 - p], bcs, stream));
+        p = 1 - p;
+    }
+    *parity_inout = p;
+    return 0;
+}
+
 }  // extern "C"

@@ -194,6 +194,22 @@ int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_
                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                      double nudtau, void* stream);
 
+/* niter PT iterations in one call (the inner loop of stokes_3d_inc_ve_T.jl:221-227):
+ * chmy_stokes_iter from buffer set p = *parity_inout into set 1-p, then the V
+ * batch on set 1-p, alternating.  The V batch: bc_counts[2*dim+side-1]
+ * entries per (dim, side) in bc! order, entry q applies bcs[q] to V component
+ * bc_comp[q] (at most 64 entries).  `flags` for every iteration, `flags_last`
+ * for the last (e.g. CHMY_ITER_RESIDUAL for a convergence check).  On return
+ * *parity_inout names the set holding the new state. */
+int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const chmy_field_t* tau0,
+                        const chmy_field_t* Pr0, const chmy_field_t* V1, const chmy_field_t* tau1,
+                        const chmy_field_t* Pr1, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                        const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
+                        const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V,
+                        const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int niter,
+                        int flags, int flags_last, double eta_s, double eta_ve_s, double G, double dt,
+                        double dtau_Pr, double dtau_r, double nudtau, int* parity_inout, void* stream);
+
 /* Defect correction (build extension; the reference has no mixed-precision
  * path).  Residuals of an f64 state in the reference's expression order
  * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:84-111):
@@ -211,6 +227,37 @@ int chmy_stokes_residual(const chmy_grid_t* grid, const chmy_field_t* V, const c
 int chmy_stokes_accumulate(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                            const chmy_field_t* Pr, const chmy_field_t* V_c, const chmy_field_t* tau_c,
                            const chmy_field_t* Pr_c, void* stream);
+
+/* ---------------------------------------------------------------- SIA (2D, f64)
+ * Shallow-ice nonlinear diffusion of examples/sia.jl with the lazy fields
+ * S = B + H, ∇S, D = (A H^(n+2) + As H^n) |∇S|^(n-1), M = min(mb_a (S - mb_b), mb_c)
+ * evaluated at every use.  A (per-cell rate factor) may be NULL (A = 1, the
+ * reference's D, sia.jl:46-49).  H, B, A are Center fields with halos (H's
+ * halo from its Neumann batch, B's filled by the caller). */
+/* update_q! (sia.jl:12-16): q at vertices 1..n in both dims. */
+int chmy_sia_flux(const chmy_grid_t* grid, const chmy_field_t* q, const chmy_field_t* H, const chmy_field_t* B,
+                  const chmy_field_t* A, double As, int npow, void* stream);
+/* update_H! (sia.jl:7-10) over cells 1..n (H_out may alias H). */
+int chmy_sia_update(const chmy_grid_t* grid, const chmy_field_t* H, const chmy_field_t* H_out,
+                    const chmy_field_t* q, const chmy_field_t* B, double mb_a, double mb_b, double mb_c,
+                    double dt, void* stream);
+/* D at cells 1..n (sia.jl:46-49), for the time-step rule Δt = Δ²/max(D)/4.1 (:76). */
+int chmy_sia_diffusivity(const chmy_grid_t* grid, const chmy_field_t* D, const chmy_field_t* H,
+                         const chmy_field_t* B, const chmy_field_t* A, double As, int npow, void* stream);
+/* One explicit step = update_q!, the q Dirichlet batch and update_H! fused,
+ * H -> H_out (distinct); apply H's Neumann batch to H_out afterwards. */
+int chmy_sia_step(const chmy_grid_t* grid, const chmy_field_t* H, const chmy_field_t* H_out,
+                  const chmy_field_t* B, const chmy_field_t* A, double As, int npow, double mb_a, double mb_b,
+                  double mb_c, double dt, void* stream);
+/* Adjoint of chmy_sia_step + Neumann batch (build extension): given the step's
+ * input state H (with halos) and lam_in = dJ/dH_out (cells), writes
+ * lam_out = dJ/dH (cells) and adds dJ/dlog(A) into grad_logA (if A != NULL).
+ * work: 8 scratch fields {mu: Center, Dvx, gdx: (Vertex,Center),
+ * Dvy, gdy: (Center,Vertex), ax, ay, HD: Center}. */
+int chmy_sia_step_adjoint(const chmy_grid_t* grid, const chmy_field_t* H, const chmy_field_t* lam_in,
+                          const chmy_field_t* lam_out, const chmy_field_t* grad_logA, const chmy_field_t* B,
+                          const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
+                          double dt, const chmy_field_t* work, void* stream);
 
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),

@@ -1,0 +1,172 @@
+"""SIA forward kernels against the oracle (bitwise) and the hand-written
+adjoint against finite differences of the GPU forward map."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import chmy
+from chmy import Center as Cc
+from chmy import _lib
+from chmy.kernels import _arr
+from chmy.sia import SIA, bed_bumps, smooth_field
+import oracle as O
+from _helpers import assert_same, dev, ogrid, pair, vloc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def arch():
+    return chmy.Arch(chmy.MI355XBackend())
+
+
+def _setup(arch, n=(40, 33), seed=0, with_A=True):
+    g = chmy.UniformGrid(arch, origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=n)
+    og = ogrid(g)
+    rng = np.random.default_rng(seed)
+    H, hH = pair(arch, g, og, Cc(), fill=0)
+    xi, yi = g.coords(Cc(), 0), g.coords(Cc(), 1)
+    hH.set_interior(0.3 + 0.2 * np.exp(-(xi[:, None] ** 2 + yi[None, :] ** 2) / 0.3) + 0.05 * rng.random(n))
+    O.bc_apply(og, [(hH, O.Neumann())])
+    chmy.set_storage(H, hH.arr)
+    B, hB = pair(arch, g, og, Cc(), fill=0)
+    xs = g.coords(Cc(), 0, first=-1, count=n[0] + 4)
+    ys = g.coords(Cc(), 1, first=-1, count=n[1] + 4)
+    hB.xyz()[...] = np.exp(-(xs[:, None] / 0.5) ** 2 - (ys[None, :] / 0.5) ** 2)
+    chmy.set_storage(B, hB.arr)
+    A = hA = None
+    if with_A:
+        A, hA = pair(arch, g, og, Cc(), rng=rng, lo=0.5, hi=1.5)
+    return g, og, H, hH, B, hB, A, hA
+
+
+@pytest.mark.parametrize("with_A", [False, True])
+def test_sia_flux_update_bitwise(arch, with_A):
+    g, og, H, hH, B, hB, A, hA = _setup(arch, with_A=with_A)
+    q = [chmy.Field(arch, g, vloc(D, 2)) for D in range(2)]
+    hq = [O.HField(og, tuple(l.code for l in vloc(D, 2))) for D in range(2)]
+    As, n, mb, dt = 0.5, 3, (0.5, 0.3, 0.25), 2e-3
+    _lib.call("chmy_sia_flux", C.byref(g.desc()), _arr(q), C.byref(H.desc()), C.byref(B.desc()),
+              C.byref(A.desc()) if with_A else None, As, n, chmy.current_stream())
+    O.sia_flux(og, hq, hH, hB, A=hA, As=As, npow=n, mb=mb)
+    for f, h in zip(q, hq):
+        assert_same(f, h, "q")
+    _lib.call("chmy_sia_update", C.byref(g.desc()), C.byref(H.desc()), C.byref(H.desc()), _arr(q), C.byref(B.desc()),
+              *mb, dt, chmy.current_stream())
+    Hn = hH.copy()
+    O.sia_update(og, hH, Hn, hq, hB, dt, As=As, npow=n, mb=mb)
+    assert_same(H, Hn, "H after update_H!")
+
+
+def test_sia_fused_step_equals_reference_sequence(arch):
+    """chmy_sia_step + Neumann == update_q!, q Dirichlet (all sides, sia.jl:73),
+    update_H!, H Neumann (sia.jl:81), for 20 steps, bitwise."""
+    g, og, H, hH, B, hB, A, hA = _setup(arch, seed=3)
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    S.reset(hH.arr)
+    hq = [O.HField(og, tuple(l.code for l in vloc(D, 2))) for D in range(2)]
+    dt = 2e-3
+    for _ in range(20):
+        S.step(dt)
+        S.H.swap(S.H2)
+        O.sia_flux(og, hq, hH, hB, A=hA, As=0.5, npow=3)
+        O.bc_apply(og, [(hq[0], O.Dirichlet()), (hq[1], O.Dirichlet())])
+        Hn = hH.copy()
+        O.sia_update(og, hH, Hn, hq, hB, dt, As=0.5, npow=3)
+        O.bc_apply(og, [(Hn, O.Neumann())])
+        hH = Hn
+    assert_same(S.H, hH, "H after 20 fused steps", interior_only=True)
+    # D and the time-step rule (sia.jl:76)
+    dmax = S.diffusivity_max()
+    assert dmax == O.sia_Dmax(og, hH, hB, A=hA, As=0.5, npow=3)
+
+
+def test_sia_adjoint_step_equals_fd_jacobian_transpose(arch):
+    """One adjoint step equals J^T λ with J the Jacobian of the GPU forward
+    step + Neumann batch, by central differences on a 9x7 grid (state chosen
+    away from the max/min kinks)."""
+    n = (9, 7)
+    g = chmy.UniformGrid(arch, origin=(0.0, 0.0), extent=(1.0, 0.8), dims=n)
+    rng = np.random.default_rng(5)
+    full = (n[1] + 4, n[0] + 4)
+    B = 0.2 * rng.random(full)
+    logA = np.log(rng.uniform(0.5, 1.5, full))
+    S = SIA(arch, g, B, logA=logA, As=0.3, npow=3, mb=(0.05, -10.0, 100.0))
+    H0 = np.zeros(full)
+    H0[2:-2, 2:-2] = 1.0 + 0.5 * rng.random((n[1], n[0]))
+    dt = 1e-3
+
+    def F(h_int, la=None):
+        S.set_logA(logA if la is None else la)
+        H = H0.copy()
+        H[2:-2, 2:-2] = h_int.reshape(n[1], n[0])
+        S.reset(H)
+        S.step(dt)
+        return S.H2.interior().cpu().numpy().T.reshape(-1).copy()  # C-order over (y, x)
+
+    h0 = H0[2:-2, 2:-2].reshape(-1).copy()
+    m = h0.size
+    eps = 1e-6
+    Jh = np.zeros((m, m))
+    for c in range(m):
+        e = np.zeros(m)
+        e[c] = eps
+        Jh[:, c] = (F(h0 + e) - F(h0 - e)) / (2 * eps)
+    Ja = np.zeros((m, m))
+    for c in range(m):
+        la = logA.copy()
+        la[2 + c // n[0], 2 + c % n[0]] += eps
+        lb = logA.copy()
+        lb[2 + c // n[0], 2 + c % n[0]] -= eps
+        Ja[:, c] = (F(h0, la) - F(h0, lb)) / (2 * eps)
+    lam = rng.standard_normal(m)
+    # adjoint
+    S.set_logA(logA)
+    H = H0.copy()
+    S.reset(H)
+    S.grad.data.zero_()
+    S.lam.data.zero_()
+    S.lam2.data.zero_()
+    S.lam.interior().copy_(torch.from_numpy(lam.reshape(n[1], n[0]).T.copy()))
+    S._adjoint_step(S.H, dt)
+    lam_out = S.lam.interior().cpu().numpy().T.reshape(-1)
+    gA = S.grad.interior().cpu().numpy().T.reshape(-1)
+    assert np.allclose(lam_out, Jh.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Jh.T @ lam).max())
+    assert np.allclose(gA, Ja.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Ja.T @ lam).max())
+
+
+def test_sia_gradient_matches_central_differences(arch):
+    """dJ/dlogA from the reverse sweep (uniform checkpoints, recomputed
+    segments) matches central differences of J along random directions."""
+    n = 64
+    g = chmy.UniformGrid(arch, origin=(0.0, 0.0), extent=(250e3, 250e3), dims=(n, n))
+    B = bed_bumps(g, seed=4, hmin=1500.0, hmax=2500.0)
+    logA0 = np.full((n + 4, n + 4), math.log(2.8e-6))
+    mb = (0.01, 1500.0, 2.0)
+    S = SIA(arch, g, B, logA=logA0 + smooth_field(g, 0.3, seed=9), npow=3, mb=mb, dt_max=1.0)
+    S.reset()
+    nsteps = 60
+    dts = S.forward(nsteps)
+    H_obs = S.H.interior().clone()
+    S.set_logA(logA0)
+    J, grad = S.gradient(None, H_obs, nsteps, dts, ncheck=6)
+    assert J > 0
+    rng = np.random.default_rng(1)
+    gn = grad.cpu().numpy()
+    for _ in range(4):
+        v = rng.standard_normal((n + 4, n + 4))
+        h = 1e-4
+        S.set_logA(logA0 + h * v)
+        S.reset()
+        S.forward(nsteps, dts)
+        Jp = S.misfit(H_obs)
+        S.set_logA(logA0 - h * v)
+        S.reset()
+        S.forward(nsteps, dts)
+        Jm = S.misfit(H_obs)
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.sum(gn * v[2:-2, 2:-2].T))
+        assert abs(fd - ad) <= 1e-5 * abs(fd) + 1e-12 * J, (fd, ad)
