@@ -324,7 +324,12 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
     }
 }
 
-// 2D: one tile per block, no z pipeline.
+// 2D: each wave owns an x-tile of 64 columns (62 outputs + a one-column ring)
+// and marches y over a chunk of rows.  x neighbours come from wave shuffles,
+// y neighbours from registers carried along the march, so there is no LDS,
+// no barrier and no y ring.  At row j a lane computes the Center stresses and
+// Pr at row j and tau_xy at vertex row j+1, then V at row j (the 2D form of
+// the 3D z pipeline).
 template <class T>
 struct DF2 {
     T* p;
@@ -335,97 +340,109 @@ template <class T>
 struct Fused2 {
     DF2<T> V[2], tau[3], Pr, Vo[2], tauo[3], Pro, told[3], eta, eml, rg, dV, rV[2], fV[2], fP;
     int64_t n[2];
+    int32_t ntx, nchunks, ychunk;
     int32_t has_rg, write_res, has_f;
     T ix, iy;
     Kc<T> k;
 };
 
-template <class T, bool FVISC, bool EXACT, int BY>
-__global__ void __launch_bounds__(kFX * BY) fused2d_kernel(Fused2<T> a) {
+template <class T>
+__device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, kFX); }     // value of lane l-1
+template <class T>
+__device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kFX); }   // value of lane l+1
+
+constexpr int kW2 = 4;   // waves (x-tiles) per block
+
+template <class T, bool FVISC, bool EXACT>
+__global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
-    __shared__ T s_vx[BY][kFX], s_vy[BY][kFX];
-    __shared__ T s_eta[FVISC ? BY : 1][kFX], s_eml[FVISC ? BY : 1][kFX];
-    __shared__ T s_P[BY][kFX], s_txx[BY][kFX], s_tyy[BY][kFX], s_txy[BY][kFX];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int64_t i = (int64_t)blockIdx.x * (kFX - 2) - 1 + tx;
-    const int64_t j = (int64_t)blockIdx.y * (BY - 2) - 1 + ty;
+    const int lane = threadIdx.x;
+    const int xt = blockIdx.x * kW2 + threadIdx.y;
+    if (xt >= a.ntx) return;                        // whole wave leaves together
+    const int64_t i = (int64_t)xt * (kFX - 2) - 1 + lane;
     const int64_t nb0 = a.n[0] + 1, nb1 = a.n[1] + 1;
-    const bool valid = i <= a.n[0] + 2 && j <= a.n[1] + 2;
-    const bool inxy = i >= 0 && i <= nb0 && j >= 0 && j <= nb1;
-    const bool own = tx >= 1 && tx <= kFX - 2 && ty >= 1 && ty <= BY - 2 && inxy;
+    const int64_t jb = (int64_t)blockIdx.y * a.ychunk;
+    const int64_t je = min(jb + (int64_t)a.ychunk, nb1 + 1);
+    const bool valid = i <= a.n[0] + 2;
+    const bool inx = i >= 0 && i <= nb0;
+    const bool ownx = lane >= 1 && lane <= kFX - 2 && inx;
     const T ix = a.ix, iy = a.iy;
-    auto O = [i, j](DF2<T> f) { return i + j * f.sy; };
-    auto ld = [i, j, valid](DF2<T> f) -> T { return valid ? f.p[i + j * f.sy] : T(0); };
+    auto O = [i](DF2<T> f, int64_t j) { return i + j * f.sy; };
+    auto ld = [i, valid](DF2<T> f, int64_t j) -> T { return valid ? f.p[i + j * f.sy] : T(0); };
 
-    const T vx = ld(a.V[0]), vy = ld(a.V[1]);
-    T eta = T(0), eml = T(0);
-    s_vx[ty][tx] = vx;
-    s_vy[ty][tx] = vy;
-    if constexpr (FVISC) {
-        eta = ld(a.eta);
-        eml = ld(a.eml);
-        s_eta[ty][tx] = eta;
-        s_eml[ty][tx] = eml;
-    }
-    T P = ld(a.Pr), txx = ld(a.tau[0]), tyy = ld(a.tau[1]), txy = ld(a.tau[2]);
-    __syncthreads();
-    const int tx1 = tx < kFX - 1 ? tx + 1 : tx, ty1 = ty < BY - 1 ? ty + 1 : ty;
-    const int txm = tx > 0 ? tx - 1 : tx, tym = ty > 0 ? ty - 1 : ty;
-    const T exx = (s_vx[ty][tx1] - vx) * ix;
-    const T eyy = (s_vy[ty1][tx] - vy) * iy;
-    const T dv = exx + eyy;
-    if (inxy) {
-        typename AR::Loc Lc, Le;
-        if constexpr (FVISC) {
-            Lc = AR::field(eta, K);
-            Le = AR::field(lerp4(s_eta[tym][txm], s_eta[tym][tx], s_eta[ty][txm], eta), K);
-        } else {
-            Lc = Le = AR::scalar(K);
+    T vx_j = ld(a.V[0], jb - 1), vy_j = ld(a.V[1], jb - 1);
+    T eta_j = FVISC ? ld(a.eta, jb - 1) : T(0);
+    T P_jm1 = T(0), tyy_jm1 = T(0), txy_j = T(0), eml_jm1 = T(0);
+    for (int64_t j = jb - 1; j < je; ++j) {
+        const bool pro = j < jb;
+        const bool in_j = inx && j >= 0;
+        const bool in_j1 = inx && j + 1 <= nb1;
+        const T vx1 = ld(a.V[0], j + 1), vy1 = ld(a.V[1], j + 1);
+        T P = ld(a.Pr, j), txx = ld(a.tau[0], j), tyy = ld(a.tau[1], j), txy1 = ld(a.tau[2], j + 1);
+        const T eta1 = FVISC ? ld(a.eta, j + 1) : T(0);
+        const T eml = FVISC ? ld(a.eml, j) : T(0);
+        // Center stresses and Pr at row j
+        const T exx = (shdn(vx_j) - vx_j) * ix;
+        const T eyy = (vy1 - vy_j) * iy;
+        const T dv = exx + eyy;
+        if (in_j) {
+            typename AR::Loc Lc;
+            if constexpr (FVISC) Lc = AR::field(eta_j, K);
+            else Lc = AR::scalar(K);
+            P = AR::pr(P, a.has_f ? dv - a.fP.p[O(a.fP, j)] : dv, Lc, K);
+            txx = AR::diag(txx, ld(a.told[0], j), exx, dv, Lc, K);
+            tyy = AR::diag(tyy, ld(a.told[1], j), eyy, dv, Lc, K);
         }
-        P = AR::pr(P, a.has_f ? dv - a.fP.p[O(a.fP)] : dv, Lc, K);
-        txx = AR::diag(txx, ld(a.told[0]), exx, dv, Lc, K);
-        tyy = AR::diag(tyy, ld(a.told[1]), eyy, dv, Lc, K);
-        const T exy = T(0.5) * ((vx - s_vx[tym][tx]) * iy + (vy - s_vy[ty][txm]) * ix);
-        txy = AR::off(txy, ld(a.told[2]), exy, Le, K);
-    }
-    if (own) {
-        a.Pro.p[O(a.Pro)] = P;
-        a.tauo[0].p[O(a.tauo[0])] = txx;
-        a.tauo[1].p[O(a.tauo[1])] = tyy;
-        a.tauo[2].p[O(a.tauo[2])] = txy;
-        if (a.write_res) a.dV.p[O(a.dV)] = dv;
-    }
-    s_P[ty][tx] = P;
-    s_txx[ty][tx] = txx;
-    s_tyy[ty][tx] = tyy;
-    s_txy[ty][tx] = txy;
-    __syncthreads();
-    if (own) {
-        T rx = (-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) + (s_txy[ty1][tx] - txy) * iy;
-        T ry = (-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[ty][tx1] - txy) * ix;
-        if (a.has_rg) ry = ry - a.rg.p[O(a.rg)];
-        if (a.has_f) {
-            rx = rx + a.fV[0].p[O(a.fV[0])];
-            ry = ry + a.fV[1].p[O(a.fV[1])];
+        // tau_xy at vertex row j+1 (needs Vx at rows j, j+1 and Vy at column i-1)
+        const T vy1l = shup(vy1);
+        const T etal = FVISC ? shup(eta_j) : T(0), eta1l = FVISC ? shup(eta1) : T(0);
+        if (in_j1) {
+            const T exy = T(0.5) * ((vx1 - vx_j) * iy + (vy1 - vy1l) * ix);
+            typename AR::Loc Le;
+            if constexpr (FVISC) Le = AR::field(lerp4(etal, eta_j, eta1l, eta1), K);
+            else Le = AR::scalar(K);
+            txy1 = AR::off(txy1, ld(a.told[2], j + 1), exy, Le, K);
         }
-        T ex, ey;
-        if constexpr (FVISC) {
-            ex = AR::vcoef_field(jmax(s_eml[ty][txm], eml), K);
-            ey = AR::vcoef_field(jmax(s_eml[tym][tx], eml), K);
-        } else {
-            ex = ey = AR::vcoef_scalar(K);
+        const T Pl = shup(P), txxl = shup(txx), txyr = shdn(txy_j), emll = FVISC ? shup(eml) : T(0);
+        if (!pro && ownx && in_j) {
+            a.Pro.p[O(a.Pro, j)] = P;
+            a.tauo[0].p[O(a.tauo[0], j)] = txx;
+            a.tauo[1].p[O(a.tauo[1], j)] = tyy;
+            a.tauo[2].p[O(a.tauo[2], j)] = txy_j;
+            if (a.write_res) a.dV.p[O(a.dV, j)] = dv;
+            T rx = (-((P - Pl) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy;
+            T ry = (-((P - P_jm1) * iy) + (tyy - tyy_jm1) * iy) + (txyr - txy_j) * ix;
+            if (a.has_rg) ry = ry - a.rg.p[O(a.rg, j)];
+            if (a.has_f) {
+                rx = rx + a.fV[0].p[O(a.fV[0], j)];
+                ry = ry + a.fV[1].p[O(a.fV[1], j)];
+            }
+            T ex, ey;
+            if constexpr (FVISC) {
+                ex = AR::vcoef_field(jmax(emll, eml), K);
+                ey = AR::vcoef_field(jmax(eml_jm1, eml), K);
+            } else {
+                ex = ey = AR::vcoef_scalar(K);
+            }
+            a.Vo[0].p[O(a.Vo[0], j)] = AR::vel(vx_j, rx, ex, K);
+            a.Vo[1].p[O(a.Vo[1], j)] = AR::vel(vy_j, ry, ey, K);
+            if (a.write_res) {
+                a.rV[0].p[O(a.rV[0], j)] = rx;
+                a.rV[1].p[O(a.rV[1], j)] = ry;
+            }
         }
-        a.Vo[0].p[O(a.Vo[0])] = AR::vel(vx, rx, ex, K);
-        a.Vo[1].p[O(a.Vo[1])] = AR::vel(vy, ry, ey, K);
-        if (a.write_res) {
-            a.rV[0].p[O(a.rV[0])] = rx;
-            a.rV[1].p[O(a.rV[1])] = ry;
-        }
+        P_jm1 = P;
+        tyy_jm1 = tyy;
+        txy_j = txy1;
+        vx_j = vx1;
+        vy_j = vy1;
+        eta_j = eta1;
+        eml_jm1 = eml;
     }
 }
 
-constexpr int kBY3 = CHMY_FUSED_BY, kBY2 = 8, kZChunk = 32;
+constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = 32, kYChunk2 = 16;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
@@ -556,12 +573,15 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.iy = T(g->inv_spacing[1]);
         a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const int64_t nbx = g->n[0] + 2, nby = g->n[1] + 2;
-        dim3 blk(kFX, kBY2, 1);
-        dim3 grd((unsigned)((nbx + kFX - 3) / (kFX - 2)), (unsigned)((nby + kBY2 - 3) / (kBY2 - 2)), 1);
-        if (fv && exact) hipLaunchKernelGGL((fused2d_kernel<T, true, true, kBY2>), grd, blk, 0, s, a);
-        else if (fv) hipLaunchKernelGGL((fused2d_kernel<T, true, false, kBY2>), grd, blk, 0, s, a);
-        else if (exact) hipLaunchKernelGGL((fused2d_kernel<T, false, true, kBY2>), grd, blk, 0, s, a);
-        else hipLaunchKernelGGL((fused2d_kernel<T, false, false, kBY2>), grd, blk, 0, s, a);
+        a.ntx = (int32_t)((nbx + kFX - 3) / (kFX - 2));
+        a.ychunk = kYChunk2;
+        a.nchunks = (int32_t)((nby + kYChunk2 - 1) / kYChunk2);
+        dim3 blk(kFX, kW2, 1);
+        dim3 grd((unsigned)((a.ntx + kW2 - 1) / kW2), (unsigned)a.nchunks, 1);
+        if (fv && exact) hipLaunchKernelGGL((fused2d_kernel<T, true, true>), grd, blk, 0, s, a);
+        else if (fv) hipLaunchKernelGGL((fused2d_kernel<T, true, false>), grd, blk, 0, s, a);
+        else if (exact) hipLaunchKernelGGL((fused2d_kernel<T, false, true>), grd, blk, 0, s, a);
+        else hipLaunchKernelGGL((fused2d_kernel<T, false, false>), grd, blk, 0, s, a);
     }
     CHMY_HIP(hipGetLastError());
     return 0;
