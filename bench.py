@@ -157,6 +157,74 @@ def time_to_converge(args):
     print(json.dumps(out))
 
 
+def sia_gradient(args):
+    """SIA forward run versus gradient (checkpointed forward + reverse sweep)
+    wall time at 512² (300 steps, 30 checkpoints, fixed Δt), and the gradient
+    against central differences of J along 16 seeded Gaussian directions."""
+    import numpy as np
+    import torch
+    import chmy
+    from chmy import configs
+    from chmy.sia import SIA
+    arch = chmy.Arch(chmy.MI355XBackend())
+    g, B, la0, la1, mb, dt = configs.sia_ice_cap(arch, args.n)
+    nsteps, ncheck = args.sia_steps, args.sia_ckpt
+    dts = [dt] * nsteps
+    S = SIA(arch, g, B, logA=la1, As=0.0, npow=3, mb=mb)
+    S.reset()
+    S.forward(nsteps, dts)
+    H_obs = S.H.interior().clone()
+    dt_lim = S.dt_rule()
+    S.set_logA(la0)
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
+    def fwd():
+        S.reset()
+        S.forward(nsteps, dts)
+        return S.misfit(H_obs)
+
+    t_f, _ = timed(fwd, args.sia_reps)
+    out = {}
+    t_g, _ = timed(lambda: out.__setitem__("r", S.gradient(None, H_obs, nsteps, dts, ncheck)), args.sia_reps)
+    J, grad = out["r"]
+    log(f"forward {t_f * 1e3:.3f} ms, gradient {t_g * 1e3:.3f} ms, J = {J:.6e}")
+    gn = grad.double().cpu().numpy()
+    rng = np.random.default_rng(args.sia_seed)
+    rows = []
+    for d in range(args.sia_dirs):
+        v = rng.standard_normal(la0.shape)
+        h = args.sia_h
+        S.set_logA(la0 + h * v)
+        Jp = fwd()
+        S.set_logA(la0 - h * v)
+        Jm = fwd()
+        fd = (Jp - Jm) / (2 * h)
+        ad = float(np.sum(gn * v[2:-2, 2:-2].T))
+        rows.append({"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / abs(fd)})
+    S.set_logA(la0)
+    worst = max((r["rel_err"] for r in rows), default=None)
+    log(f"FD directions: worst rel err {worst}")
+    res = {"metric": "adjoint/fwd cost", "value": round(t_g / t_f, 3), "unit": "x forward wall time",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+           "config": {"workload": f"2D SIA ice cap, {args.n}^2 cells, {nsteps} steps of fixed dt {dt} a, "
+                                  f"{ncheck} checkpoints, gradient of J = 1/2|H - H_obs|^2 wrt log A",
+                      "forward_ms": round(t_f * 1e3, 3), "gradient_ms": round(t_g * 1e3, 3),
+                      "repeats": args.sia_reps, "J": J, "dt_limit_at_end": dt_lim},
+           "fd_check": {"directions": args.sia_dirs, "h": args.sia_h, "worst_rel_err": worst, "rows": rows}}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +243,14 @@ def main():
     ap.add_argument("--inner", type=int, default=200)
     ap.add_argument("--ncheck", type=int, default=200)
     ap.add_argument("--max-iter", type=int, default=400000)
+    ap.add_argument("--sia", action="store_true",
+                    help="SIA gradient cost and finite-difference check instead of the T_eff line")
+    ap.add_argument("--sia-steps", type=int, default=300)
+    ap.add_argument("--sia-ckpt", type=int, default=30)
+    ap.add_argument("--sia-dirs", type=int, default=16)
+    ap.add_argument("--sia-h", type=float, default=1e-5)
+    ap.add_argument("--sia-seed", type=int, default=16)
+    ap.add_argument("--sia-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -182,6 +258,11 @@ def main():
         sys.exit(2)
     if args.ttc:
         time_to_converge(args)
+        return
+    if args.sia:
+        if args.n == 255:
+            args.n = 512
+        sia_gradient(args)
         return
 
     import torch

@@ -8,6 +8,11 @@
   seeded positions (centres U[-0.5,0.5]², radii U[0.01,0.04], log10 η
   U[-3,3]; later inclusions overwrite earlier ones), background η = 1,
   G = 1, same dt rule, pure shear ±1.
+- sia_ice_cap: 2D SIA, [0, 250 km]², 8 seeded Gaussian bed bumps (heights
+  U(0, 1] km, centres U[0.3L, 0.7L]², widths U[30, 60] km), H0 = 0, mass
+  balance min(0.01 (S - 1500 m), 2 m/a) (ELA 1.5 km), n = 3, log A = log(1e-4)
+  plus, for the observation, a smooth seeded perturbation of σ = 0.3; fixed
+  Δt = 0.25 a.
 η is returned on the full padded storage (logical -1..n+2 per dim) evaluated
 at the fma cell-centre coordinates.
 """
@@ -48,3 +53,14 @@ def stokes2d_inclusions(arch, n=2047, *, count=32, seed=2047, rmin=0.01, rmax=0.
     for q in range(count):
         eta = np.where((X - cx[q]) ** 2 + (Y - cy[q]) ** 2 < rad[q] ** 2, 10.0 ** le[q], eta)
     return g, eta, dt_diffusion(g)
+
+
+def sia_ice_cap(arch, n=512, *, seed=0, sigma=0.3, dt=0.25, logA0=float(np.log(1e-4))):
+    """Grid, bed B, uniform log A, perturbed log A (for H_obs), mass-balance
+    triple and the fixed Δt of the SIA gradient workload (full storage arrays)."""
+    from .sia import bed_bumps, smooth_field
+    g = UniformGrid(arch, origin=(0.0, 0.0), extent=(250e3, 250e3), dims=(n, n))
+    L = g.extent[0]
+    B = bed_bumps(g, nbump=8, seed=seed, hmin=0.0, hmax=1000.0, smin=30e3, smax=60e3, cmin=0.3 * L, cmax=0.7 * L)
+    la = np.full((n + 4, n + 4), logA0)
+    return g, B, la, la + smooth_field(g, sigma, seed=seed + 1, corr=16), (0.01, 1500.0, 2.0), dt

@@ -5,10 +5,11 @@ Forward: the explicit loop of sia.jl:71-96 with the fused step kernel
 follows the reference's rule Δt = min(Δ²/max(D)/4.1, Δt_max) (sia.jl:76-78)
 and the sequence is recorded so the adjoint differentiates the same discrete
 map.  Gradient (build extension): J = ½‖H_N - H_obs‖² over the cells, d J /
-d log A by hand-written adjoint steps (chmy_sia_step_adjoint) in a reverse
-sweep with uniform checkpointing: `ncheck` stored states, each segment
-recomputed once from its checkpoint and kept in HBM while its adjoint steps
-run (the two-level revolve schedule).
+d log A by hand-written adjoint steps in a native reverse sweep
+(chmy_sia_reverse) with uniform checkpointing: `ncheck` stored states, each
+segment recomputed once from its checkpoint and kept in HBM while its adjoint
+steps run (the two-level revolve schedule); the recompute of the previous
+segment shares launches with the adjoint steps.
 """
 from __future__ import annotations
 
@@ -67,80 +68,118 @@ class SIA:
             set_storage(self.H, H0)
         self._neumann(self.H)
 
-    def diffusivity_max(self):
+    def diffusivity_max(self, H=None):
         g = self.grid
-        _lib.call("chmy_sia_diffusivity", C.byref(g.desc()), C.byref(self.D.desc()), C.byref(self.H.desc()),
+        H = H or self.H
+        _lib.call("chmy_sia_diffusivity", C.byref(g.desc()), C.byref(self.D.desc()), C.byref(H.desc()),
                   C.byref(self.B.desc()), C.byref(self.A.desc()) if self.A is not None else None, self.As,
                   self.npow, current_stream())
         return maxabs(self.D)
 
-    def dt_rule(self):
+    def dt_rule(self, H=None):
         """Δt = min(min(Δ)²/max(D)/4.1, Δt_max) (sia.jl:76-78)."""
-        dmax = self.diffusivity_max()
+        dmax = self.diffusivity_max(H)
         dt_diff = min(spacing(self.grid)) ** 2 / dmax / 4.1 if dmax > 0 else math.inf
         return min(dt_diff, self.dt_max)
 
+    def _args(self):
+        A = C.byref(self.A.desc()) if self.A is not None else None
+        return C.byref(self.B.desc()), A, self.As, self.npow, *self.mb
+
     def step(self, dt, H=None, H_out=None):
+        """One step as the reference composes it: the fused update_q! + q
+        Dirichlet + update_H! kernel, then the H Neumann batch."""
         g = self.grid
         H = H or self.H
         H_out = H_out or self.H2
         _lib.call("chmy_sia_step", C.byref(g.desc()), C.byref(H.desc()), C.byref(H_out.desc()),
-                  C.byref(self.B.desc()), C.byref(self.A.desc()) if self.A is not None else None, self.As,
-                  self.npow, *self.mb, float(dt), current_stream())
+                  *self._args(), float(dt), current_stream())
         self._neumann(H_out)
 
+    def run(self, states, dts):
+        """states[q] -> states[q+1] for the given Δt (one tiled launch per step,
+        Neumann batch included)."""
+        if len(states) < 2:
+            return
+        d = (C.c_double * (len(states) - 1))(*[float(x) for x in dts])
+        _lib.call("chmy_sia_run", C.byref(self.grid.desc()), _arr(states), len(states), *self._args(), d,
+                  current_stream())
+
+    def _field(self):
+        return Field(self.arch, self.grid, Center(), torch.float64)
+
     def forward(self, nsteps, dts=None, ncheck=0):
-        """Run nsteps from the current H; returns the Δt sequence.  With
-        ncheck > 0, stores ncheck uniformly spaced states for the adjoint."""
-        self.ckpt = []
+        """Run nsteps from the current H; returns the Δt sequence (the
+        reference's adaptive rule when dts is None, evaluated on each state).
+        With ncheck > 0 the state before every stride-th step, stride =
+        ceil(nsteps/ncheck), is written straight into a checkpoint field."""
         stride = max(1, -(-nsteps // ncheck)) if ncheck else 0
-        out_dts = []
-        for k in range(nsteps):
-            if stride and k % stride == 0:
-                self.ckpt.append((k, self.H.data.clone()))
-            dt = dts[k] if dts is not None else self.dt_rule()
-            out_dts.append(dt)
-            self.step(dt)
-            self.H.swap(self.H2)
+        nck = -(-nsteps // stride) if stride else 0
+        self.ckpt = getattr(self, "ckpt", [])[:nck]
+        while len(self.ckpt) < nck:
+            self.ckpt.append(self._field())
         self.stride = stride
-        return out_dts
+        work = [self.H, self.H2]
+        states = []
+        for k in range(nsteps + 1):
+            if stride and k % stride == 0 and k < nsteps:
+                states.append(self.ckpt[k // stride])
+            else:
+                states.append(work[k % 2])
+        if states[0] is not self.H:
+            states[0].data.copy_(self.H.data)
+        if dts is None:
+            out = []
+            for k in range(nsteps):
+                out.append(self.dt_rule(states[k]))
+                self.run(states[k:k + 2], out[-1:])
+        else:
+            out = [float(x) for x in dts[:nsteps]]
+            self.run(states, out)
+        if states[-1] is not self.H:
+            self.H.swap(self.H2)
+        return out
 
     def misfit(self, H_obs):
         d = self.H.interior() - H_obs
         return 0.5 * float(torch.sum(d * d))
 
     # ------------------------------------------------------------------ adjoint
-    def _adjoint_step(self, Hk, dt):
+    def _adjoint_step(self, Hk, dt, passes=1):
+        """lam <- (d step / d H)^T lam at state Hk, grad += the logA part;
+        passes=1 the tiled kernel, passes=5 the five gather kernels."""
         g = self.grid
-        if self._work_arr is None:
+        if passes == 5 and self._work_arr is None:
             self._work_arr = _arr(self.work)
         _lib.call("chmy_sia_step_adjoint", C.byref(g.desc()), C.byref(Hk.desc()), C.byref(self.lam.desc()),
                   C.byref(self.lam2.desc()), C.byref(self.grad.desc()) if self.A is not None else None,
                   C.byref(self.B.desc()), C.byref(self.A.desc()) if self.A is not None else None, self.As,
-                  self.npow, *self.mb, float(dt), self._work_arr, current_stream())
+                  self.npow, *self.mb, float(dt), self._work_arr if passes == 5 else None, current_stream())
         self.lam.swap(self.lam2)
 
     def gradient(self, H0, H_obs, nsteps, dts, ncheck):
-        """J(logA) and dJ/dlogA (cells) for the recorded Δt sequence."""
+        """J(logA) and dJ/dlogA (cells) for the recorded Δt sequence: the
+        checkpointed forward run, then the native reverse sweep."""
         self.reset(H0)
-        self.forward(nsteps, dts, ncheck)
+        dts = self.forward(nsteps, dts, ncheck)
         J = self.misfit(H_obs)
         self.grad.data.zero_()
         self.lam.data.zero_()
         self.lam2.data.zero_()
         self.lam.interior().copy_(self.H.interior() - H_obs)
-        seg_states = []
-        for s in reversed(range(len(self.ckpt))):
-            k0, state = self.ckpt[s]
-            k1 = self.ckpt[s + 1][0] if s + 1 < len(self.ckpt) else nsteps
-            # recompute the segment's states from its checkpoint (kept in HBM)
-            while len(seg_states) < k1 - k0:
-                seg_states.append(Field(self.arch, self.grid, Center(), torch.float64))
-            seg_states[0].data.copy_(state)
-            for q in range(1, k1 - k0):
-                self.step(dts[k0 + q - 1], seg_states[q - 1], seg_states[q])
-            for q in reversed(range(k1 - k0)):
-                self._adjoint_step(seg_states[q], dts[k0 + q])
+        stride = self.stride
+        self.seg = getattr(self, "seg", [])[:2 * (stride - 1)]
+        while len(self.seg) < 2 * (stride - 1):
+            self.seg.append(self._field())
+        d = (C.c_double * nsteps)(*dts)
+        slot = C.c_int(-1)
+        seg = _arr(self.seg) if self.seg else None
+        _lib.call("chmy_sia_reverse", C.byref(self.grid.desc()), _arr(self.ckpt), len(self.ckpt), stride, nsteps,
+                  seg, C.byref(self.lam.desc()), C.byref(self.lam2.desc()),
+                  C.byref(self.grad.desc()) if self.A is not None else None, *self._args(), d, C.byref(slot),
+                  current_stream())
+        if slot.value == 1:
+            self.lam.swap(self.lam2)
         g = self.grad.interior().clone() if self.A is not None else None
         return J, g
 

@@ -20,6 +20,8 @@
 //   A5 cells   lam' = mu + HDbar + Sbar, Sbar = iΔ(gbar_left - gbar_right) (+ M branch);
 //              halo cells fold into their neighbour (the Neumann copy H_0 = H_1).
 // Each pass is a gather, so no float atomics and the result is deterministic.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace chmy {
@@ -243,6 +245,273 @@ __global__ void __launch_bounds__(256) sia_adj5(SiaF<T> s, SiaW<T> w, DF<T> lo, 
     lo(i, j, 0) = l;
 }
 
+// ---------------------------------------------------------------- tiled one-pass forms
+// The forward step and the adjoint step as one launch each: a block owns a
+// kSX x kSY tile of cells and stages the lazily evaluated fields it needs in
+// LDS (D on the tile + 1 ring for the step; D on +3, mu on +2 and abar on +1
+// for the adjoint), so D is evaluated once per cell instead of once per use.
+// The arithmetic of every value is the one of the per-point kernels above,
+// so the results are bitwise equal to them.  With blockIdx.z = 1 a launch of
+// the adjoint also advances an independent forward step (the recompute of
+// the previous checkpoint segment), which fills the GPU at small grids.
+constexpr int kSX = 32, kSY = 16, kST = 256;
+
+// S = B + H staged in LDS on the tile + R ring (zero outside cells 0..n+1,
+// which no evaluation reads); the formulas are those of SiaF.
+template <class T, int R>
+struct SiaLds {
+    static constexpr int W = kSX + 2 * R, HH = kSY + 2 * R;
+    const T* Ss;
+    int64_t i0, j0;
+    const SiaF<T>* f;
+    __device__ static void load(const SiaF<T>& s, T* Ss, int64_t i0, int64_t j0, int64_t nx, int64_t ny) {
+        for (int q = threadIdx.x; q < W * HH; q += kST) {
+            const int a = q % W, b = q / W;
+            const int64_t i = i0 - R + a, j = j0 - R + b;
+            Ss[q] = (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1) ? s.S(i, j) : T(0);
+        }
+    }
+    __device__ T S(int64_t i, int64_t j) const { return Ss[(j - j0 + R) * W + (i - i0 + R)]; }
+    __device__ T gx(int64_t i, int64_t j) const { return (S(i, j) - S(i - 1, j)) * f->ix; }
+    __device__ T gy(int64_t i, int64_t j) const { return (S(i, j) - S(i, j - 1)) * f->iy; }
+    __device__ T ax(int64_t i, int64_t j) const { T l = gx(i, j), r = gx(i + 1, j); return fma(T(0.5), r - l, l); }
+    __device__ T ay(int64_t i, int64_t j) const { T l = gy(i, j), r = gy(i, j + 1); return fma(T(0.5), r - l, l); }
+    __device__ T D(int64_t i, int64_t j) const {
+        const T h = f->H(i, j, 0);
+        const T a = f->hasA ? f->A(i, j, 0) : T(1);
+        const T u = ax(i, j), w = ay(i, j);
+        const T vm = sqrt(u * u + w * w);
+        return (a * ipow_(h, f->n + 2) + f->As * ipow_(h, f->n)) * ipow_(vm, f->n - 1);
+    }
+    __device__ T M(int64_t i, int64_t j) const {
+        const T m = f->mba * (S(i, j) - f->mbb);
+        return m < f->mbc ? m : f->mbc;
+    }
+};
+
+template <class T>
+struct SiaStep {   // one forward step H -> Ho (+ the H Neumann batch when bc != 0)
+    SiaF<T> s;
+    DF<T> Ho;
+    T dt;
+    int bc;
+};
+
+template <class T>
+__device__ __forceinline__ void sia_step_block(const SiaStep<T>& p, int64_t nx, int64_t ny, T* lds) {
+    constexpr int W = kSX + 2, Hh = kSY + 2;
+    using LS = SiaLds<T, 2>;
+    T(*Ds)[W] = reinterpret_cast<T(*)[W]>(lds + LS::W * LS::HH);
+    const SiaF<T>& s = p.s;
+    const int tid = threadIdx.x;
+    const int64_t i0 = 1 + (int64_t)blockIdx.x * kSX, j0 = 1 + (int64_t)blockIdx.y * kSY;
+    LS::load(s, lds, i0, j0, nx, ny);
+    const LS L{lds, i0, j0, &s};
+    __syncthreads();
+    for (int q = tid; q < W * Hh; q += kST) {
+        const int a = q % W, b = q / W;
+        const int64_t i = i0 - 1 + a, j = j0 - 1 + b;
+        Ds[b][a] = (i >= 1 && i <= nx && j >= 1 && j <= ny) ? L.D(i, j) : T(0);
+    }
+    __syncthreads();
+    for (int q = tid; q < kSX * kSY; q += kST) {
+        const int a = q % kSX + 1, b = q / kSX + 1;   // LDS position of the cell
+        const int64_t i = i0 + a - 1, j = j0 + b - 1;
+        if (i > nx || j > ny) continue;
+        auto qx = [&](int64_t f, int la) -> T {   // x-vertex f, right cell at LDS column la
+            if (f <= 1 || f >= nx + 1) return T(0);
+            const T Dl = Ds[b][la - 1], Dr = Ds[b][la];
+            return -fma(T(0.5), Dr - Dl, Dl) * L.gx(f, j);
+        };
+        auto qy = [&](int64_t f, int lb) -> T {
+            if (f <= 1 || f >= ny + 1) return T(0);
+            const T Dl = Ds[lb - 1][a], Dr = Ds[lb][a];
+            return -fma(T(0.5), Dr - Dl, Dl) * L.gy(i, f);
+        };
+        const T dv = (qx(i + 1, a + 1) - qx(i, a)) * s.ix + (qy(j + 1, b + 1) - qy(j, b)) * s.iy;
+        const T h = jmax0(s.H(i, j, 0) + p.dt * (-dv + L.M(i, j)));
+        const DF<T>& Ho = p.Ho;
+        Ho(i, j, 0) = h;
+        if (p.bc) {
+            // Neumann(0) batch, dims y then x (batch.jl:20-29): ghosts copy the
+            // adjacent cell (fma(Δ, -0, f) == f), corners the diagonal cell
+            if (i == 1) Ho(0, j, 0) = h;
+            if (i == nx) Ho(nx + 1, j, 0) = h;
+            if (j == 1) {
+                Ho(i, 0, 0) = h;
+                if (i == 1) Ho(0, 0, 0) = h;
+                if (i == nx) Ho(nx + 1, 0, 0) = h;
+            }
+            if (j == ny) {
+                Ho(i, ny + 1, 0) = h;
+                if (i == 1) Ho(0, ny + 1, 0) = h;
+                if (i == nx) Ho(nx + 1, ny + 1, 0) = h;
+            }
+        }
+    }
+}
+
+template <class T>
+struct SiaAdj {    // one adjoint step at state s.H: lam -> lo, gA += dJ/dlog A
+    SiaF<T> s;
+    DF<T> lam, lo, gA;
+    T dt;
+};
+
+template <class T>
+__device__ __forceinline__ void sia_adj_block(const SiaAdj<T>& p, int64_t nx, int64_t ny, T* lds) {
+    constexpr int W3 = kSX + 6, H3 = kSY + 6, W2 = kSX + 4, H2 = kSY + 4, W1 = kSX + 2, H1 = kSY + 2;
+    using LS = SiaLds<T, 4>;
+    T* base = lds + LS::W * LS::HH;
+    T(*Ds)[W3] = reinterpret_cast<T(*)[W3]>(base);
+    T(*Ms)[W2] = reinterpret_cast<T(*)[W2]>(base + W3 * H3);
+    T(*Ax)[W1] = reinterpret_cast<T(*)[W1]>(base + W3 * H3 + W2 * H2);
+    T(*Ay)[W1] = reinterpret_cast<T(*)[W1]>(base + W3 * H3 + W2 * H2 + W1 * H1);
+    T(*HDs)[kSX] = reinterpret_cast<T(*)[kSX]>(base + W3 * H3 + W2 * H2 + 2 * W1 * H1);
+    const SiaF<T>& s = p.s;
+    const T dt = p.dt;
+    const int tid = threadIdx.x;
+    const int64_t i0 = 1 + (int64_t)blockIdx.x * kSX, j0 = 1 + (int64_t)blockIdx.y * kSY;
+    LS::load(s, lds, i0, j0, nx, ny);
+    const LS L{lds, i0, j0, &s};
+    __syncthreads();
+    auto inside = [nx, ny](int64_t i, int64_t j) { return i >= 1 && i <= nx && j >= 1 && j <= ny; };
+    // D on the tile + 3 ring; cell (i, j) at Ds[j - j0 + 3][i - i0 + 3]
+    for (int q = tid; q < W3 * H3; q += kST) {
+        const int a = q % W3, b = q / W3;
+        const int64_t i = i0 - 3 + a, j = j0 - 3 + b;
+        Ds[b][a] = inside(i, j) ? L.D(i, j) : T(0);
+    }
+    __syncthreads();
+    // pass A1: mu on the tile + 2 ring; cell at Ms[j - j0 + 2][i - i0 + 2]
+    for (int q = tid; q < W2 * H2; q += kST) {
+        const int a = q % W2, b = q / W2;
+        const int64_t i = i0 - 2 + a, j = j0 - 2 + b;
+        T mu = T(0);
+        if (inside(i, j)) {
+            const int da = a + 1, db = b + 1;   // the cell in Ds
+            auto qx = [&](int64_t f, int la) -> T {
+                if (f <= 1 || f >= nx + 1) return T(0);
+                const T Dl = Ds[db][la - 1], Dr = Ds[db][la];
+                return -fma(T(0.5), Dr - Dl, Dl) * L.gx(f, j);
+            };
+            auto qy = [&](int64_t f, int lb) -> T {
+                if (f <= 1 || f >= ny + 1) return T(0);
+                const T Dl = Ds[lb - 1][da], Dr = Ds[lb][da];
+                return -fma(T(0.5), Dr - Dl, Dl) * L.gy(i, f);
+            };
+            const T dv = (qx(i + 1, da + 1) - qx(i, da)) * s.ix + (qy(j + 1, db + 1) - qy(j, db)) * s.iy;
+            const T x = s.H(i, j, 0) + dt * (-dv + L.M(i, j));
+            mu = x > T(0) ? p.lam(i, j, 0) : T(0);
+        }
+        Ms[b][a] = mu;
+    }
+    __syncthreads();
+    // passes A2+A3: Dbar and abar on the tile + 1 ring; HD and grad on the tile
+    for (int q = tid; q < W1 * H1; q += kST) {
+        const int a = q % W1, b = q / W1;
+        const int64_t i = i0 - 1 + a, j = j0 - 1 + b;
+        T abx = T(0), aby = T(0);
+        if (inside(i, j)) {
+            const int ma = a + 1, mb = b + 1;   // the cell in Ms
+            auto Dvx = [&](int64_t f, int la) -> T {   // x-vertex f, right cell at Ms column la
+                if (f < 2 || f > nx) return T(0);
+                const T qb = dt * s.ix * (Ms[mb][la] - Ms[mb][la - 1]);
+                return -L.gx(f, j) * qb;
+            };
+            auto Dvy = [&](int64_t f, int lb) -> T {
+                if (f < 2 || f > ny) return T(0);
+                const T qb = dt * s.iy * (Ms[lb][ma] - Ms[lb - 1][ma]);
+                return -L.gy(i, f) * qb;
+            };
+            const T Db = T(0.5) * (Dvx(i, ma) + Dvx(i + 1, ma + 1)) + T(0.5) * (Dvy(j, mb) + Dvy(j + 1, mb + 1));
+            const int n = s.n;
+            const T h = s.H(i, j, 0);
+            const T av = s.hasA ? s.A(i, j, 0) : T(1);
+            const T u = L.ax(i, j), v = L.ay(i, j);
+            const T vm = sqrt(u * u + v * v);
+            const T P = ipow_(vm, n - 1);
+            const T Cc = av * ipow_(h, n + 2) + s.As * ipow_(h, n);
+            const bool tile = a >= 1 && a <= kSX && b >= 1 && b <= kSY;
+            if (tile) {
+                if (s.hasA) p.gA(i, j, 0) = p.gA(i, j, 0) + av * (Db * ipow_(h, n + 2) * P);
+                HDs[b - 1][a - 1] = Db * (T(n + 2) * av * ipow_(h, n + 1) + T(n) * s.As * ipow_(h, n - 1)) * P;
+            }
+            T coef = T(0);
+            if (n >= 3) coef = T(n - 1) * ipow_(vm, n - 3);
+            else if (n == 2 && vm > T(0)) coef = T(1) / vm;
+            const T Pb = Db * Cc;
+            abx = Pb * coef * u;
+            aby = Pb * coef * v;
+        }
+        Ax[b][a] = abx;
+        Ay[b][a] = aby;
+    }
+    __syncthreads();
+    // passes A4+A5: gbar on the 4 faces of each tile cell, then lam'
+    for (int q = tid; q < kSX * kSY; q += kST) {
+        const int t = q % kSX, u = q / kSX;
+        const int64_t i = i0 + t, j = j0 + u;
+        if (i > nx || j > ny) continue;
+        const int a1 = t + 1, b1 = u + 1;   // cell in Ax/Ay
+        const int a2 = t + 2, b2 = u + 2;   // cell in Ms
+        const int a3 = t + 3, b3 = u + 3;   // cell in Ds
+        auto gbx = [&](int64_t f, int o) -> T {   // x-vertex f = i + o, o in {0, 1}
+            T g = T(0);
+            if (f >= 2 && f <= nx) {
+                const T qb = dt * s.ix * (Ms[b2][a2 + o] - Ms[b2][a2 + o - 1]);
+                const T Dl = Ds[b3][a3 + o - 1], Dr = Ds[b3][a3 + o];
+                g = -fma(T(0.5), Dr - Dl, Dl) * qb;
+            }
+            if (f - 1 >= 1) g = g + T(0.5) * Ax[b1][a1 + o - 1];
+            if (f <= nx) g = g + T(0.5) * Ax[b1][a1 + o];
+            return g;
+        };
+        auto gby = [&](int64_t f, int o) -> T {
+            T g = T(0);
+            if (f >= 2 && f <= ny) {
+                const T qb = dt * s.iy * (Ms[b2 + o][a2] - Ms[b2 + o - 1][a2]);
+                const T Dl = Ds[b3 + o - 1][a3], Dr = Ds[b3 + o][a3];
+                g = -fma(T(0.5), Dr - Dl, Dl) * qb;
+            }
+            if (f - 1 >= 1) g = g + T(0.5) * Ay[b1 + o - 1][a1];
+            if (f <= ny) g = g + T(0.5) * Ay[b1 + o][a1];
+            return g;
+        };
+        const T gx0 = gbx(i, 0), gx1 = gbx(i + 1, 1), gy0 = gby(j, 0), gy1 = gby(j + 1, 1);
+        T r = T(0);
+        r = r + s.ix * gx0;
+        r = r - s.ix * gx1;
+        r = r + s.iy * gy0;
+        r = r - s.iy * gy1;
+        const T mu = Ms[b2][a2];
+        T l = mu + HDs[u][t] + r;
+        const T m = s.mba * (L.S(i, j) - s.mbb);
+        if (m < s.mbc) l = l + dt * mu * s.mba;
+        if (i == 1) l = l + (-s.ix * gx0);
+        if (i == nx) l = l + s.ix * gx1;
+        if (j == 1) l = l + (-s.iy * gy0);
+        if (j == ny) l = l + s.iy * gy1;
+        p.lo(i, j, 0) = l;
+    }
+}
+
+constexpr int kAdjLds = (kSX + 8) * (kSY + 8) + (kSX + 6) * (kSY + 6) + (kSX + 4) * (kSY + 4) + 2 * (kSX + 2) * (kSY + 2) + kSX * kSY;
+
+template <class T>
+__global__ void __launch_bounds__(kST) sia_step_tile(SiaStep<T> f, int64_t nx, int64_t ny) {
+    __shared__ T lds[(kSX + 4) * (kSY + 4) + (kSX + 2) * (kSY + 2)];
+    sia_step_block(f, nx, ny, lds);
+}
+
+// blockIdx.z == 0: adjoint tile; blockIdx.z == 1: forward tile of an independent step
+template <class T>
+__global__ void __launch_bounds__(kST) sia_adj_tile(SiaAdj<T> a, SiaStep<T> f, int64_t nx, int64_t ny) {
+    __shared__ T lds[kAdjLds];
+    if (blockIdx.z == 0) sia_adj_block(a, nx, ny, lds);
+    else sia_step_block(f, nx, ny, lds);
+}
+
 template <class T>
 static SiaF<T> make_sia(const chmy_grid_t* g, const chmy_field_t* H, const chmy_field_t* B, const chmy_field_t* A,
                         const SiaPar& p) {
@@ -270,6 +539,38 @@ static int sia_check(const chmy_grid_t* g, const chmy_field_t* H, const chmy_fie
     CHMY_TRY(check_center(g, B, "B", CHMY_F64));
     if (A) CHMY_TRY(check_center(g, A, "A", CHMY_F64));
     // D at cell 0 reads S at -1: halo 1 covers logical -1..n+2
+    return 0;
+}
+
+static dim3 sia_tiles(int64_t nx, int64_t ny, int z) {
+    return dim3((unsigned)((nx + kSX - 1) / kSX), (unsigned)((ny + kSY - 1) / kSY), (unsigned)z);
+}
+
+static SiaStep<double> sia_stepper(const chmy_grid_t* g, const chmy_field_t* H, const chmy_field_t* Ho,
+                                   const chmy_field_t* B, const chmy_field_t* A, const SiaPar& p, double dt) {
+    SiaStep<double> f;
+    f.s = make_sia<double>(g, H, B, A, p);
+    f.Ho = view<double>(*Ho);
+    f.dt = dt;
+    f.bc = 1;
+    return f;
+}
+
+static SiaAdj<double> sia_adjointer(const chmy_grid_t* g, const chmy_field_t* H, const chmy_field_t* lam,
+                                    const chmy_field_t* lo, const chmy_field_t* gA, const chmy_field_t* B,
+                                    const chmy_field_t* A, const SiaPar& p, double dt) {
+    SiaAdj<double> a;
+    a.s = make_sia<double>(g, H, B, A, p);
+    a.lam = view<double>(*lam);
+    a.lo = view<double>(*lo);
+    a.gA = gA ? view<double>(*gA) : a.lo;
+    a.dt = dt;
+    return a;
+}
+
+static int sia_check_states(const chmy_grid_t* g, const chmy_field_t* st, int n, const char* what) {
+    if (!st) return set_error(CHMY_E_ARG, "%s is NULL", what);
+    for (int q = 0; q < n; ++q) CHMY_TRY(check_center(g, &st[q], what, CHMY_F64));
     return 0;
 }
 
@@ -344,7 +645,14 @@ int chmy_sia_step_adjoint(const chmy_grid_t* g, const chmy_field_t* H, const chm
     CHMY_TRY(check_center(g, lam_out, "lam_out", CHMY_F64));
     if (lam_in->data == lam_out->data) return set_error(CHMY_E_ARG, "lam_in and lam_out must differ");
     if (A) CHMY_TRY(check_center(g, grad_logA, "grad_logA", CHMY_F64));
-    if (!work) return set_error(CHMY_E_ARG, "work is NULL");
+    SiaPar p{As, npow, mb_a, mb_b, mb_c, dt};
+    if (!work) {   // one tiled pass
+        SiaAdj<double> a = sia_adjointer(g, H, lam_in, lam_out, A ? grad_logA : nullptr, B, A, p, dt);
+        hipLaunchKernelGGL(sia_adj_tile<double>, sia_tiles(g->n[0], g->n[1], 1), dim3(kST), 0, (hipStream_t)stream,
+                           a, SiaStep<double>{}, g->n[0], g->n[1]);
+        CHMY_HIP(hipGetLastError());
+        return 0;
+    }
     const int vx[3] = {1, 0, 0}, vy[3] = {0, 1, 0};
     CHMY_TRY(check_center(g, &work[0], "work.mu", CHMY_F64));
     CHMY_TRY(check_field(g, &work[1], "work.Dvx", vx, CHMY_F64));
@@ -352,7 +660,6 @@ int chmy_sia_step_adjoint(const chmy_grid_t* g, const chmy_field_t* H, const chm
     CHMY_TRY(check_field(g, &work[3], "work.Dvy", vy, CHMY_F64));
     CHMY_TRY(check_field(g, &work[4], "work.gdy", vy, CHMY_F64));
     for (int q = 5; q < 8; ++q) CHMY_TRY(check_center(g, &work[q], "work", CHMY_F64));
-    SiaPar p{As, npow, mb_a, mb_b, mb_c, dt};
     SiaF<double> s = make_sia<double>(g, H, B, A, p);
     SiaW<double> w{view<double>(work[0]), view<double>(work[1]), view<double>(work[2]), view<double>(work[3]),
                    view<double>(work[4]), view<double>(work[5]), view<double>(work[6]), view<double>(work[7])};
@@ -365,6 +672,85 @@ int chmy_sia_step_adjoint(const chmy_grid_t* g, const chmy_field_t* H, const chm
     hipLaunchKernelGGL(sia_adj4<double>, sia_grid(nx, ny), sia_blk(), 0, st, w, nx, ny);
     hipLaunchKernelGGL(sia_adj5<double>, sia_grid(nx, ny), sia_blk(), 0, st, s, w, view<double>(*lam_out), nx, ny, dt);
     CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+int chmy_sia_run(const chmy_grid_t* g, const chmy_field_t* states, int nstates, const chmy_field_t* B,
+                 const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
+                 const double* dts, void* stream) {
+    if (nstates < 1) return set_error(CHMY_E_ARG, "nstates=%d < 1", nstates);
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(sia_check_states(g, states, nstates, "states"));
+    CHMY_TRY(sia_check(g, &states[0], B, A, npow));
+    if (nstates > 1 && !dts) return set_error(CHMY_E_ARG, "dts is NULL");
+    for (int q = 0; q + 1 < nstates; ++q)
+        if (states[q].data == states[q + 1].data)
+            return set_error(CHMY_E_ARG, "states[%d] and states[%d] share storage", q, q + 1);
+    const SiaPar p{As, npow, mb_a, mb_b, mb_c, 0.0};
+    hipStream_t st = (hipStream_t)stream;
+    for (int q = 0; q + 1 < nstates; ++q) {
+        SiaStep<double> f = sia_stepper(g, &states[q], &states[q + 1], B, A, p, dts[q]);
+        hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(g->n[0], g->n[1], 1), dim3(kST), 0, st, f, g->n[0],
+                           g->n[1]);
+    }
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
+                     const chmy_field_t* seg, const chmy_field_t* lam_a, const chmy_field_t* lam_b,
+                     const chmy_field_t* grad_logA, const chmy_field_t* B, const chmy_field_t* A, double As, int npow,
+                     double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot, void* stream) {
+    if (nsteps < 1 || stride < 1) return set_error(CHMY_E_ARG, "nsteps=%d stride=%d", nsteps, stride);
+    const int nseg = (nsteps + stride - 1) / stride;
+    if (nckpt != nseg) return set_error(CHMY_E_ARG, "nckpt=%d, need ceil(nsteps/stride)=%d", nckpt, nseg);
+    if (!dts || !lam_slot) return set_error(CHMY_E_ARG, "dts or lam_slot is NULL");
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(sia_check_states(g, ckpt, nckpt, "ckpt"));
+    CHMY_TRY(sia_check(g, &ckpt[0], B, A, npow));
+    if (stride > 1) CHMY_TRY(sia_check_states(g, seg, 2 * (stride - 1), "seg"));
+    CHMY_TRY(check_center(g, lam_a, "lam_a", CHMY_F64));
+    CHMY_TRY(check_center(g, lam_b, "lam_b", CHMY_F64));
+    if (lam_a->data == lam_b->data) return set_error(CHMY_E_ARG, "lam_a and lam_b must differ");
+    if (A) CHMY_TRY(check_center(g, grad_logA, "grad_logA", CHMY_F64));
+    const SiaPar p{As, npow, mb_a, mb_b, mb_c, 0.0};
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nx = g->n[0], ny = g->n[1];
+    // state q of segment s: q = 0 its checkpoint, q >= 1 seg[(s % 2)(stride - 1) + q - 1]
+    auto state = [&](int sg, int q) -> const chmy_field_t* {
+        return q == 0 ? &ckpt[sg] : &seg[(sg % 2) * (stride - 1) + q - 1];
+    };
+    auto len = [&](int sg) { return std::min(stride, nsteps - sg * stride); };
+    // recompute the last segment on its own
+    for (int q = 1; q < len(nseg - 1); ++q) {
+        SiaStep<double> f = sia_stepper(g, state(nseg - 1, q - 1), state(nseg - 1, q), B, A, p,
+                                        dts[(nseg - 1) * stride + q - 1]);
+        hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(nx, ny, 1), dim3(kST), 0, st, f, nx, ny);
+    }
+    int slot = 0;   // lam_in of the next adjoint step: 0 = lam_a, 1 = lam_b
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int L = len(sg), Lp = sg > 0 ? len(sg - 1) : 0;
+        for (int r = 0; r < std::max(L, Lp - 1); ++r) {
+            // the recompute of segment sg - 1 (state r -> r + 1) rides along with
+            // adjoint step q = L - 1 - r of segment sg
+            const bool fw = sg > 0 && r + 1 < Lp;
+            SiaStep<double> f{};
+            if (fw) f = sia_stepper(g, state(sg - 1, r), state(sg - 1, r + 1), B, A, p, dts[(sg - 1) * stride + r]);
+            if (r >= L) {
+                hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(nx, ny, 1), dim3(kST), 0, st, f, nx, ny);
+                continue;
+            }
+            const int q = L - 1 - r;
+            const chmy_field_t* li = slot ? lam_b : lam_a;
+            const chmy_field_t* lo = slot ? lam_a : lam_b;
+            SiaAdj<double> a = sia_adjointer(g, state(sg, q), li, lo, A ? grad_logA : nullptr, B, A, p,
+                                             dts[sg * stride + q]);
+            hipLaunchKernelGGL(sia_adj_tile<double>, sia_tiles(nx, ny, fw ? 2 : 1), dim3(kST), 0, st, a, f, nx, ny);
+            slot ^= 1;
+        }
+    }
+    CHMY_HIP(hipGetLastError());
+    *lam_slot = slot;
     return 0;
 }
 

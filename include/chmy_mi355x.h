@@ -252,12 +252,34 @@ int chmy_sia_step(const chmy_grid_t* grid, const chmy_field_t* H, const chmy_fie
 /* Adjoint of chmy_sia_step + Neumann batch (build extension): given the step's
  * input state H (with halos) and lam_in = dJ/dH_out (cells), writes
  * lam_out = dJ/dH (cells) and adds dJ/dlog(A) into grad_logA (if A != NULL).
- * work: 8 scratch fields {mu: Center, Dvx, gdx: (Vertex,Center),
- * Dvy, gdy: (Center,Vertex), ax, ay, HD: Center}. */
+ * work == NULL: one tiled pass (LDS-staged D, mu, abar).  Otherwise five
+ * gather passes through 8 scratch fields {mu: Center, Dvx, gdx: (Vertex,Center),
+ * Dvy, gdy: (Center,Vertex), ax, ay, HD: Center}; both give the same bits. */
 int chmy_sia_step_adjoint(const chmy_grid_t* grid, const chmy_field_t* H, const chmy_field_t* lam_in,
                           const chmy_field_t* lam_out, const chmy_field_t* grad_logA, const chmy_field_t* B,
                           const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
                           double dt, const chmy_field_t* work, void* stream);
+/* The time loop of sia.jl:71-96 with a given Δt sequence (the reference's
+ * adaptive rule evaluated beforehand, or fixed): states[q] -> states[q+1]
+ * by one step followed by the H Neumann batch (sia.jl:81), both in one tiled
+ * launch, for q = 0..nstates-2.  Consecutive states must not share storage;
+ * a plain run alternates two fields, a checkpointed run passes its
+ * checkpoint fields at the stored steps (build extension: no copies). */
+int chmy_sia_run(const chmy_grid_t* grid, const chmy_field_t* states, int nstates, const chmy_field_t* B,
+                 const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
+                 const double* dts, void* stream);
+/* Reverse sweep of a checkpointed run (build extension): ckpt[s] holds the
+ * state before step s*stride (nckpt = ceil(nsteps/stride)); each segment is
+ * recomputed once into seg (2*(stride-1) Center fields, two alternating
+ * halves), the recompute of segment s-1 riding in the same launches as the
+ * adjoint steps of segment s.  lam_a holds dJ/dH_N on entry; on return the
+ * field named by *lam_slot (0 = lam_a, 1 = lam_b) holds dJ/dH_0, and
+ * grad_logA has dJ/dlog(A) added (A != NULL). */
+int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
+                     const chmy_field_t* seg, const chmy_field_t* lam_a, const chmy_field_t* lam_b,
+                     const chmy_field_t* grad_logA, const chmy_field_t* B, const chmy_field_t* A, double As,
+                     int npow, double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot,
+                     void* stream);
 
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),

@@ -84,6 +84,85 @@ def test_sia_fused_step_equals_reference_sequence(arch):
     assert dmax == O.sia_Dmax(og, hH, hB, A=hA, As=0.5, npow=3)
 
 
+def test_sia_run_equals_step_and_neumann_sequence(arch):
+    """chmy_sia_run (step + Neumann batch in one tiled launch, Δt per step)
+    gives the same bits as chmy_sia_step followed by the Neumann batch, on a
+    grid that is not a multiple of the tile (45 x 21), for 15 steps."""
+    n = (45, 21)
+    g, og, H, hH, B, hB, A, hA = _setup(arch, n=n, seed=7)
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    T = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    S.reset(hH.arr)
+    T.reset(hH.arr)
+    dts = [2e-3 * (1 + 0.1 * k) for k in range(15)]
+    for dt in dts:
+        S.step(dt)
+        S.H.swap(S.H2)
+    T.forward(15, dts)
+    assert np.array_equal(dev(T.H), dev(S.H))
+    # checkpointed run: same final state, checkpoints equal the stepped states
+    T.reset(hH.arr)
+    T.forward(15, dts, ncheck=4)
+    assert np.array_equal(dev(T.H), dev(S.H))
+    assert T.stride == 4 and len(T.ckpt) == 4
+    S.reset(hH.arr)
+    for k, dt in enumerate(dts):
+        if k % 4 == 0:
+            assert np.array_equal(dev(T.ckpt[k // 4]), dev(S.H)), k
+        S.step(dt)
+        S.H.swap(S.H2)
+
+
+@pytest.mark.parametrize("n", [(45, 21), (64, 32)])
+def test_sia_adjoint_tiled_equals_five_pass(arch, n):
+    """The tiled one-launch adjoint step and the five gather passes give the
+    same bits (λ and the log A gradient), interior and tile edges included."""
+    g, og, H, hH, B, hB, A, hA = _setup(arch, n=n, seed=11)
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    rng = np.random.default_rng(2)
+    lam0 = torch.from_numpy(rng.standard_normal((n[1], n[0])).T.copy()).to(S.lam.data.device)
+    out = []
+    for passes in (1, 5):
+        S.reset(hH.arr)
+        S.grad.data.zero_()
+        S.lam.data.zero_()
+        S.lam2.data.zero_()
+        S.lam.interior().copy_(lam0)
+        S._adjoint_step(S.H, 2e-3, passes=passes)
+        out.append((S.lam.interior().cpu().numpy().copy(), S.grad.interior().cpu().numpy().copy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_sia_reverse_sweep_equals_stepwise_adjoint(arch):
+    """chmy_sia_reverse (checkpoints, segment recompute riding along the
+    adjoint launches, a short last segment) equals storing every state and
+    applying the adjoint step by step, bitwise."""
+    n = (40, 33)
+    g, og, H, hH, B, hB, A, hA = _setup(arch, n=n, seed=3)
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    nsteps = 23
+    S.reset(hH.arr)
+    dts = S.forward(nsteps)
+    H_obs = S.H.interior().clone() * 0.97
+    J, grad = S.gradient(hH.arr, H_obs, nsteps, dts, ncheck=5)   # stride 5, last segment 3 steps
+    lam_rev = S.lam.interior().cpu().numpy().copy()
+    # stepwise: every state stored
+    S.reset(hH.arr)
+    states = [S._field() for _ in range(nsteps + 1)]
+    states[0].data.copy_(S.H.data)
+    for k in range(nsteps):
+        S.run(states[k:k + 2], dts[k:k + 1])
+    S.grad.data.zero_()
+    S.lam.data.zero_()
+    S.lam2.data.zero_()
+    S.lam.interior().copy_(states[-1].interior() - H_obs)
+    for k in reversed(range(nsteps)):
+        S._adjoint_step(states[k], dts[k])
+    assert np.array_equal(S.lam.interior().cpu().numpy(), lam_rev)
+    assert np.array_equal(S.grad.interior().cpu().numpy(), grad.cpu().numpy())
+
+
 def test_sia_adjoint_step_equals_fd_jacobian_transpose(arch):
     """One adjoint step equals J^T λ with J the Jacobian of the GPU forward
     step + Neumann batch, by central differences on a 9x7 grid (state chosen
@@ -127,15 +206,16 @@ def test_sia_adjoint_step_equals_fd_jacobian_transpose(arch):
     S.set_logA(logA)
     H = H0.copy()
     S.reset(H)
-    S.grad.data.zero_()
-    S.lam.data.zero_()
-    S.lam2.data.zero_()
-    S.lam.interior().copy_(torch.from_numpy(lam.reshape(n[1], n[0]).T.copy()))
-    S._adjoint_step(S.H, dt)
-    lam_out = S.lam.interior().cpu().numpy().T.reshape(-1)
-    gA = S.grad.interior().cpu().numpy().T.reshape(-1)
-    assert np.allclose(lam_out, Jh.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Jh.T @ lam).max())
-    assert np.allclose(gA, Ja.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Ja.T @ lam).max())
+    for passes in (1, 5):
+        S.grad.data.zero_()
+        S.lam.data.zero_()
+        S.lam2.data.zero_()
+        S.lam.interior().copy_(torch.from_numpy(lam.reshape(n[1], n[0]).T.copy()))
+        S._adjoint_step(S.H, dt, passes=passes)
+        lam_out = S.lam.interior().cpu().numpy().T.reshape(-1)
+        gA = S.grad.interior().cpu().numpy().T.reshape(-1)
+        assert np.allclose(lam_out, Jh.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Jh.T @ lam).max()), passes
+        assert np.allclose(gA, Ja.T @ lam, rtol=1e-6, atol=1e-7 * np.abs(Ja.T @ lam).max()), passes
 
 
 def test_sia_gradient_matches_central_differences(arch):
