@@ -225,6 +225,19 @@ def sia_gradient(args):
     print(json.dumps(res))
 
 
+def copy_calibration(args):
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch): the reference point for scaling HBM counter bytes."""
+    import torch
+    nbytes = 1 << 30
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": args.steps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,11 +264,16 @@ def main():
     ap.add_argument("--sia-h", type=float, default=1e-5)
     ap.add_argument("--sia-seed", type=int, default=16)
     ap.add_argument("--sia-reps", type=int, default=5)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is out of scope)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        copy_calibration(args)
+        return
     if args.ttc:
         time_to_converge(args)
         return

@@ -92,6 +92,21 @@ class PTStokes:
         self.exact = bool(exact)
 
     # ------------------------------------------------------------------ state
+    def set_eta(self, eta_field):
+        """Replace the viscosity field (full storage array) and its maxloc."""
+        set_storage(self.eta, eta_field)
+        Launcher(self.arch, self.grid)(self.arch, self.grid, (maxloc, (self.eta_ml, self.eta)))
+
+    def reset_state(self, V_init=None):
+        """Zero (V, τ, Pr) in both buffer sets, optionally start V from
+        V_init(grid, V), then apply the V batch and sync the buffers."""
+        for f in list(self.V) + list(self.tau) + [self.Pr] + list(self.V2) + list(self.tau2) + [self.Pr2]:
+            f.data.zero_()
+        if V_init is not None:
+            V_init(self.grid, self.V)
+        self.apply_bc()
+        self.sync_buffers()
+
     def sync_buffers(self):
         """Make the output buffers equal to the inputs (also outside the box)."""
         for a, b in zip(list(self.V) + list(self.tau) + [self.Pr], list(self.V2) + list(self.tau2) + [self.Pr2]):
@@ -224,7 +239,8 @@ class PTStokes:
         g = self.grid
         _lib.call("chmy_stokes_residual", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
                   _arr(self.tau_old), C.byref(self.eta.desc()) if self.eta is not None else None,
-                  C.byref(self.rhog.desc()) if self.rhog is not None else None, self.eta_s, self.G, self.dt,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  _arr(self.f_V) if self.f_V is not None else None, self.eta_s, self.G, self.dt,
                   _arr(toc), _arr(fV), C.byref(fP.desc()), C.byref(self.divV.desc()), _arr(self.r_V),
                   current_stream())
         from .kernels import maxabs
@@ -319,12 +335,23 @@ class MixedPTStokes:
         x, d, g = self.x, self.d, self.grid
         _lib.call("chmy_stokes_residual", C.byref(g.desc()), _arr(x.V), _arr(x.tau), C.byref(x.Pr.desc()),
                   _arr(x.tau_old), C.byref(x.eta.desc()) if x.eta is not None else None,
-                  C.byref(x.rhog.desc()) if x.rhog is not None else None, x.eta_s, x.G, x.dt,
+                  C.byref(x.rhog.desc()) if x.rhog is not None else None,
+                  _arr(x.f_V) if x.f_V is not None else None, x.eta_s, x.G, x.dt,
                   _arr(d.tau_old), _arr(d.f_V), C.byref(d.f_P.desc()), C.byref(x.divV.desc()), _arr(x.r_V),
                   current_stream())
         from .kernels import maxabs
         err_tau = max(maxabs(t) for t in d.tau_old) / (x.G * x.dt)
         return x.residual() + [err_tau]
+
+    def set_eta(self, eta_field):
+        self.x.set_eta(eta_field)
+        self.d.set_eta(eta_field)
+
+    def reset_state(self, V_init=None):
+        self.x.reset_state(V_init)
+        self.reset_correction()
+        self.cycles = 0
+        self.sweeps = 0
 
     def reset_correction(self):
         d = self.d

@@ -145,12 +145,23 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 #define CHMY_FUSED_WPE 0
 #endif
 
-template <class T, bool FVISC, bool EXACT, int BY>
+// XTRA = false compiles out the optional inputs/outputs (forcing, residual
+// fields, rho g) so that the plain iteration keeps its pointers in SGPRs.
+//
+// Rows: thread row ty sits at y = j = j0 - 1 + ty (j0 = tile * (BY - 1)) and
+// computes the Center stresses and tau_xz at row j, and the y-vertex edge
+// stresses tau_xy, tau_yz at row j + 1.  Velocities need Center values at
+// rows j-1, j and y-vertex edges at rows j, j+1, so rows ty = 1..BY-1 are
+// outputs and only row 0 is a ring.  The last wave also stages row j + 1 of
+// V and eta (LDS row BY).
+template <class T, bool FVISC, bool EXACT, int BY, bool XTRA>
 __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
-    __shared__ T s_vx[BY][kFX], s_vy[BY][kFX], s_vz1[BY][kFX];
-    __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? BY : 1][kFX], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][kFX];
+    const bool has_f = XTRA && a.has_f, write_res = XTRA && a.write_res, has_rg = XTRA && a.has_rg;
+    constexpr int NR = BY + 1;   // LDS rows of the V and eta planes
+    __shared__ T s_vx[NR][kFX], s_vy[2][NR][kFX], s_vz1[NR][kFX];
+    __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? NR : 1][kFX], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][kFX];
     __shared__ T s_P[BY][kFX], s_txx[BY][kFX], s_tyy[BY][kFX], s_txy[BY][kFX];
     __shared__ T s_txz[2][BY][kFX], s_tyz[2][BY][kFX];
 
@@ -159,134 +170,175 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
     const int lt = xcd_remap(blockIdx.x, nb);
     const int bx = lt % a.ntx, by = (lt / a.ntx) % a.nty, bz = lt / (a.ntx * a.nty);
     const int i = bx * (kFX - 2) - 1 + tx;
-    const int j = by * (BY - 2) - 1 + ty;
+    const int j = by * (BY - 1) - 1 + ty;
     const int nb0 = a.n[0] + 1, nb1 = a.n[1] + 1, nb2 = a.n[2] + 1;   // box is [0, n+1]
     const int kb = bz * a.kchunk;
     const int ke = min(kb + a.kchunk, nb2 + 1);
     // storage holds logical [-1, n+2] (Center) / [-1, n+3] (Vertex) per dim
     const bool valid = i <= a.n[0] + 2 && j <= a.n[1] + 2;
-    const bool inxy = i >= 0 && i <= nb0 && j >= 0 && j <= nb1;
-    const bool own = tx >= 1 && tx <= kFX - 2 && ty >= 1 && ty <= BY - 2 && inxy;
+    const bool inx = i >= 0 && i <= nb0;
+    const bool inc = inx && j >= 0 && j <= nb1;            // Center / tau_xz column in the box
+    const bool ine = inx && j + 1 <= nb1;                  // y-vertex edge row j+1 in the box
+    const bool ownx = tx >= 1 && tx <= kFX - 2;
+    const bool ownc = ownx && ty >= 1 && inc;              // outputs: Center, tau_xz, V at row j
+    const bool owne = ownx && ty <= BY - 2 && ine;         // outputs: tau_xy, tau_yz at row j+1
+    const bool last = ty == BY - 1;                        // stages row j+1 (wave-uniform)
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     constexpr uint32_t E = sizeof(T);
 
-    // byte offsets at plane k = kb - 1 and the per-step z increments
-    uint32_t o[NLOC], dz[NLOC];
+    // byte offsets at plane k = kb - 1 and the per-step z increments; lanes
+    // outside the storage keep offset 0 plus whole z planes, which stays inside
+    // every field (at most n+4 planes are walked), so all loads are
+    // unconditional and such lanes' values reach no output
+    // (row j+1 of a lane with offset 0 is still inside plane 0 + the walk).
+    // The staging wave reads row j+1 only where it is inside the storage
+    // (j + 1 <= n+2 for Center-in-y fields), else its own row (unused values).
+    uint32_t o[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
+    const bool row8 = last && j + 1 <= a.n[1] + 2;
 #pragma unroll
     for (int c = 0; c < NLOC; ++c) {
         const LocStride s = a.L[c];
         o[c] = valid ? (s.o0 + (uint32_t)i + (uint32_t)j * s.sy + (uint32_t)(kb - 1) * s.sz) * E : 0u;
         dz[c] = s.sz * E;
+        dy[c] = s.sy * E;
+        d8[c] = row8 ? dy[c] : 0u;
     }
-    auto ld = [valid](const T* p, uint32_t off) -> T { return valid ? ldo(p, off) : T(0); };
+    auto ld = [](const T* p, uint32_t off) -> T { return ldo(p, off); };
 
+    int p = 0;
     T vx_k = ld(a.Vx, o[LVX]), vy_k = ld(a.Vy, o[LVY]), vz_k = ld(a.Vz, o[LVZ]);
     T eta_k = T(0), eml_km1 = T(0);
+    s_vy[0][ty][tx] = vy_k;
     if constexpr (FVISC) {
         eta_k = ld(a.eta, o[LC]);
         s_eta[0][ty][tx] = eta_k;
     }
+    if (last) {   // row j+1 at plane kb - 1
+        s_vy[0][BY][tx] = ld(a.Vy, o[LVY] + d8[LVY]);
+        if constexpr (FVISC) s_eta[0][BY][tx] = ld(a.eta, o[LC] + d8[LC]);
+    }
     T P_km1 = T(0), tzz_km1 = T(0), txz_k = T(0), tyz_k = T(0);
-    int p = 0;
 
     for (int k = kb - 1; k < ke; ++k) {
         const bool pro = k < kb;
-        // ---------------- A: plane k+1 loads; V and eta planes to LDS
+        // ---------------- A: all of this step's global loads first (plane k+1 V
+        // and eta, plane k stresses and tau_old), then the V and eta planes to LDS
         const T vx_k1 = ld(a.Vx, o[LVX] + dz[LVX]);
         const T vy_k1 = ld(a.Vy, o[LVY] + dz[LVY]);
         const T vz_k1 = ld(a.Vz, o[LVZ] + dz[LVZ]);
         T eta_k1 = T(0), eml_k = T(0);
-        s_vx[ty][tx] = vx_k;
-        s_vy[ty][tx] = vy_k;
-        s_vz1[ty][tx] = vz_k1;
         if constexpr (FVISC) {
             eta_k1 = ld(a.eta, o[LC] + dz[LC]);
             eml_k = ld(a.eml, o[LC]);
+        }
+        T P = ld(a.Pr, o[LC]), txx = ld(a.txx, o[LC]), tyy = ld(a.tyy, o[LC]), tzz = ld(a.tzz, o[LC]);
+        T txy1 = ld(a.txy, o[LXY] + dy[LXY]);
+        T txz1 = ld(a.txz, o[LXZ] + dz[LXZ]), tyz1 = ld(a.tyz, o[LYZ] + dy[LYZ] + dz[LYZ]);
+        const T oxx = ldo(a.Oxx, o[LC]), oyy = ldo(a.Oyy, o[LC]), ozz = ldo(a.Ozz, o[LC]);
+        const T oxy = ldo(a.Oxy, o[LXY] + dy[LXY]);
+        const T oxz = ldo(a.Oxz, o[LXZ] + dz[LXZ]), oyz = ldo(a.Oyz, o[LYZ] + dy[LYZ] + dz[LYZ]);
+        T vx8 = T(0), vy8_1 = T(0), vz8_1 = T(0), eta8_1 = T(0);   // row j+1 (staging wave)
+        if (last) {
+            vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
+            vy8_1 = ld(a.Vy, o[LVY] + d8[LVY] + dz[LVY]);
+            vz8_1 = ld(a.Vz, o[LVZ] + d8[LVZ] + dz[LVZ]);
+            if constexpr (FVISC) eta8_1 = ld(a.eta, o[LC] + d8[LC] + dz[LC]);
+        }
+        const bool in_k = k >= 0;
+        const bool in_k1 = k + 1 <= nb2;
+        s_vx[ty][tx] = vx_k;
+        s_vy[p ^ 1][ty][tx] = vy_k1;
+        s_vz1[ty][tx] = vz_k1;
+        if constexpr (FVISC) {
             s_eta[p ^ 1][ty][tx] = eta_k1;
             s_eml[p][ty][tx] = eml_k;
         }
-        // stress-state loads for this step (independent of LDS)
-        T P = ld(a.Pr, o[LC]), txx = ld(a.txx, o[LC]), tyy = ld(a.tyy, o[LC]), tzz = ld(a.tzz, o[LC]);
-        T txy = ld(a.txy, o[LXY]);
-        T txz1 = ld(a.txz, o[LXZ] + dz[LXZ]), tyz1 = ld(a.tyz, o[LYZ] + dz[LYZ]);
-        const bool in_k = inxy && k >= 0;
-        const bool in_k1 = inxy && k + 1 <= nb2;
-        T oxx = T(0), oyy = T(0), ozz = T(0), oxy = T(0), oxz = T(0), oyz = T(0);
-        if (in_k) {
-            oxx = ldo(a.Oxx, o[LC]);
-            oyy = ldo(a.Oyy, o[LC]);
-            ozz = ldo(a.Ozz, o[LC]);
-            oxy = ldo(a.Oxy, o[LXY]);
-        }
-        if (in_k1) {
-            oxz = ldo(a.Oxz, o[LXZ] + dz[LXZ]);
-            oyz = ldo(a.Oyz, o[LYZ] + dz[LYZ]);
+        if (last) {
+            s_vx[BY][tx] = vx8;
+            s_vy[p ^ 1][BY][tx] = vy8_1;
+            s_vz1[BY][tx] = vz8_1;
+            if constexpr (FVISC) s_eta[p ^ 1][BY][tx] = eta8_1;
         }
         __syncthreads();
 
-        // ---------------- B: stresses
-        const int tx1 = tx < kFX - 1 ? tx + 1 : tx, ty1 = ty < BY - 1 ? ty + 1 : ty;
-        const int txm = tx > 0 ? tx - 1 : tx, tym = ty > 0 ? ty - 1 : ty;
+        // ---------------- B: stresses (computed on every lane, kept in the box)
+        const int tx1 = tx < kFX - 1 ? tx + 1 : tx, txm = tx > 0 ? tx - 1 : tx;
+        const int tym = ty > 0 ? ty - 1 : ty;
         const T exx = (s_vx[ty][tx1] - vx_k) * ix;
-        const T eyy = (s_vy[ty1][tx] - vy_k) * iy;
+        const T eyy = (s_vy[p][ty + 1][tx] - vy_k) * iy;
         const T ezz = (vz_k1 - vz_k) * iz;
         const T dv = (exx + eyy) + ezz;
-        if (in_k) {
-            typename AR::Loc Lc, Le;
-            if constexpr (FVISC) {
-                Lc = AR::field(eta_k, K);
-                Le = AR::field(lerp4(s_eta[p][tym][txm], s_eta[p][tym][tx], s_eta[p][ty][txm], eta_k), K);
-            } else {
-                Lc = Le = AR::scalar(K);
-            }
-            P = AR::pr(P, a.has_f ? dv - ldo(a.fP, o[LC]) : dv, Lc, K);
-            txx = AR::diag(txx, oxx, exx, dv, Lc, K);
-            tyy = AR::diag(tyy, oyy, eyy, dv, Lc, K);
-            tzz = AR::diag(tzz, ozz, ezz, dv, Lc, K);
-            const T exy = T(0.5) * ((vx_k - s_vx[tym][tx]) * iy + (vy_k - s_vy[ty][txm]) * ix);
-            txy = AR::off(txy, oxy, exy, Le, K);
+        {
+            typename AR::Loc Lc;
+            if constexpr (FVISC) Lc = AR::field(eta_k, K);
+            else Lc = AR::scalar(K);
+            const T Pn = AR::pr(P, has_f ? dv - ldo(a.fP, o[LC]) : dv, Lc, K);
+            const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
+            const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+            const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
+            const bool u = inc && in_k;
+            P = u ? Pn : P;
+            txx = u ? txxn : txx;
+            tyy = u ? tyyn : tyy;
+            tzz = u ? tzzn : tzz;
         }
-        if (in_k1) {
+        {   // tau_xy at (i, j+1, k)
+            const T exy = T(0.5) * ((s_vx[ty + 1][tx] - vx_k) * iy + (s_vy[p][ty + 1][tx] - s_vy[p][ty + 1][txm]) * ix);
+            typename AR::Loc Le;
+            if constexpr (FVISC)
+                Le = AR::field(lerp4(s_eta[p][ty][txm], eta_k, s_eta[p][ty + 1][txm], s_eta[p][ty + 1][tx]), K);
+            else Le = AR::scalar(K);
+            const T n = AR::off(txy1, oxy, exy, Le, K);
+            txy1 = ine && in_k ? n : txy1;
+        }
+        {   // tau_xz at (i, j, k+1) and tau_yz at (i, j+1, k+1)
             const T exz = T(0.5) * ((vx_k1 - vx_k) * iz + (vz_k1 - s_vz1[ty][txm]) * ix);
-            const T eyz = T(0.5) * ((vy_k1 - vy_k) * iz + (vz_k1 - s_vz1[tym][tx]) * iy);
+            const T eyz = T(0.5) * ((s_vy[p ^ 1][ty + 1][tx] - s_vy[p][ty + 1][tx]) * iz + (s_vz1[ty + 1][tx] - vz_k1) * iy);
             typename AR::Loc L4, L5;
             if constexpr (FVISC) {
                 L4 = AR::field(lerp4(s_eta[p][ty][txm], eta_k, s_eta[p ^ 1][ty][txm], eta_k1), K);
-                L5 = AR::field(lerp4(s_eta[p][tym][tx], eta_k, s_eta[p ^ 1][tym][tx], eta_k1), K);
+                L5 = AR::field(lerp4(eta_k, s_eta[p][ty + 1][tx], eta_k1, s_eta[p ^ 1][ty + 1][tx]), K);
             } else {
                 L4 = L5 = AR::scalar(K);
             }
-            txz1 = AR::off(txz1, oxz, exz, L4, K);
-            tyz1 = AR::off(tyz1, oyz, eyz, L5, K);
+            const T nxz = AR::off(txz1, oxz, exz, L4, K);
+            const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
+            txz1 = inc && in_k1 ? nxz : txz1;
+            tyz1 = ine && in_k1 ? nyz : tyz1;
         }
-        if (!pro && own && in_k) {
-            sto(a.oPr, o[LC], P);
-            sto(a.otxx, o[LC], txx);
-            sto(a.otyy, o[LC], tyy);
-            sto(a.otzz, o[LC], tzz);
-            sto(a.otxy, o[LXY], txy);
-            sto(a.otxz, o[LXZ], txz_k);
-            sto(a.otyz, o[LYZ], tyz_k);
-            if (a.write_res) sto(a.dV, o[LC], dv);
+        if (!pro && in_k) {
+            if (ownc) {
+                sto(a.oPr, o[LC], P);
+                sto(a.otxx, o[LC], txx);
+                sto(a.otyy, o[LC], tyy);
+                sto(a.otzz, o[LC], tzz);
+                sto(a.otxz, o[LXZ], txz_k);
+                if (write_res) sto(a.dV, o[LC], dv);
+            }
+            if (owne) {
+                sto(a.otxy, o[LXY] + dy[LXY], txy1);
+                sto(a.otyz, o[LYZ] + dy[LYZ], tyz_k);
+            }
         }
         s_P[ty][tx] = P;
         s_txx[ty][tx] = txx;
         s_tyy[ty][tx] = tyy;
-        s_txy[ty][tx] = txy;
+        s_txy[ty][tx] = txy1;
         s_txz[p][ty][tx] = txz1;
         s_tyz[p][ty][tx] = tyz1;
         __syncthreads();
 
-        // ---------------- C: velocity at plane k
-        if (!pro && own && in_k) {
-            T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) +
-                    (s_txy[ty1][tx] - txy) * iy) + (txz1 - txz_k) * iz;
-            T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) +
-                    (s_txy[ty][tx1] - txy) * ix) + (tyz1 - tyz_k) * iz;
+        // ---------------- C: velocity at plane k, rows ty >= 1
+        if (!pro && ownc && in_k) {
+            const T txy_j = s_txy[tym][tx];   // tau_xy at (i, j, k), from row ty-1
+            T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - s_txx[ty][txm]) * ix) + (txy1 - txy_j) * iy) +
+                   (txz1 - txz_k) * iz;
+            T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[tym][tx1] - txy_j) * ix) +
+                   (s_tyz[p][tym][tx] - s_tyz[p ^ 1][tym][tx]) * iz;
             T rz = ((-((P - P_km1) * iz) + (tzz - tzz_km1) * iz) + (s_txz[p ^ 1][ty][tx1] - txz_k) * ix) +
-                   (s_tyz[p ^ 1][ty1][tx] - tyz_k) * iy;
-            if (a.has_rg) rz = rz - ldo(a.rg, o[LVZ]);
-            if (a.has_f) {
+                   (tyz_k - s_tyz[p ^ 1][tym][tx]) * iy;
+            if (has_rg) rz = rz - ldo(a.rg, o[LVZ]);
+            if (has_f) {
                 rx = rx + ldo(a.fVx, o[LVX]);
                 ry = ry + ldo(a.fVy, o[LVY]);
                 rz = rz + ldo(a.fVz, o[LVZ]);
@@ -302,7 +354,7 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
             sto(a.oVx, o[LVX], AR::vel(vx_k, rx, ex, K));
             sto(a.oVy, o[LVY], AR::vel(vy_k, ry, ey, K));
             sto(a.oVz, o[LVZ], AR::vel(vz_k, rz, ez, K));
-            if (a.write_res) {
+            if (write_res) {
                 sto(a.rVx, o[LVX], rx);
                 sto(a.rVy, o[LVY], ry);
                 sto(a.rVz, o[LVZ], rz);
@@ -529,7 +581,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
         a.kchunk = kZChunk;
         a.ntx = (int32_t)((g->n[0] + 2 + kFX - 3) / (kFX - 2));
-        a.nty = (int32_t)((g->n[1] + 2 + kBY3 - 3) / (kBY3 - 2));
+        a.nty = (int32_t)((g->n[1] + 2 + kBY3 - 2) / (kBY3 - 1));
         a.nchunks = (int32_t)((g->n[2] + 2 + kZChunk - 1) / kZChunk);
         a.has_rg = rg != nullptr;
         a.write_res = wres;
@@ -539,10 +591,17 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
         dim3 blk(kFX, kBY3, 1), grd(nb, 1, 1);
-        if (fv && exact) hipLaunchKernelGGL((fused3d_kernel<T, true, true, kBY3>), grd, blk, 0, s, a);
-        else if (fv) hipLaunchKernelGGL((fused3d_kernel<T, true, false, kBY3>), grd, blk, 0, s, a);
-        else if (exact) hipLaunchKernelGGL((fused3d_kernel<T, false, true, kBY3>), grd, blk, 0, s, a);
-        else hipLaunchKernelGGL((fused3d_kernel<T, false, false, kBY3>), grd, blk, 0, s, a);
+        const bool xtra = a.has_f || a.write_res || a.has_rg;
+#define CHMY_F3(FV, EX)                                                                                   \
+    do {                                                                                                  \
+        if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true>), grd, blk, 0, s, a);          \
+        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, false>), grd, blk, 0, s, a);              \
+    } while (0)
+        if (fv && exact) CHMY_F3(true, true);
+        else if (fv) CHMY_F3(true, false);
+        else if (exact) CHMY_F3(false, true);
+        else CHMY_F3(false, false);
+#undef CHMY_F3
     } else {
         Fused2<T> a;
         std::memset(&a, 0, sizeof a);

@@ -19,11 +19,11 @@ constexpr int kRX = 64, kRY = 4;
 
 template <class T, class TO>
 struct ResArgs {
-    DF<T> V[3], tau[6], told[6], Pr, eta, rg;
+    DF<T> V[3], tau[6], told[6], Pr, eta, rg, fe[3];
     DF<TO> toc[6], fV[3], fP;
     DF<T> dV, rV[3];
     T ix, iy, iz, eta_s, eve_s_unused, Gs, dts, Gdt;
-    int has_eta, has_rg, has_norm;
+    int has_eta, has_rg, has_norm, has_fe;
 };
 
 template <class T>
@@ -111,6 +111,8 @@ __global__ void __launch_bounds__(256) residual_kernel(Box3 b, ResArgs<T, TO> a)
                 (txz.at(o4 + 1) - txz.at(o4)) * a.ix) + (tyz.at(o5 + tyz.sy) - tyz.at(o5)) * a.iy;
         if (a.has_rg) r[2] = r[2] - a.rg.at(O(a.rg));
     }
+    if (a.has_fe)
+        for (int d = 0; d < ND; ++d) r[d] = r[d] + a.fe[d].at(O(a.fe[d]));
     for (int d = 0; d < ND; ++d) {
         a.fV[d].at(O(a.fV[d])) = TO(r[d]);
         if (a.has_norm) a.rV[d].at(O(a.rV[d])) = r[d];
@@ -141,8 +143,8 @@ __global__ void __launch_bounds__(256) accumulate_kernel(Box3 b, AccArgs<T, TO> 
 template <class T, class TO>
 static int residual_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_field_t* V,
                            const chmy_field_t* tau, const chmy_field_t* Pr, const chmy_field_t* told,
-                           const chmy_field_t* eta, const chmy_field_t* rg, double eta_s, double G, double dt,
-                           const chmy_field_t* toc, const chmy_field_t* fV, const chmy_field_t* fP,
+                           const chmy_field_t* eta, const chmy_field_t* rg, const chmy_field_t* fe,
+                           double eta_s, double G, double dt, const chmy_field_t* toc, const chmy_field_t* fV, const chmy_field_t* fP,
                            const chmy_field_t* dV, const chmy_field_t* rV, hipStream_t s) {
     ResArgs<T, TO> a;
     std::memset(&a, 0, sizeof a);
@@ -151,7 +153,9 @@ static int residual_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chm
         a.V[d] = view<T>(V[d]);
         a.fV[d] = view<TO>(fV[d]);
         a.rV[d] = rV ? view<T>(rV[d]) : a.V[d];
+        a.fe[d] = fe ? view<T>(fe[d]) : a.V[d];
     }
+    a.has_fe = fe != nullptr;
     for (int c = 0; c < nt; ++c) {
         a.tau[c] = view<T>(tau[c]);
         a.told[c] = view<T>(told[c]);
@@ -210,7 +214,8 @@ extern "C" {
 
 int chmy_stokes_residual(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
                          const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* rhog,
-                         double eta_s, double G, double dt, const chmy_field_t* tau_old_c, const chmy_field_t* f_V,
+                         const chmy_field_t* f_ext, double eta_s, double G, double dt,
+                         const chmy_field_t* tau_old_c, const chmy_field_t* f_V,
                          const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V, void* stream) {
     CHMY_TRY(check_grid(g));
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
@@ -224,6 +229,7 @@ int chmy_stokes_residual(const chmy_grid_t* g, const chmy_field_t* V, const chmy
         loc[g->nd - 1] = 1;
         CHMY_TRY(check_field(g, rhog, "rhog", loc, CHMY_F64));
     }
+    if (f_ext) CHMY_TRY(check_vector(g, f_ext, "f_ext", CHMY_F64));
     CHMY_TRY(check_center(g, f_P, "f_P", -1));
     const int to = f_P->dtype;
     CHMY_TRY(check_vector(g, f_V, "f_V", to));
@@ -236,9 +242,9 @@ int chmy_stokes_residual(const chmy_grid_t* g, const chmy_field_t* V, const chmy
     chmy_box_t bx;
     CHMY_TRY(check_box(g, nullptr, &bx));
     hipStream_t s = (hipStream_t)stream;
-    return to == CHMY_F32 ? residual_launch<double, float>(g, bx, V, tau, Pr, tau_old, eta, rhog, eta_s, G, dt,
+    return to == CHMY_F32 ? residual_launch<double, float>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt,
                                                            tau_old_c, f_V, f_P, divV, r_V, s)
-                          : residual_launch<double, double>(g, bx, V, tau, Pr, tau_old, eta, rhog, eta_s, G, dt,
+                          : residual_launch<double, double>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt,
                                                             tau_old_c, f_V, f_P, divV, r_V, s);
 }
 

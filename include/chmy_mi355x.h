@@ -214,19 +214,30 @@ int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const c
  * path).  Residuals of an f64 state in the reference's expression order
  * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:84-111):
  * R_tau = -(tau - tau_old)/(G dt) - tau/eta + 2 dev(eps(V)), R_P = -divg(V),
- * R_V = -grad(Pr) + divg(tau) - rhog, over the Launcher box.  Writes the
+ * R_V = -grad(Pr) + divg(tau) - rhog (+ f_ext, an optional f64 vector field:
+ * the forcing of an adjoint solve), over the Launcher box.  Writes the
  * correction forcing in the dtype of the output fields: tau_old_c = G dt R_tau,
  * f_P = R_P, f_V = R_V; divV/r_V (f64, optional) receive divg(V) and R_V for
  * the residual norms.  Viscosity as in chmy_stokes_stress. */
 int chmy_stokes_residual(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                          const chmy_field_t* Pr, const chmy_field_t* tau_old, const chmy_field_t* eta,
-                         const chmy_field_t* rhog, double eta_s, double G, double dt,
+                         const chmy_field_t* rhog, const chmy_field_t* f_ext, double eta_s, double G, double dt,
                          const chmy_field_t* tau_old_c, const chmy_field_t* f_V, const chmy_field_t* f_P,
                          const chmy_field_t* divV, const chmy_field_t* r_V, void* stream);
 /* x += d over the Launcher box for (V, tau, Pr) in f64, d in f32 or f64. */
 int chmy_stokes_accumulate(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                            const chmy_field_t* Pr, const chmy_field_t* V_c, const chmy_field_t* tau_c,
                            const chmy_field_t* Pr_c, void* stream);
+/* Gradient of a functional J of the converged steady Stokes state with respect
+ * to the Center viscosity field (build extension).  tau: the forward solve's
+ * stresses; tau_a: the stresses of the adjoint solve, i.e. the same PT problem
+ * with forcing f_V = -dJ/dV, homogeneous boundary values and tau_old = 0
+ * (the steady operator is symmetric up to Pr -> -lambda_P).  Writes
+ * grad = dJ/deta (logscale = 0) or dJ/dlog(eta) (logscale != 0) at centres
+ * 0..n+1:  [interior] sum_d tau_a_dd tau_dd / (2 eta^2) + 1/4 sum over the
+ * active edges that lerp eta from the centre of tau_a_e tau_e / eta_e^2. */
+int chmy_stokes_grad_eta(const chmy_grid_t* grid, const chmy_field_t* tau, const chmy_field_t* tau_a,
+                         const chmy_field_t* eta, const chmy_field_t* grad, int logscale, void* stream);
 
 /* ---------------------------------------------------------------- SIA (2D, f64)
  * Shallow-ice nonlinear diffusion of examples/sia.jl with the lazy fields

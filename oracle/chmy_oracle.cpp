@@ -503,6 +503,67 @@ template <class T> void residual_t(const orc_grid* gg, const orc_box* b, const o
     });
 }
 
+// Gradient of a functional J of the converged PT Stokes state with respect
+// to the Center viscosity field (build extension).  With the steady stress
+// equations R_tau = -(tau - tau_old)/(G dt) - tau/eta_loc + 2 dev(eps) = 0,
+// the adjoint multiplier of a stored stress component is -tau_a/2 (diagonal)
+// or -tau_a (edge), tau_a the stresses of the adjoint Stokes solve (forcing
+// -dJ/dV, homogeneous BCs, tau_old = 0), and dR_tau/deta_loc = tau/eta_loc^2:
+//   g(c) = [c interior] sum_d tau_a_dd tau_dd / (2 eta_c eta_c)
+//        + 0.25 sum over the edges e of each off-diagonal component that
+//          lerp eta from c (vertex index c or c+1 in its vertex dims) and lie
+//          in the active range (vertex 1..n+1, centre 1..n) of
+//          tau_a_e tau_e / (eta_e eta_e),  eta_e = lerp(eta) to e;
+// logscale != 0 returns dJ/dlog(eta) = eta_c g(c).  Evaluated over centres
+// 0..n+1; fixed summation order (diagonal d ascending; components xy, xz, yz;
+// edges with the first vertex dimension fastest).
+template <class T> void stokes_grad_t(const orc_grid* gg, const orc_field* tau, const orc_field* taua,
+                                      const orc_field* eta_, const orc_field* grad_, int logscale) {
+    G<T> g(gg);
+    const int nd = g.nd, nt = nd == 2 ? 3 : 6;
+    alignas(FV<T>) unsigned char b1[6 * sizeof(FV<T>)], b2[6 * sizeof(FV<T>)];
+    FV<T>* t = reinterpret_cast<FV<T>*>(b1);
+    FV<T>* ta = reinterpret_cast<FV<T>*>(b2);
+    views(tau, nt, t);
+    views(taua, nt, ta);
+    FV<T> eta(eta_), gr(grad_);
+    static const int L2[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 1, 0}};
+    static const int L3[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    orc_box b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.len[d] = d < nd ? g.n[d] + 2 : 1; }
+    for_box(&b, nd, [&](const Ix& I) {
+        const T ec = eta(I);
+        bool interior = true;
+        for (int d = 0; d < nd; ++d) interior = interior && I.v[d] >= 1 && I.v[d] <= g.n[d];
+        T gd = T(0);
+        if (interior)
+            for (int d = 0; d < nd; ++d) gd = gd + (ta[d](I) * t[d](I)) / (T(2) * ec * ec);
+        T ge = T(0);
+        for (int c = nd; c < nt; ++c) {
+            const int* L = nd == 2 ? L2[c] : L3[c];
+            int vd[2], nv = 0;
+            for (int d = 0; d < nd; ++d) if (L[d] == 1) vd[nv++] = d;
+            for (int bb = 0; bb < 2; ++bb)
+                for (int aa = 0; aa < 2; ++aa) {
+                    Ix E = I;
+                    E.v[vd[0]] += aa;
+                    E.v[vd[1]] += bb;
+                    bool act = true;
+                    for (int d = 0; d < nd; ++d) {
+                        const long hi = L[d] == 1 ? g.n[d] + 1 : g.n[d];
+                        act = act && E.v[d] >= 1 && E.v[d] <= hi;
+                    }
+                    if (!act) continue;
+                    const T et = lerp(eta, L, E);
+                    ge = ge + (ta[c](E) * t[c](E)) / (et * et);
+                }
+        }
+        T r = gd + T(0.25) * ge;
+        if (logscale) r = r * ec;
+        gr(I) = r;
+    });
+}
+
 // examples/stokes_3d_inc_ve_T.jl:11-21 (2D: stokes_2d_inc_ve_T.jl:11-18)
 template <class T> void update_old_t(const orc_box* b, int nd, const orc_field* Tf, const orc_field* tau,
                                      const orc_field* Told, const orc_field* tauo) {
@@ -788,6 +849,12 @@ int orc_stokes_residual(const orc_grid* g, const orc_box* b, const orc_field* V,
     residual_t<double>(g, b, V, tau, Pr, tauo, eta, rg, eta_s, G, dt, Rt, RV, RP);
     return g_err;
 }
+int orc_stokes_grad_eta(const orc_grid* g, const orc_field* tau, const orc_field* taua, const orc_field* eta,
+                        const orc_field* grad, int logscale) {
+    stokes_grad_t<double>(g, tau, taua, eta, grad, logscale);
+    return g_err;
+}
+
 int orc_thermal_flux(const orc_grid* g, const orc_box* b, const orc_field* qT, const orc_field* T,
                      const orc_field* V, double lam) {
     if (T->dtype == 0) thermal_flux_t<double>(g, b, qT, T, V, lam); else thermal_flux_t<float>(g, b, qT, T, V, lam);

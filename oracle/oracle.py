@@ -78,6 +78,7 @@ def _declare(L):
     L.orc_stokes_residual.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
                                       P(OField), C.c_double, C.c_double, C.c_double, P(OField), P(OField),
                                       P(OField)]
+    L.orc_stokes_grad_eta.argtypes = [P(OGrid), P(OField), P(OField), P(OField), P(OField), C.c_int]
     L.orc_thermal_flux.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), C.c_double]
     L.orc_thermal.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), C.c_double]
     L.orc_maxloc.argtypes = [P(OBox), P(OField), P(OField)]
@@ -341,6 +342,12 @@ def stokes_residual(grid, V, tau, Pr, tauo, eta, rhog, G, dt, Rtau, RV, RP, b=No
     _check(lib().orc_stokes_residual(C.byref(grid.desc()), C.byref(b), _fields(V), _fields(tau), C.byref(Pr.desc()),
                                      _fields(tauo), _opt(ef), _opt(rhog), es, G, dt, _fields(Rtau), _fields(RV),
                                      C.byref(RP.desc())))
+
+
+def stokes_grad_eta(grid, tau, tau_a, eta, grad, logscale=True):
+    """dJ/d(log) eta at centres 0..n+1 from forward and adjoint-solve stresses."""
+    _check(lib().orc_stokes_grad_eta(C.byref(grid.desc()), _fields(tau), _fields(tau_a), C.byref(eta.desc()),
+                                     C.byref(grad.desc()), int(bool(logscale))))
 
 
 def thermal_flux(grid, qT, T, V, lam, b=None):

@@ -53,7 +53,7 @@ def test_residual_kernel(arch, nd, visc, out):
     G, dt, eta_s = 1.0, 0.3, 0.7
     _lib.call("chmy_stokes_residual", C.byref(g.desc()), _arr([a for a, _ in V]), _arr([a for a, _ in tau]),
               C.byref(Pr[0].desc()), _arr([a for a, _ in told]), C.byref(eta[0].desc()) if fv else None,
-              C.byref(rg[0].desc()), eta_s, G, dt, _arr(toc), _arr(fV), C.byref(fP.desc()), C.byref(dV[0].desc()),
+              C.byref(rg[0].desc()), None, eta_s, G, dt, _arr(toc), _arr(fV), C.byref(fP.desc()), C.byref(dV[0].desc()),
               _arr([a for a, _ in rV]), chmy.current_stream())
     chmy.synchronize()
     H = lambda L: [b for _, b in L]
