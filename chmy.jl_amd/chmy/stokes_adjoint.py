@@ -1,8 +1,9 @@
 """Gradient of a steady PT Stokes solve with respect to log viscosity
 (build extension; the reference has no adjoint).
 
-J(log η) = ½ Σ over the top cell layer (z index n_z, all x/y points of the
-Vx and Vy arrays there) of (V - V_obs)², with V the converged solution of
+J(log η) = ½ Σ over the top cell layer (last-dimension index n, all points
+of the tangential velocity arrays there: Vx, Vy in 3D, Vx in 2D) of
+(V - V_obs)², with V the converged solution of
 the viscoelastic Stokes problem of examples/stokes_3d_inc_ve_T.jl
 (τ_old = 0, free-slip walls with the given normal velocities) and η a Center
 field on the full storage.  The steady operator is symmetric up to the sign
@@ -69,7 +70,8 @@ class StokesAdjoint:
         return self._solve(self.fwd, eps, max_iter, V_init)
 
     def top(self, f):
-        """View of the top cell layer (z = n_z) of a Vx or Vy array."""
+        """View of the top cell layer (last-dimension index n) of a tangential
+        velocity array."""
         return f.interior()[..., -1]
 
     def misfit(self, V_obs):
@@ -79,7 +81,7 @@ class StokesAdjoint:
         J = 0.0
         for f in self.f_adj:
             f.data.zero_()
-        for D in range(2):
+        for D in range(self.grid.nd - 1):
             d = self.top(V[D]) - V_obs[D]
             J += 0.5 * float(torch.sum(d * d))
             self.top(self.f_adj[D]).copy_(-d)   # adjoint forcing -dJ/dV
@@ -87,7 +89,7 @@ class StokesAdjoint:
 
     def observe(self):
         V = self._state(self.fwd).V
-        return [self.top(V[D]).clone() for D in range(2)]
+        return [self.top(V[D]).clone() for D in range(self.grid.nd - 1)]
 
     def solve_adjoint(self, eps=1e-8, max_iter=400000):
         return self._solve(self.adj, eps, max_iter, None)

@@ -400,9 +400,11 @@ template <class T> void stress_t(const orc_grid* gg, const orc_box* b, const orc
     });
 }
 
+// fV (build extension, optional): a forcing added to r_V after rho g, the
+// form in which defect correction and adjoint solves drive the iteration.
 template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const orc_field* V_, const orc_field* rV_,
                                    const orc_field* Pr_, const orc_field* tau, const orc_field* rhog_, Visc vs,
-                                   double nudtau) {
+                                   double nudtau, const orc_field* fV_ = nullptr) {
     G<T> g(gg);
     const int nd = g.nd;
     const int nt = nd == 2 ? 3 : 6;
@@ -416,6 +418,10 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
     FV<T> Pr(Pr_);
     const bool hasrg = rhog_ != nullptr;
     FV<T> rg(hasrg ? rhog_ : Pr_);
+    const bool hasf = fV_ != nullptr;
+    alignas(FV<T>) unsigned char b4[3 * sizeof(FV<T>)];
+    FV<T>* fV = reinterpret_cast<FV<T>*>(b4);
+    views(hasf ? fV_ : V_, nd, fV);
     const bool fieldvisc = vs.eta_ml != nullptr;
     FV<T> eml(fieldvisc ? vs.eta_ml : Pr_);
     const T nu = T(nudtau), eve_s = T(vs.eta_ve_s), Gs = T(vs.G), dts = T(vs.dt);
@@ -433,6 +439,8 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
             r[2] = ((-dd(Pr, g, 2, I) + dd(t[2], g, 2, I)) + dd(t[4], g, 0, I)) + dd(t[5], g, 1, I);
             if (hasrg) r[2] = r[2] - rg(I);
         }
+        if (hasf)
+            for (int d = 0; d < nd; ++d) r[d] = r[d] + fV[d](I);
         for (int d = 0; d < nd; ++d) {
             rV[d](I) = r[d];
             T ev = eve_s;
@@ -836,10 +844,10 @@ int orc_update_stress(const orc_grid* g, const orc_box* b, const orc_field* tau,
 }
 int orc_update_velocity(const orc_grid* g, const orc_box* b, const orc_field* V, const orc_field* rV,
                         const orc_field* Pr, const orc_field* tau, const orc_field* rhog, const orc_field* eta_ml,
-                        double eta_ve_s, double G, double dt, double nudtau) {
+                        double eta_ve_s, double G, double dt, double nudtau, const orc_field* fV) {
     Visc vs{nullptr, eta_ml, 0.0, eta_ve_s, G, dt};
-    if (Pr->dtype == 0) velocity_t<double>(g, b, V, rV, Pr, tau, rhog, vs, nudtau);
-    else velocity_t<float>(g, b, V, rV, Pr, tau, rhog, vs, nudtau);
+    if (Pr->dtype == 0) velocity_t<double>(g, b, V, rV, Pr, tau, rhog, vs, nudtau, fV);
+    else velocity_t<float>(g, b, V, rV, Pr, tau, rhog, vs, nudtau, fV);
     return g_err;
 }
 int orc_stokes_residual(const orc_grid* g, const orc_box* b, const orc_field* V, const orc_field* tau,

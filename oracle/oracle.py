@@ -74,7 +74,7 @@ def _declare(L):
     L.orc_update_stress.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
                                     P(OField)] + [C.c_double] * 6
     L.orc_update_velocity.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
-                                      P(OField)] + [C.c_double] * 4
+                                      P(OField)] + [C.c_double] * 4 + [P(OField)]
     L.orc_stokes_residual.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), P(OField), P(OField), P(OField),
                                       P(OField), C.c_double, C.c_double, C.c_double, P(OField), P(OField),
                                       P(OField)]
@@ -327,11 +327,12 @@ def update_stress(grid, tau, Pr, dV, V, tauo, eta, eta_ve, G, dt, dtau_Pr, dtau_
                                    float(eta_ve), G, dt, dtau_Pr, dtau_r))
 
 
-def update_velocity(grid, V, rV, Pr, tau, rhog, eta_ve, nudtau, eta_ml=None, G=1.0, dt=1.0, b=None):
+def update_velocity(grid, V, rV, Pr, tau, rhog, eta_ve, nudtau, eta_ml=None, G=1.0, dt=1.0, b=None, fV=None):
+    """fV: optional forcing added to r_V (build extension)."""
     b = b or grid.launcher_box()
     _check(lib().orc_update_velocity(C.byref(grid.desc()), C.byref(b), _fields(V), _fields(rV),
                                      C.byref(Pr.desc()), _fields(tau), _opt(rhog), _opt(eta_ml),
-                                     float(eta_ve), G, dt, nudtau))
+                                     float(eta_ve), G, dt, nudtau, _fields(fV) if fV is not None else None))
 
 
 def stokes_residual(grid, V, tau, Pr, tauo, eta, rhog, G, dt, Rtau, RV, RP, b=None):
