@@ -225,6 +225,87 @@ def sia_gradient(args):
     print(json.dumps(res))
 
 
+def stokes_gradient(args):
+    """Steady 3D Stokes gradient dJ/dlog(eta), J the top-layer tangential
+    velocity misfit against the solution for a perturbed viscosity: forward
+    and adjoint PT solves as FP32 sweeps with FP64 defect correction, then the
+    gradient pass; cost ratio (forward + adjoint + gradient) / forward.  The
+    gradient is checked against central differences of J along seeded
+    Gaussian directions at a smaller grid with f64 solves."""
+    import numpy as np
+    import torch
+    import chmy
+    from chmy import configs
+    from chmy.stokes import pure_shear_values
+    from chmy.stokes_adjoint import StokesAdjoint, smooth_log_field
+    arch = chmy.Arch(chmy.MI355XBackend())
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+
+    def problem(n, mixed):
+        g, eta, dt = configs.stokes3d_inclusion(arch, n)
+        la0 = np.log(eta)
+        A = StokesAdjoint(arch, g, la0, G=1.0, dt=dt, normal_values=pure_shear_values(3), mixed=mixed,
+                          inner=args.inner)
+        return g, la0, A
+
+    g, la0, A = problem(args.n, True)
+    A.set_logeta(la0 + smooth_log_field(g, 0.5, seed=args.sia_seed, corr=8))
+    sw_obs, _ = A.solve_forward(eps=args.eps, max_iter=args.max_iter)
+    V_obs = A.observe()
+    A.set_logeta(la0)
+    log(f"observation solve: {sw_obs} sweeps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sw_f, err_f = A.solve_forward(eps=args.eps, max_iter=args.max_iter)
+    torch.cuda.synchronize()
+    t_f = time.perf_counter() - t0
+    J = A.misfit(V_obs)
+    t0 = time.perf_counter()
+    sw_a, err_a = A.solve_adjoint(eps=args.eps, max_iter=args.max_iter)
+    torch.cuda.synchronize()
+    t_a = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    A.gradient()
+    torch.cuda.synchronize()
+    t_g = time.perf_counter() - t0
+    log(f"forward {t_f:.2f} s ({sw_f} sweeps), adjoint {t_a:.2f} s ({sw_a}), gradient pass {t_g * 1e3:.2f} ms")
+    del A
+    torch.cuda.empty_cache()
+    # central differences at the check size, f64 solves
+    gf, laf, B = problem(args.fd_n, False)
+    B.set_logeta(laf + smooth_log_field(gf, 0.5, seed=args.sia_seed, corr=8))
+    B.solve_forward(eps=args.fd_eps, max_iter=args.max_iter)
+    Vo = B.observe()
+    B.set_logeta(laf)
+    Jf, grad = B.J_and_gradient(Vo, eps=args.fd_eps, max_iter=args.max_iter)
+    gn = grad.xyz().cpu().numpy()
+    rng = np.random.default_rng(args.sia_seed)
+    rows = []
+    for _ in range(args.fd_dirs):
+        v = rng.standard_normal(laf.shape)
+        Js = []
+        for sgn in (1, -1):
+            B.set_logeta(laf + sgn * args.fd_h * v)
+            B.solve_forward(eps=args.fd_eps, max_iter=args.max_iter)
+            Js.append(B.misfit(Vo))
+        fd = (Js[0] - Js[1]) / (2 * args.fd_h)
+        ad = float(np.sum(gn * np.transpose(v, (2, 1, 0))))
+        rows.append({"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / abs(fd)})
+        log(f"FD direction: fd {fd:.10e} adjoint {ad:.10e}")
+    worst = max((r["rel_err"] for r in rows), default=None)
+    res = {"metric": "adjoint/fwd cost", "value": round((t_f + t_a + t_g) / t_f, 3),
+           "unit": "x forward wall time (forward + adjoint + gradient over forward)", "higher_is_better": False,
+           "n_gpus": 1, "dtype": "f32 sweeps + f64 residual",
+           "config": {"workload": f"3D VE Stokes, {args.n}^3 cells, inclusion, J = top-layer tangential velocity "
+                                  f"misfit, gradient wrt log eta, eps {args.eps}",
+                      "forward_s": round(t_f, 3), "adjoint_s": round(t_a, 3), "gradient_pass_ms": round(t_g * 1e3, 3),
+                      "forward_sweeps": sw_f, "adjoint_sweeps": sw_a, "residual_forward": err_f,
+                      "residual_adjoint": err_a, "J": J},
+           "fd_check": {"grid": [args.fd_n] * 3, "directions": args.fd_dirs, "h": args.fd_h, "solve_eps": args.fd_eps,
+                        "J": Jf, "worst_rel_err": worst, "rows": rows}}
+    print(json.dumps(res))
+
+
 def copy_calibration(args):
     """A plain device copy of known size (1 GiB read + 1 GiB written per
     launch): the reference point for scaling HBM counter bytes."""
@@ -264,6 +345,12 @@ def main():
     ap.add_argument("--sia-h", type=float, default=1e-5)
     ap.add_argument("--sia-seed", type=int, default=16)
     ap.add_argument("--sia-reps", type=int, default=5)
+    ap.add_argument("--stokes-adjoint", action="store_true",
+                    help="3D Stokes gradient cost and finite-difference check instead of the T_eff line")
+    ap.add_argument("--fd-n", type=int, default=63)
+    ap.add_argument("--fd-dirs", type=int, default=8)
+    ap.add_argument("--fd-h", type=float, default=1e-4)
+    ap.add_argument("--fd-eps", type=float, default=1e-11)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -276,6 +363,9 @@ def main():
         return
     if args.ttc:
         time_to_converge(args)
+        return
+    if args.stokes_adjoint:
+        stokes_gradient(args)
         return
     if args.sia:
         if args.n == 255:
