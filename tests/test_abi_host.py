@@ -72,6 +72,53 @@ def test_validation_rejects_misuse_on_host():
     assert rc == -2
 
 
+def test_adjoint_entry_points_validate_on_host():
+    """SIA run/reverse and the Stokes gradient pass reject misuse before any
+    launch: missing arrays, aliasing, checkpoint counts, wrong locations."""
+    L = _lib.lib()
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(8, 6))
+    gd = g.desc()
+
+    def fdesc(loc, n, data=0x1000):
+        f = _lib.ChmyField()
+        f.data = data
+        f.nd = 2
+        f.dtype = _lib.F64
+        for d in range(2):
+            f.n[d], f.halo[d], f.loc[d] = n[d], 1, loc[d]
+        return f
+
+    Cc = lambda data=0x1000: fdesc((0, 0), (8, 6), data)
+    dts = (C.c_double * 4)(1.0, 1.0, 1.0, 1.0)
+    B = Cc(0x9000)
+    # states alias: states[0] and states[1] share storage
+    st = (_lib.ChmyField * 3)(Cc(0x1000), Cc(0x1000), Cc(0x2000))
+    assert L.chmy_sia_run(C.byref(gd), st, 3, C.byref(B), None, 0.0, 3, 0.0, 0.0, 0.0, dts, None) == -1
+    assert b"share storage" in L.chmy_last_error()
+    # a state at the wrong location
+    st = (_lib.ChmyField * 2)(Cc(0x1000), fdesc((1, 0), (9, 6), 0x2000))
+    assert L.chmy_sia_run(C.byref(gd), st, 2, C.byref(B), None, 0.0, 3, 0.0, 0.0, 0.0, dts, None) == -3
+    # reverse: checkpoint count must be ceil(nsteps / stride)
+    ck = (_lib.ChmyField * 2)(Cc(0x1000), Cc(0x2000))
+    seg = (_lib.ChmyField * 2)(Cc(0x3000), Cc(0x4000))
+    la, lb = Cc(0x5000), Cc(0x6000)
+    slot = C.c_int(-1)
+    rc = L.chmy_sia_reverse(C.byref(gd), ck, 2, 2, 5, seg, C.byref(la), C.byref(lb), None, C.byref(B), None, 0.0,
+                            3, 0.0, 0.0, 0.0, dts, C.byref(slot), None)
+    assert rc == -1 and b"nckpt" in L.chmy_last_error()
+    rc = L.chmy_sia_reverse(C.byref(gd), ck, 2, 2, 4, seg, C.byref(la), C.byref(la), None, C.byref(B), None, 0.0,
+                            3, 0.0, 0.0, 0.0, dts, C.byref(slot), None)
+    assert rc == -1 and b"lam_a and lam_b" in L.chmy_last_error()
+    # Stokes gradient pass: tau_a with a wrong location
+    tl = [(0, 0), (0, 0), (1, 1)]
+    tn = lambda l: (8 + l[0], 6 + l[1])
+    tau = (_lib.ChmyField * 3)(*[fdesc(l, tn(l)) for l in tl])
+    bad = (_lib.ChmyField * 3)(*[fdesc(l, tn(l)) for l in tl[:2]], fdesc((1, 0), (9, 6)))
+    eta, gr = Cc(), Cc()
+    assert L.chmy_stokes_grad_eta(C.byref(gd), tau, bad, C.byref(eta), C.byref(gr), 1, None) == -3
+    assert L.chmy_stokes_grad_eta(C.byref(gd), tau, None, C.byref(eta), C.byref(gr), 1, None) == -1
+
+
 def test_host_field_refuses_kernel_use():
     g = chmy.UniformGrid(origin=(0.0,), extent=(1.0,), dims=(4,))
     f = chmy.Field(None, g, chmy.Center())
