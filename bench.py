@@ -23,6 +23,23 @@ sys.path.insert(0, os.path.join(ROOT, "chmy.jl_amd"))
 PEAK_HBM_GBS = 8000.0
 
 
+def counter_traffic(n, dtype):
+    """HBM bytes per fused launch from the committed rocprofv3 counter summary
+    of this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scale factors of a
+    1 GiB device copy), or None when the summary does not cover (n, dtype)."""
+    path = os.path.join(ROOT, "profiles", "fused3d_stokes_traffic.txt")
+    try:
+        for line in open(path):
+            line = line.strip()
+            if line.startswith("{"):
+                d = json.loads(line)
+                if d.get("n") == n and d.get("dtype") == dtype:
+                    return d["hbm_bytes_per_launch"]
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_bytes(n, itemsize=8):
     C = n ** 3
     Vf = (n + 1) * n * n
@@ -442,7 +459,8 @@ def main():
                    "teff_reference_convention_GBs": round(reference_bytes(n, isz) / (ms_step * 1e-3) / 1e9, 2),
                    "residual_after_timing": err},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": counter_traffic(n, args.dtype) if fused else None,
                      "kernel": "fused3d_kernel" if fused else "stress_kernel + velocity_kernel",
                      "kernel_ms": round(k_ms, 4)},
         "cpu_baseline": cpu,

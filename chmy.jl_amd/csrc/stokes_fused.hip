@@ -494,7 +494,10 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     }
 }
 
-constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = 32, kYChunk2 = 16;
+#ifndef CHMY_FUSED_ZCHUNK
+#define CHMY_FUSED_ZCHUNK 32
+#endif
+constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = CHMY_FUSED_ZCHUNK, kYChunk2 = 16;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {

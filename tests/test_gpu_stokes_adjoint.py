@@ -32,7 +32,7 @@ def test_grad_eta_kernel_matches_oracle(arch, dims, logscale):
     g = chmy.UniformGrid(arch, origin=(0.0,) * nd, extent=(1.0,) * nd, dims=dims)
     og = ogrid(g)
     rng = np.random.default_rng(3)
-    tl = chmy.tensor_locations(nd)
+    tl = list(chmy.tensor_locations(nd).values())
     tau = [pair(arch, g, og, l, rng=rng) for l in tl]
     taua = [pair(arch, g, og, l, rng=rng) for l in tl]
     eta = pair(arch, g, og, Cc(), rng=rng, lo=0.5, hi=2.0)
