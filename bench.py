@@ -106,12 +106,24 @@ def cpu_baseline(n, iters):
         O.bc_apply(og, spec)
 
     step()  # warm-up
-    t0 = time.perf_counter()
+    ts = []
     for _ in range(iters):
+        t0 = time.perf_counter()
         step()
-    el = (time.perf_counter() - t0) / iters
+        ts.append(time.perf_counter() - t0)
+    el = sorted(ts)[len(ts) // 2]   # median
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return el, cores
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def time_to_converge(args):
@@ -437,8 +449,9 @@ def main():
     if not args.no_cpu:
         c_s, cores = cpu_baseline(n, args.cpu_iters)
         cpu = {"value": round(A / c_s / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-               "sample": f"{args.cpu_iters} PT iterations at {n}^3 FP64 (oracle: unfused update_stress!, "
-                         f"update_velocity!, bc!; {c_s:.3f} s/iteration)"}
+               "sample": f"median of {args.cpu_iters} PT iterations at {n}^3 FP64 (oracle: unfused "
+                         f"update_stress!, update_velocity!, bc!, OpenMP; {c_s:.3f} s/iteration; "
+                         f"CPU {cpu_model()})"}
     out = {
         "metric": "T_eff GB/s (% HBM peak); time-to-1e-8 FP32+fix vs FP64; adjoint/fwd cost",
         "value": round(teff, 2),

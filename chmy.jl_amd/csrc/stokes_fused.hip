@@ -63,13 +63,22 @@ struct Fused3 {
     Kc<T> k;
 };
 
+#ifndef CHMY_FUSED_NT_STORE
+#define CHMY_FUSED_NT_STORE 0
+#endif
+
 template <class T>
 __device__ __forceinline__ T ldo(const T* p, uint32_t boff) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
 }
 template <class T>
 __device__ __forceinline__ void sto(T* p, uint32_t boff, T v) {
+#if CHMY_FUSED_NT_STORE
+    // streaming store: the advanced state is not re-read in this launch
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(p) + boff));
+#else
     *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + boff) = v;
+#endif
 }
 
 // EXACT: the reference's expressions (stokes_3d_inc_ve_T.jl:91-114), bitwise.
