@@ -390,17 +390,22 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
 // y neighbours from registers carried along the march, so there is no LDS,
 // no barrier and no y ring.  At row j a lane computes the Center stresses and
 // Pr at row j and tau_xy at vertex row j+1, then V at row j (the 2D form of
-// the 3D z pipeline).
-template <class T>
-struct DF2 {
-    T* p;
-    int64_t sy;
-};
+// the 3D z pipeline).  Addressing as in 3D: one 32-bit byte offset per
+// staggered location (C, Vx, Vy, xy), advanced by one row per step; lanes
+// right of the storage address the last stored column (their values reach no
+// output), so every load is unconditional.
+enum { L2C = 0, L2X = 1, L2Y = 2, L2XY = 3, NLOC2 = 4 };
 
 template <class T>
 struct Fused2 {
-    DF2<T> V[2], tau[3], Pr, Vo[2], tauo[3], Pro, told[3], eta, eml, rg, dV, rV[2], fV[2], fP;
-    int64_t n[2];
+    const T *Vx, *Vy, *Pr, *txx, *tyy, *txy;
+    T *oVx, *oVy, *oPr, *otxx, *otyy, *otxy;
+    const T *Oxx, *Oyy, *Oxy;
+    const T *eta, *eml, *rg;
+    const T *fVx, *fVy, *fP;
+    T *dV, *rVx, *rVy;
+    LocStride L[NLOC2];
+    int32_t n[2];
     int32_t ntx, nchunks, ychunk;
     int32_t has_rg, write_res, has_f;
     T ix, iy;
@@ -414,70 +419,82 @@ __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kFX); }   // v
 
 constexpr int kW2 = 4;   // waves (x-tiles) per block
 
-template <class T, bool FVISC, bool EXACT>
+template <class T, bool FVISC, bool EXACT, bool XTRA>
 __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
+    const bool has_f = XTRA && a.has_f, write_res = XTRA && a.write_res, has_rg = XTRA && a.has_rg;
     const int lane = threadIdx.x;
     const int xt = blockIdx.x * kW2 + threadIdx.y;
     if (xt >= a.ntx) return;                        // whole wave leaves together
-    const int64_t i = (int64_t)xt * (kFX - 2) - 1 + lane;
-    const int64_t nb0 = a.n[0] + 1, nb1 = a.n[1] + 1;
-    const int64_t jb = (int64_t)blockIdx.y * a.ychunk;
-    const int64_t je = min(jb + (int64_t)a.ychunk, nb1 + 1);
-    const bool valid = i <= a.n[0] + 2;
+    const int i = xt * (kFX - 2) - 1 + lane;
+    const int nb0 = a.n[0] + 1, nb1 = a.n[1] + 1;
+    const int jb = blockIdx.y * a.ychunk;
+    const int je = min(jb + a.ychunk, nb1 + 1);
     const bool inx = i >= 0 && i <= nb0;
     const bool ownx = lane >= 1 && lane <= kFX - 2 && inx;
+    const int ic = min(i, a.n[0] + 2);              // stored column
     const T ix = a.ix, iy = a.iy;
-    auto O = [i](DF2<T> f, int64_t j) { return i + j * f.sy; };
-    auto ld = [i, valid](DF2<T> f, int64_t j) -> T { return valid ? f.p[i + j * f.sy] : T(0); };
+    constexpr uint32_t E = sizeof(T);
+    uint32_t o[NLOC2], dy[NLOC2];
+#pragma unroll
+    for (int c = 0; c < NLOC2; ++c) {
+        o[c] = (a.L[c].o0 + (uint32_t)ic + (uint32_t)(jb - 1) * a.L[c].sy) * E;   // row jb - 1
+        dy[c] = a.L[c].sy * E;
+    }
 
-    T vx_j = ld(a.V[0], jb - 1), vy_j = ld(a.V[1], jb - 1);
-    T eta_j = FVISC ? ld(a.eta, jb - 1) : T(0);
+    T vx_j = ldo(a.Vx, o[L2X]), vy_j = ldo(a.Vy, o[L2Y]);
+    T eta_j = FVISC ? ldo(a.eta, o[L2C]) : T(0);
     T P_jm1 = T(0), tyy_jm1 = T(0), txy_j = T(0), eml_jm1 = T(0);
-    for (int64_t j = jb - 1; j < je; ++j) {
+    for (int j = jb - 1; j < je; ++j) {
         const bool pro = j < jb;
         const bool in_j = inx && j >= 0;
         const bool in_j1 = inx && j + 1 <= nb1;
-        const T vx1 = ld(a.V[0], j + 1), vy1 = ld(a.V[1], j + 1);
-        T P = ld(a.Pr, j), txx = ld(a.tau[0], j), tyy = ld(a.tau[1], j), txy1 = ld(a.tau[2], j + 1);
-        const T eta1 = FVISC ? ld(a.eta, j + 1) : T(0);
-        const T eml = FVISC ? ld(a.eml, j) : T(0);
+        const T vx1 = ldo(a.Vx, o[L2X] + dy[L2X]), vy1 = ldo(a.Vy, o[L2Y] + dy[L2Y]);
+        T P = ldo(a.Pr, o[L2C]), txx = ldo(a.txx, o[L2C]), tyy = ldo(a.tyy, o[L2C]);
+        T txy1 = ldo(a.txy, o[L2XY] + dy[L2XY]);
+        const T oxx = ldo(a.Oxx, o[L2C]), oyy = ldo(a.Oyy, o[L2C]), oxy = ldo(a.Oxy, o[L2XY] + dy[L2XY]);
+        const T eta1 = FVISC ? ldo(a.eta, o[L2C] + dy[L2C]) : T(0);
+        const T eml = FVISC ? ldo(a.eml, o[L2C]) : T(0);
         // Center stresses and Pr at row j
         const T exx = (shdn(vx_j) - vx_j) * ix;
         const T eyy = (vy1 - vy_j) * iy;
         const T dv = exx + eyy;
-        if (in_j) {
+        {
             typename AR::Loc Lc;
             if constexpr (FVISC) Lc = AR::field(eta_j, K);
             else Lc = AR::scalar(K);
-            P = AR::pr(P, a.has_f ? dv - a.fP.p[O(a.fP, j)] : dv, Lc, K);
-            txx = AR::diag(txx, ld(a.told[0], j), exx, dv, Lc, K);
-            tyy = AR::diag(tyy, ld(a.told[1], j), eyy, dv, Lc, K);
+            const T Pn = AR::pr(P, has_f ? dv - ldo(a.fP, o[L2C]) : dv, Lc, K);
+            const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
+            const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+            P = in_j ? Pn : P;
+            txx = in_j ? txxn : txx;
+            tyy = in_j ? tyyn : tyy;
         }
         // tau_xy at vertex row j+1 (needs Vx at rows j, j+1 and Vy at column i-1)
         const T vy1l = shup(vy1);
         const T etal = FVISC ? shup(eta_j) : T(0), eta1l = FVISC ? shup(eta1) : T(0);
-        if (in_j1) {
+        {
             const T exy = T(0.5) * ((vx1 - vx_j) * iy + (vy1 - vy1l) * ix);
             typename AR::Loc Le;
             if constexpr (FVISC) Le = AR::field(lerp4(etal, eta_j, eta1l, eta1), K);
             else Le = AR::scalar(K);
-            txy1 = AR::off(txy1, ld(a.told[2], j + 1), exy, Le, K);
+            const T n = AR::off(txy1, oxy, exy, Le, K);
+            txy1 = in_j1 ? n : txy1;
         }
         const T Pl = shup(P), txxl = shup(txx), txyr = shdn(txy_j), emll = FVISC ? shup(eml) : T(0);
         if (!pro && ownx && in_j) {
-            a.Pro.p[O(a.Pro, j)] = P;
-            a.tauo[0].p[O(a.tauo[0], j)] = txx;
-            a.tauo[1].p[O(a.tauo[1], j)] = tyy;
-            a.tauo[2].p[O(a.tauo[2], j)] = txy_j;
-            if (a.write_res) a.dV.p[O(a.dV, j)] = dv;
+            sto(a.oPr, o[L2C], P);
+            sto(a.otxx, o[L2C], txx);
+            sto(a.otyy, o[L2C], tyy);
+            sto(a.otxy, o[L2XY], txy_j);
+            if (write_res) sto(a.dV, o[L2C], dv);
             T rx = (-((P - Pl) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy;
             T ry = (-((P - P_jm1) * iy) + (tyy - tyy_jm1) * iy) + (txyr - txy_j) * ix;
-            if (a.has_rg) ry = ry - a.rg.p[O(a.rg, j)];
-            if (a.has_f) {
-                rx = rx + a.fV[0].p[O(a.fV[0], j)];
-                ry = ry + a.fV[1].p[O(a.fV[1], j)];
+            if (has_rg) ry = ry - ldo(a.rg, o[L2Y]);
+            if (has_f) {
+                rx = rx + ldo(a.fVx, o[L2X]);
+                ry = ry + ldo(a.fVy, o[L2Y]);
             }
             T ex, ey;
             if constexpr (FVISC) {
@@ -486,11 +503,11 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             } else {
                 ex = ey = AR::vcoef_scalar(K);
             }
-            a.Vo[0].p[O(a.Vo[0], j)] = AR::vel(vx_j, rx, ex, K);
-            a.Vo[1].p[O(a.Vo[1], j)] = AR::vel(vy_j, ry, ey, K);
-            if (a.write_res) {
-                a.rV[0].p[O(a.rV[0], j)] = rx;
-                a.rV[1].p[O(a.rV[1], j)] = ry;
+            sto(a.oVx, o[L2X], AR::vel(vx_j, rx, ex, K));
+            sto(a.oVy, o[L2Y], AR::vel(vy_j, ry, ey, K));
+            if (write_res) {
+                sto(a.rVx, o[L2X], rx);
+                sto(a.rVy, o[L2Y], ry);
             }
         }
         P_jm1 = P;
@@ -500,6 +517,8 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
         vy_j = vy1;
         eta_j = eta1;
         eml_jm1 = eml;
+#pragma unroll
+        for (int c = 0; c < NLOC2; ++c) o[c] += dy[c];
     }
 }
 
@@ -529,18 +548,15 @@ static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double
 }
 
 
-template <class T>
-static DF2<T> v2(const chmy_field_t& f) {
-    DF<T> d = view<T>(f);
-    return DF2<T>{d.p, d.sy};
-}
 
 static LocStride loc_stride(const chmy_field_t& f) {
     const int64_t p0 = pad_of(f, 0), p1 = pad_of(f, 1);
     LocStride s;
     s.sy = (uint32_t)p0;
     s.sz = (uint32_t)(p0 * p1);
-    s.o0 = (uint32_t)((2 * f.halo[0] - 1) + (2 * f.halo[1] - 1) * p0 + (2 * f.halo[2] - 1) * p0 * p1);
+    int64_t o0 = (2 * f.halo[0] - 1) + (2 * f.halo[1] - 1) * p0;
+    if (f.nd >= 3) o0 += (2 * f.halo[2] - 1) * p0 * p1;
+    s.o0 = (uint32_t)o0;
     return s;
 }
 
@@ -553,14 +569,19 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
                         hipStream_t s) {
     const bool fv = eta != nullptr;
-    if (g->nd == 3) {
-        // every byte offset a thread forms must fit in 32 bits
-        const chmy_field_t* all[] = {&V[0], &V[1], &V[2], &tau[3], &tau[4], &tau[5], Pr};
+    // every byte offset a thread forms must fit in 32 bits
+    {
+        const int nd = g->nd;
+        const chmy_field_t* all[] = {&V[0], &V[1], nd == 3 ? &V[2] : Pr, &tau[nd == 3 ? 3 : 2],
+                                     &tau[nd == 3 ? 4 : 2], &tau[nd == 3 ? 5 : 2], Pr};
         for (const chmy_field_t* f : all) {
-            int64_t el = pad_of(*f, 0) * pad_of(*f, 1) * pad_of(*f, 2);
+            int64_t el = 1;
+            for (int d = 0; d < nd; ++d) el *= pad_of(*f, d);
             if (el * (int64_t)sizeof(T) >= (int64_t)UINT32_MAX)
                 return set_error(CHMY_E_SIZE, "fused iteration: a field exceeds 4 GiB (32-bit offsets)");
         }
+    }
+    if (g->nd == 3) {
         Fused3<T> a;
         std::memset(&a, 0, sizeof a);
         auto P = [](const chmy_field_t& f) { return static_cast<T*>(f.data); };
@@ -617,27 +638,28 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
     } else {
         Fused2<T> a;
         std::memset(&a, 0, sizeof a);
-        for (int d = 0; d < 2; ++d) {
-            a.V[d] = v2<T>(V[d]);
-            a.Vo[d] = v2<T>(Vo[d]);
-            a.rV[d] = wres ? v2<T>(rV[d]) : a.Vo[d];
-        }
-        for (int c = 0; c < 3; ++c) {
-            a.tau[c] = v2<T>(tau[c]);
-            a.tauo[c] = v2<T>(tauo[c]);
-            a.told[c] = v2<T>(told[c]);
-        }
-        a.Pr = v2<T>(*Pr);
-        a.Pro = v2<T>(*Pro);
-        a.eta = fv ? v2<T>(*eta) : a.Pr;
-        a.eml = fv ? v2<T>(*eml) : a.Pr;
-        a.rg = rg ? v2<T>(*rg) : a.Pr;
-        a.dV = wres ? v2<T>(*dV) : a.Pro;
+        auto P = [](const chmy_field_t& f) { return static_cast<T*>(f.data); };
+        a.Vx = P(V[0]); a.Vy = P(V[1]); a.Pr = P(*Pr);
+        a.txx = P(tau[0]); a.tyy = P(tau[1]); a.txy = P(tau[2]);
+        a.oVx = P(Vo[0]); a.oVy = P(Vo[1]); a.oPr = P(*Pro);
+        a.otxx = P(tauo[0]); a.otyy = P(tauo[1]); a.otxy = P(tauo[2]);
+        a.Oxx = P(told[0]); a.Oyy = P(told[1]); a.Oxy = P(told[2]);
+        a.eta = fv ? P(*eta) : a.Pr;
+        a.eml = fv ? P(*eml) : a.Pr;
+        a.rg = rg ? P(*rg) : a.Pr;
+        a.dV = wres ? P(*dV) : a.oPr;
+        a.rVx = wres ? P(rV[0]) : a.oVx;
+        a.rVy = wres ? P(rV[1]) : a.oVy;
         a.has_f = fV != nullptr;
-        for (int d = 0; d < 2; ++d) a.fV[d] = fV ? v2<T>(fV[d]) : a.Pr;
-        a.fP = fV ? v2<T>(*fP) : a.Pr;
-        a.n[0] = g->n[0];
-        a.n[1] = g->n[1];
+        a.fVx = fV ? P(fV[0]) : a.Pr;
+        a.fVy = fV ? P(fV[1]) : a.Pr;
+        a.fP = fV ? P(*fP) : a.Pr;
+        a.L[L2C] = loc_stride(*Pr);
+        a.L[L2X] = loc_stride(V[0]);
+        a.L[L2Y] = loc_stride(V[1]);
+        a.L[L2XY] = loc_stride(tau[2]);
+        a.n[0] = (int32_t)g->n[0];
+        a.n[1] = (int32_t)g->n[1];
         a.has_rg = rg != nullptr;
         a.write_res = wres;
         a.ix = T(g->inv_spacing[0]);
@@ -649,10 +671,17 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.nchunks = (int32_t)((nby + kYChunk2 - 1) / kYChunk2);
         dim3 blk(kFX, kW2, 1);
         dim3 grd((unsigned)((a.ntx + kW2 - 1) / kW2), (unsigned)a.nchunks, 1);
-        if (fv && exact) hipLaunchKernelGGL((fused2d_kernel<T, true, true>), grd, blk, 0, s, a);
-        else if (fv) hipLaunchKernelGGL((fused2d_kernel<T, true, false>), grd, blk, 0, s, a);
-        else if (exact) hipLaunchKernelGGL((fused2d_kernel<T, false, true>), grd, blk, 0, s, a);
-        else hipLaunchKernelGGL((fused2d_kernel<T, false, false>), grd, blk, 0, s, a);
+        const bool xtra = a.has_f || a.write_res || a.has_rg;
+#define CHMY_F2(FV, EX)                                                                                   \
+    do {                                                                                                  \
+        if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true>), grd, blk, 0, s, a);               \
+        else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, false>), grd, blk, 0, s, a);                   \
+    } while (0)
+        if (fv && exact) CHMY_F2(true, true);
+        else if (fv) CHMY_F2(true, false);
+        else if (exact) CHMY_F2(false, true);
+        else CHMY_F2(false, false);
+#undef CHMY_F2
     }
     CHMY_HIP(hipGetLastError());
     return 0;
