@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CHMY_MI355X_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libchmy_mi355x.so")
 
-F64, F32 = 0, 1
+F64, F32, BF16 = 0, 1, 2
 CENTER, VERTEX = 0, 1
 DIRICHLET, NEUMANN = 0, 1
 VALUE_ZERO, VALUE_SCALAR, VALUE_FIELD = 0, 1, 2

@@ -15,7 +15,7 @@ import torch
 from . import _lib
 from .grids import Center, Vertex, expand_loc
 
-_DTYPES = {torch.float64: _lib.F64, torch.float32: _lib.F32}
+_DTYPES = {torch.float64: _lib.F64, torch.float32: _lib.F32, torch.bfloat16: _lib.BF16}
 
 
 class Field:

@@ -304,6 +304,9 @@ def pure_shear_initial(grid, V):
 class MixedPTStokes:
     """FP32 PT sweeps with FP64 defect correction (build extension).
 
+    forcing_dtype: storage of the sweeps' forcing (torch.bfloat16 by default,
+    or torch.float32).
+
     The f64 master state x = (V, τ, Pr) lives in `self.x` (a PTStokes in f64;
     only its storage and BC batch are used).  Every `inner` iterations the f64
     residual R(x) is evaluated (chmy_stokes_residual) and becomes the forcing
@@ -315,15 +318,19 @@ class MixedPTStokes:
     """
 
     def __init__(self, arch, grid, *, eta_field=None, eta=None, G=1.0, dt=1.0, normal_values=None,
-                 re_m=2.3 * math.pi, r=0.5, rhog=None, inner=200, exact=False):
+                 re_m=2.3 * math.pi, r=0.5, rhog=None, inner=200, exact=False, forcing_dtype=torch.bfloat16):
         self.grid = grid
         self.inner = int(inner)
         self.x = PTStokes(arch, grid, dtype=torch.float64, eta_field=eta_field, eta=eta, G=G, dt=dt,
                           normal_values=normal_values, re_m=re_m, r=r, rhog=rhog, exact=exact)
         self.d = PTStokes(arch, grid, dtype=torch.float32, eta_field=eta_field, eta=eta, G=G, dt=dt,
                           normal_values={D: (0.0, 0.0) for D in range(grid.nd)}, re_m=re_m, r=r, exact=exact)
-        self.d.f_V = VectorField(arch, grid, torch.float32)
-        self.d.f_P = Field(arch, grid, Center(), torch.float32)
+        # the correction's forcing (G dt R_tau in tau_old, R_V, R_P) is read once
+        # per sweep and only sets the direction of each correction: bf16
+        # storage halves its bytes; the f64 residual still sets the accuracy
+        self.d.tau_old = TensorField(arch, grid, forcing_dtype)
+        self.d.f_V = VectorField(arch, grid, forcing_dtype)
+        self.d.f_P = Field(arch, grid, Center(), forcing_dtype)
         self.cycles = 0
         self.sweeps = 0
 

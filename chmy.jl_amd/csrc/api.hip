@@ -52,11 +52,14 @@ void tensor_loc(int nd, int c, int* loc) {
     for (int d = 0; d < 3; ++d) loc[d] = nd == 2 ? L2[c][d] : L3[c][d];
 }
 
-int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, const int* loc, int dtype) {
+int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, const int* loc, int dtype,
+                bool allow_bf16) {
     if (!f || !f->data) return set_error(CHMY_E_ARG, "field %s is NULL", name);
     if (f->nd != g->nd) return set_error(CHMY_E_LOC, "field %s: nd %d != grid nd %d", name, f->nd, g->nd);
-    if (f->dtype != CHMY_F64 && f->dtype != CHMY_F32)
+    if (f->dtype != CHMY_F64 && f->dtype != CHMY_F32 && f->dtype != CHMY_BF16)
         return set_error(CHMY_E_DTYPE, "field %s: unknown dtype %d", name, f->dtype);
+    if (f->dtype == CHMY_BF16 && !allow_bf16)
+        return set_error(CHMY_E_DTYPE, "field %s: bf16 storage is not accepted here", name);
     if (dtype >= 0 && f->dtype != dtype)
         return set_error(CHMY_E_DTYPE, "field %s: dtype %d, expected %d", name, f->dtype, dtype);
     for (int d = 0; d < g->nd; ++d) {
@@ -72,24 +75,24 @@ int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, c
     return 0;
 }
 
-int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, int dtype) {
+int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, int dtype, bool allow_bf16) {
     const int c[3] = {0, 0, 0};
-    return check_field(g, f, name, c, dtype);
+    return check_field(g, f, name, c, dtype, allow_bf16);
 }
 
-int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype) {
+int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype, bool allow_bf16) {
     if (!V) return set_error(CHMY_E_ARG, "vector field %s is NULL", name);
     char nm[64];
     for (int D = 0; D < g->nd; ++D) {
         int loc[3] = {0, 0, 0};
         loc[D] = 1;
         std::snprintf(nm, sizeof nm, "%s[%d]", name, D);
-        CHMY_TRY(check_field(g, &V[D], nm, loc, dtype));
+        CHMY_TRY(check_field(g, &V[D], nm, loc, dtype, allow_bf16));
     }
     return 0;
 }
 
-int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype) {
+int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype, bool allow_bf16) {
     if (!tau) return set_error(CHMY_E_ARG, "tensor field %s is NULL", name);
     if (g->nd < 2) return set_error(CHMY_E_ARG, "tensor fields need nd >= 2");
     char nm[64];
@@ -98,7 +101,7 @@ int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name
         int loc[3];
         tensor_loc(g->nd, c, loc);
         std::snprintf(nm, sizeof nm, "%s[%d]", name, c);
-        CHMY_TRY(check_field(g, &tau[c], nm, loc, dtype));
+        CHMY_TRY(check_field(g, &tau[c], nm, loc, dtype, allow_bf16));
     }
     return 0;
 }

@@ -27,6 +27,28 @@ int hip_status(hipError_t e, const char* what);
         if (rc_ != 0) return rc_;    \
     } while (0)
 
+// ------------------------------------------------------------------ bfloat16
+// Storage type of bf16 fields (the forcing of FP32 correction sweeps):
+// f32 -> bf16 rounds to nearest even, bf16 -> f32 is exact.
+struct bf16s {
+    uint16_t u;
+};
+__device__ __forceinline__ float bf16_to_f32(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);   // NaN stays a quiet NaN
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (uint16_t)(x >> 16);
+}
+// value conversions used by kernels that write in a chosen precision
+template <class TO, class T>
+__device__ __forceinline__ void put(TO& dst, T v) { dst = TO(v); }
+template <class T>
+__device__ __forceinline__ void put(bf16s& dst, T v) { dst.u = f32_to_bf16(float(v)); }
+template <class T>
+__device__ __forceinline__ double as_double(T v) { return (double)v; }
+__device__ __forceinline__ double as_double(bf16s v) { return (double)bf16_to_f32(v.u); }
+
 // ------------------------------------------------------------------ views
 // Device view of a Field: p points at logical index (0,0,0), so logical I is
 // p[i + j*sy + k*sz] (storage offset I + 2H - 1 per dim, field.jl:18).
@@ -87,10 +109,13 @@ DFL<T> lview(const chmy_field_t& f) {
 int check_grid(const chmy_grid_t* g);
 int check_box(const chmy_grid_t* g, const chmy_box_t* b, chmy_box_t* out);
 // loc: expected location per dim (-1 = any); dtype: expected (-1 = any).
-int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, const int* loc, int dtype);
-int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, int dtype);
-int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype);
-int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype);
+// bf16 storage is rejected unless allow_bf16 (only the forcing of FP32
+// correction sweeps and the reductions read it).
+int check_field(const chmy_grid_t* g, const chmy_field_t* f, const char* name, const int* loc, int dtype,
+                bool allow_bf16 = false);
+int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, int dtype, bool allow_bf16 = false);
+int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype, bool allow_bf16 = false);
+int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype, bool allow_bf16 = false);
 void tensor_loc(int nd, int c, int* loc);
 
 struct Box3 {

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned l
     for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
         int64_t i = t % nx, r = t / nx;
         int64_t j = r % ny, k = r / ny;
-        double a = fabs((double)f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k));
+        double a = fabs(as_double(f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k)));
         if (a != a) isnan_ = 1;
         else m = fmax(m, a);
     }
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kRB) sumsq_kernel(Box3 b, DF<T> f, double* par
     for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
         int64_t i = t % nx, r = t / nx;
         int64_t j = r % ny, k = r / ny;
-        double a = (double)f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k);
+        double a = as_double(f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k));
         s += a * a;
     }
     s = wave_sum(s);
@@ -116,6 +116,8 @@ static int scratch(size_t bytes, void** p) {
 static int reduce_box(const chmy_field_t* f, const chmy_box_t* box, Box3* out) {
     if (!f || !f->data) return set_error(CHMY_E_ARG, "field is NULL");
     if (f->nd < 1 || f->nd > 3) return set_error(CHMY_E_ARG, "field nd %d", f->nd);
+    if (f->dtype != CHMY_F64 && f->dtype != CHMY_F32 && f->dtype != CHMY_BF16)
+        return set_error(CHMY_E_DTYPE, "field dtype %d", f->dtype);
     chmy_box_t b;
     for (int d = 0; d < 3; ++d) { b.lo[d] = 1; b.len[d] = d < f->nd ? f->n[d] : 1; }
     if (box) b = *box;
@@ -151,6 +153,8 @@ int chmy_maxabs_dev(const chmy_field_t* f, const chmy_box_t* box, double* out_de
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, o);
     if (f->dtype == CHMY_F64)
         hipLaunchKernelGGL(maxabs_kernel<double>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<double>(*f), o);
+    else if (f->dtype == CHMY_BF16)
+        hipLaunchKernelGGL(maxabs_kernel<bf16s>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<bf16s>(*f), o);
     else
         hipLaunchKernelGGL(maxabs_kernel<float>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<float>(*f), o);
     CHMY_HIP(hipGetLastError());
@@ -177,7 +181,9 @@ int chmy_sumsq(const chmy_field_t* f, const chmy_box_t* box, double* out_host, v
     double* partial = static_cast<double*>(p);
     double* res = partial + kSumBlocks;
     hipStream_t s = (hipStream_t)stream;
-    if (f->dtype == CHMY_F64)
+    if (f->dtype == CHMY_BF16)
+        hipLaunchKernelGGL(sumsq_kernel<bf16s>, dim3(kSumBlocks), dim3(kRB), 0, s, b, view<bf16s>(*f), partial);
+    else if (f->dtype == CHMY_F64)
         hipLaunchKernelGGL(sumsq_kernel<double>, dim3(kSumBlocks), dim3(kRB), 0, s, b, view<double>(*f), partial);
     else
         hipLaunchKernelGGL(sumsq_kernel<float>, dim3(kSumBlocks), dim3(kRB), 0, s, b, view<float>(*f), partial);

@@ -28,6 +28,8 @@
 // kernels (stokes.hip), so the results are bitwise those of update_stress! +
 // update_velocity!; the default fast form (Arith<T, false>) is algebraically
 // equal and agrees to a few ulp per iteration.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace chmy {
@@ -71,6 +73,19 @@ template <class T>
 __device__ __forceinline__ T ldo(const T* p, uint32_t boff) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
 }
+// Loads of the sweep's read-only forcing (tau_old, f_V, f_P): in the sweep
+// precision, or (BF, FP32 sweeps only) from bf16 storage at half the byte
+// offset of the FP32 layout.
+template <class T, bool BF>
+__device__ __forceinline__ T ldf(const T* p, uint32_t boff) {
+    if constexpr (BF) {
+        static_assert(sizeof(T) == 4, "bf16 forcing goes with FP32 sweeps");
+        return T(bf16_to_f32(*reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(p) + (boff >> 1))));
+    } else {
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void sto(T* p, uint32_t boff, T v) {
 #if CHMY_FUSED_NT_STORE
@@ -163,7 +178,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // rows j-1, j and y-vertex edges at rows j, j+1, so rows ty = 1..BY-1 are
 // outputs and only row 0 is a ring.  The last wave also stages row j + 1 of
 // V and eta (LDS row BY).
-template <class T, bool FVISC, bool EXACT, int BY, bool XTRA>
+template <class T, bool FVISC, bool EXACT, int BY, bool XTRA, bool BF>
 __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
@@ -243,9 +258,9 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
         T P = ld(a.Pr, o[LC]), txx = ld(a.txx, o[LC]), tyy = ld(a.tyy, o[LC]), tzz = ld(a.tzz, o[LC]);
         T txy1 = ld(a.txy, o[LXY] + dy[LXY]);
         T txz1 = ld(a.txz, o[LXZ] + dz[LXZ]), tyz1 = ld(a.tyz, o[LYZ] + dy[LYZ] + dz[LYZ]);
-        const T oxx = ldo(a.Oxx, o[LC]), oyy = ldo(a.Oyy, o[LC]), ozz = ldo(a.Ozz, o[LC]);
-        const T oxy = ldo(a.Oxy, o[LXY] + dy[LXY]);
-        const T oxz = ldo(a.Oxz, o[LXZ] + dz[LXZ]), oyz = ldo(a.Oyz, o[LYZ] + dy[LYZ] + dz[LYZ]);
+        const T oxx = ldf<T, BF>(a.Oxx, o[LC]), oyy = ldf<T, BF>(a.Oyy, o[LC]), ozz = ldf<T, BF>(a.Ozz, o[LC]);
+        const T oxy = ldf<T, BF>(a.Oxy, o[LXY] + dy[LXY]);
+        const T oxz = ldf<T, BF>(a.Oxz, o[LXZ] + dz[LXZ]), oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ] + dz[LYZ]);
         T vx8 = T(0), vy8_1 = T(0), vz8_1 = T(0), eta8_1 = T(0);   // row j+1 (staging wave)
         if (last) {
             vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
@@ -281,7 +296,7 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
             typename AR::Loc Lc;
             if constexpr (FVISC) Lc = AR::field(eta_k, K);
             else Lc = AR::scalar(K);
-            const T Pn = AR::pr(P, has_f ? dv - ldo(a.fP, o[LC]) : dv, Lc, K);
+            const T Pn = AR::pr(P, has_f ? dv - ldf<T, BF>(a.fP, o[LC]) : dv, Lc, K);
             const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
             const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
             const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
@@ -348,9 +363,9 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
                    (tyz_k - s_tyz[p ^ 1][tym][tx]) * iy;
             if (has_rg) rz = rz - ldo(a.rg, o[LVZ]);
             if (has_f) {
-                rx = rx + ldo(a.fVx, o[LVX]);
-                ry = ry + ldo(a.fVy, o[LVY]);
-                rz = rz + ldo(a.fVz, o[LVZ]);
+                rx = rx + ldf<T, BF>(a.fVx, o[LVX]);
+                ry = ry + ldf<T, BF>(a.fVy, o[LVY]);
+                rz = rz + ldf<T, BF>(a.fVz, o[LVZ]);
             }
             T ex, ey, ez;
             if constexpr (FVISC) {
@@ -419,7 +434,7 @@ __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kFX); }   // v
 
 constexpr int kW2 = 4;   // waves (x-tiles) per block
 
-template <class T, bool FVISC, bool EXACT, bool XTRA>
+template <class T, bool FVISC, bool EXACT, bool XTRA, bool BF>
 __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
@@ -453,7 +468,7 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
         const T vx1 = ldo(a.Vx, o[L2X] + dy[L2X]), vy1 = ldo(a.Vy, o[L2Y] + dy[L2Y]);
         T P = ldo(a.Pr, o[L2C]), txx = ldo(a.txx, o[L2C]), tyy = ldo(a.tyy, o[L2C]);
         T txy1 = ldo(a.txy, o[L2XY] + dy[L2XY]);
-        const T oxx = ldo(a.Oxx, o[L2C]), oyy = ldo(a.Oyy, o[L2C]), oxy = ldo(a.Oxy, o[L2XY] + dy[L2XY]);
+        const T oxx = ldf<T, BF>(a.Oxx, o[L2C]), oyy = ldf<T, BF>(a.Oyy, o[L2C]), oxy = ldf<T, BF>(a.Oxy, o[L2XY] + dy[L2XY]);
         const T eta1 = FVISC ? ldo(a.eta, o[L2C] + dy[L2C]) : T(0);
         const T eml = FVISC ? ldo(a.eml, o[L2C]) : T(0);
         // Center stresses and Pr at row j
@@ -464,7 +479,7 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             typename AR::Loc Lc;
             if constexpr (FVISC) Lc = AR::field(eta_j, K);
             else Lc = AR::scalar(K);
-            const T Pn = AR::pr(P, has_f ? dv - ldo(a.fP, o[L2C]) : dv, Lc, K);
+            const T Pn = AR::pr(P, has_f ? dv - ldf<T, BF>(a.fP, o[L2C]) : dv, Lc, K);
             const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
             const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
             P = in_j ? Pn : P;
@@ -493,8 +508,8 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             T ry = (-((P - P_jm1) * iy) + (tyy - tyy_jm1) * iy) + (txyr - txy_j) * ix;
             if (has_rg) ry = ry - ldo(a.rg, o[L2Y]);
             if (has_f) {
-                rx = rx + ldo(a.fVx, o[L2X]);
-                ry = ry + ldo(a.fVy, o[L2Y]);
+                rx = rx + ldf<T, BF>(a.fVx, o[L2X]);
+                ry = ry + ldf<T, BF>(a.fVy, o[L2Y]);
             }
             T ex, ey;
             if constexpr (FVISC) {
@@ -523,7 +538,7 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
 }
 
 #ifndef CHMY_FUSED_ZCHUNK
-#define CHMY_FUSED_ZCHUNK 32
+#define CHMY_FUSED_ZCHUNK 16
 #endif
 constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = CHMY_FUSED_ZCHUNK, kYChunk2 = 16;
 
@@ -569,6 +584,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
                         hipStream_t s) {
     const bool fv = eta != nullptr;
+    const bool bf = told[0].dtype == CHMY_BF16;   // validated: FP32 sweep with bf16 forcing
     // every byte offset a thread forms must fit in 32 bits
     {
         const int nd = g->nd;
@@ -627,8 +643,10 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         const bool xtra = a.has_f || a.write_res || a.has_rg;
 #define CHMY_F3(FV, EX)                                                                                   \
     do {                                                                                                  \
-        if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true>), grd, blk, 0, s, a);          \
-        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, false>), grd, blk, 0, s, a);              \
+        if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true, std::is_same<T, float>::value>), \
+                                   grd, blk, 0, s, a);                                                    \
+        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true, false>), grd, blk, 0, s, a); \
+        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, false, false>), grd, blk, 0, s, a);       \
     } while (0)
         if (fv && exact) CHMY_F3(true, true);
         else if (fv) CHMY_F3(true, false);
@@ -674,8 +692,10 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         const bool xtra = a.has_f || a.write_res || a.has_rg;
 #define CHMY_F2(FV, EX)                                                                                   \
     do {                                                                                                  \
-        if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true>), grd, blk, 0, s, a);               \
-        else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, false>), grd, blk, 0, s, a);                   \
+        if (bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true, std::is_same<T, float>::value>), grd, \
+                                   blk, 0, s, a);                                                         \
+        else if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true, false>), grd, blk, 0, s, a);    \
+        else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, false, false>), grd, blk, 0, s, a);            \
     } while (0)
         if (fv && exact) CHMY_F2(true, true);
         else if (fv) CHMY_F2(true, false);
@@ -712,7 +732,11 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
     CHMY_TRY(check_vector(g, V_out, "V_out", dt_));
     CHMY_TRY(check_tensor(g, tau, "tau", dt_));
     CHMY_TRY(check_tensor(g, tau_out, "tau_out", dt_));
-    CHMY_TRY(check_tensor(g, tau_old, "tau_old", dt_));
+    // bf16 tau_old goes with bf16 f_V/f_P in an FP32 sweep (correction forcing)
+    const bool bf = tau_old && tau_old[0].dtype == CHMY_BF16;
+    if (bf && (dt_ != CHMY_F32 || !f_V || f_V[0].dtype != CHMY_BF16))
+        return set_error(CHMY_E_DTYPE, "bf16 tau_old needs an FP32 sweep with bf16 f_V and f_P");
+    CHMY_TRY(check_tensor(g, tau_old, "tau_old", bf ? CHMY_BF16 : dt_, bf));
     if ((eta == nullptr) != (eta_ml == nullptr)) return set_error(CHMY_E_ARG, "eta and eta_ml go together");
     if (eta) {
         CHMY_TRY(check_center(g, eta, "eta", dt_));
@@ -729,8 +753,8 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
     }
     if ((f_V == nullptr) != (f_P == nullptr)) return set_error(CHMY_E_ARG, "f_V and f_P go together");
     if (f_V) {
-        CHMY_TRY(check_vector(g, f_V, "f_V", dt_));
-        CHMY_TRY(check_center(g, f_P, "f_P", dt_));
+        CHMY_TRY(check_vector(g, f_V, "f_V", bf ? CHMY_BF16 : dt_, bf));
+        CHMY_TRY(check_center(g, f_P, "f_P", bf ? CHMY_BF16 : dt_, bf));
     }
     // one stride set per staggered location: every field must have halo 1
     const int nd = g->nd, nt = nd == 2 ? 3 : 6;

@@ -85,9 +85,9 @@ __global__ void __launch_bounds__(256) residual_kernel(Box3 b, ResArgs<T, TO> a)
         }
         const T t = a.tau[c].at(O(a.tau[c])), to = a.told[c].at(O(a.told[c]));
         const T r = diag ? r_tau_diag(t, to, e[c], dv, eta, a.Gdt) : r_tau_off(t, to, e[c], eta, a.Gdt);
-        a.toc[c].at(O(a.toc[c])) = TO(a.Gdt * r);
+        put(a.toc[c].at(O(a.toc[c])), a.Gdt * r);
     }
-    a.fP.at(O(a.fP)) = TO(-dv);
+    put(a.fP.at(O(a.fP)), -dv);
     // R_V (update_velocity!'s r_V, stokes_3d_inc_ve_T.jl:109-111 / 2D :39-40)
     const T P = a.Pr.at(oc);
     T r[3];
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) residual_kernel(Box3 b, ResArgs<T, TO> a)
     if (a.has_fe)
         for (int d = 0; d < ND; ++d) r[d] = r[d] + a.fe[d].at(O(a.fe[d]));
     for (int d = 0; d < ND; ++d) {
-        a.fV[d].at(O(a.fV[d])) = TO(r[d]);
+        put(a.fV[d].at(O(a.fV[d])), r[d]);
         if (a.has_norm) a.rV[d].at(O(a.rV[d])) = r[d];
     }
     if (a.has_norm) a.dV.at(O(a.dV)) = dv;
@@ -230,10 +230,10 @@ int chmy_stokes_residual(const chmy_grid_t* g, const chmy_field_t* V, const chmy
         CHMY_TRY(check_field(g, rhog, "rhog", loc, CHMY_F64));
     }
     if (f_ext) CHMY_TRY(check_vector(g, f_ext, "f_ext", CHMY_F64));
-    CHMY_TRY(check_center(g, f_P, "f_P", -1));
+    CHMY_TRY(check_center(g, f_P, "f_P", -1, true));
     const int to = f_P->dtype;
-    CHMY_TRY(check_vector(g, f_V, "f_V", to));
-    CHMY_TRY(check_tensor(g, tau_old_c, "tau_old_c", to));
+    CHMY_TRY(check_vector(g, f_V, "f_V", to, true));
+    CHMY_TRY(check_tensor(g, tau_old_c, "tau_old_c", to, true));
     if ((divV == nullptr) != (r_V == nullptr)) return set_error(CHMY_E_ARG, "divV and r_V go together");
     if (divV) {
         CHMY_TRY(check_center(g, divV, "divV", CHMY_F64));
@@ -242,10 +242,14 @@ int chmy_stokes_residual(const chmy_grid_t* g, const chmy_field_t* V, const chmy
     chmy_box_t bx;
     CHMY_TRY(check_box(g, nullptr, &bx));
     hipStream_t s = (hipStream_t)stream;
-    return to == CHMY_F32 ? residual_launch<double, float>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt,
-                                                           tau_old_c, f_V, f_P, divV, r_V, s)
-                          : residual_launch<double, double>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt,
-                                                            tau_old_c, f_V, f_P, divV, r_V, s);
+    if (to == CHMY_F32)
+        return residual_launch<double, float>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt, tau_old_c,
+                                              f_V, f_P, divV, r_V, s);
+    if (to == CHMY_BF16)
+        return residual_launch<double, bf16s>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt, tau_old_c,
+                                              f_V, f_P, divV, r_V, s);
+    return residual_launch<double, double>(g, bx, V, tau, Pr, tau_old, eta, rhog, f_ext, eta_s, G, dt, tau_old_c,
+                                           f_V, f_P, divV, r_V, s);
 }
 
 int chmy_stokes_accumulate(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau,

@@ -35,7 +35,9 @@
 extern "C" {
 #endif
 
-enum { CHMY_F64 = 0, CHMY_F32 = 1 };
+/* CHMY_BF16 (build extension): bfloat16 storage, accepted only for the
+ * forcing/tau_old fields of FP32 correction sweeps and for chmy_maxabs*. */
+enum { CHMY_F64 = 0, CHMY_F32 = 1, CHMY_BF16 = 2 };
 enum { CHMY_CENTER = 0, CHMY_VERTEX = 1 };
 enum { CHMY_DIRICHLET = 0, CHMY_NEUMANN = 1 };
 enum { CHMY_VALUE_ZERO = 0, CHMY_VALUE_SCALAR = 1, CHMY_VALUE_FIELD = 2 };

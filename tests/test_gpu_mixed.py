@@ -30,7 +30,17 @@ def arch():
 
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("visc", ["scalar", "field"])
-@pytest.mark.parametrize("out", [np.float64, np.float32])
+def _bf16_bits(a):
+    """Round-to-nearest-even bf16 bits of float32(a) (numpy), the device rule."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _devbits(f):
+    return f.data.detach().view(torch.int16).cpu().numpy().view(np.uint16)
+
+
+@pytest.mark.parametrize("out", [np.float64, np.float32, "bf16"])
 def test_residual_kernel(arch, nd, visc, out):
     g = chmy.UniformGrid(arch, origin=(-0.5,) * nd, extent=(1.0,) * nd, dims=(37, 12, 9)[:nd] if nd == 3 else (70, 33))
     og = ogrid(g)
@@ -45,7 +55,7 @@ def test_residual_kernel(arch, nd, visc, out):
     rg = pair(arch, g, og, vloc(nd - 1, nd), T, rng=rng)
     dV = pair(arch, g, og, Cc(), T, fill=0)
     rV = [pair(arch, g, og, vloc(D, nd), T, fill=0) for D in range(nd)]
-    to_t = torch.float64 if out == np.float64 else torch.float32
+    to_t = {np.float64: torch.float64, np.float32: torch.float32, "bf16": torch.bfloat16}[out]
     toc = [chmy.Field(arch, g, l, to_t) for l in tl]
     fV = [chmy.Field(arch, g, vloc(D, nd), to_t) for D in range(nd)]
     fP = chmy.Field(arch, g, Cc(), to_t)
@@ -62,13 +72,17 @@ def test_residual_kernel(arch, nd, visc, out):
     RP = O.HField(og, 0)
     O.stokes_residual(og, H(V), H(tau), Pr[1], H(told), eta[1] if fv else eta_s, rg[1], G, dt, Rt, RV, RP)
     sl = tuple(slice(1, -1) for _ in range(nd))
+    if out == "bf16":   # f64 value -> f32 -> bf16, both rounded to nearest even
+        same = lambda f, ref: np.array_equal(_devbits(f)[sl], _bf16_bits(ref[sl]))
+    else:
+        same = lambda f, ref: np.array_equal(dev(f)[sl], ref[sl].astype(out))
     for D in range(nd):
         assert np.array_equal(dev(rV[D][0])[sl], RV[D].arr[sl])
-        assert np.array_equal(dev(fV[D])[sl], RV[D].arr[sl].astype(out))
+        assert same(fV[D], RV[D].arr)
     assert np.array_equal(dev(dV[0])[sl], -RP.arr[sl])
-    assert np.array_equal(dev(fP)[sl], RP.arr[sl].astype(out))
+    assert same(fP, RP.arr)
     for c in range(len(tl)):
-        assert np.array_equal(dev(toc[c])[sl], ((G * dt) * Rt[c].arr[sl]).astype(out))
+        assert same(toc[c], (G * dt) * Rt[c].arr)
 
 
 def _setup_2d(arch, n):
@@ -108,3 +122,32 @@ def test_defect_correction_reaches_f64_fixed_point(arch, nd):
         assert np.max(np.abs(ai - bi)) / scale < 1e-6
     # the f32 sweeps did not need vastly more iterations than the f64 solve
     assert sweeps < 3 * it64 + 10 * mx.inner
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("exact", [False, True])
+def test_bf16_forcing_sweep_equals_f32_forcing_of_same_values(arch, nd, exact):
+    """An FP32 sweep reading bf16 tau_old/f_V/f_P gives the same bits as one
+    reading FP32 fields holding the same (bf16-representable) values."""
+    dims = (40, 19, 11)[:nd] if nd == 3 else (70, 45)
+    g = chmy.UniformGrid(arch, origin=(-0.5,) * nd, extent=(1.0,) * nd, dims=dims)
+    rng = np.random.default_rng(9)
+    eta = np.exp(rng.uniform(-1, 1, tuple(n + 4 for n in reversed(dims))))
+    runs = []
+    for fdt in (torch.bfloat16, torch.float32):
+        S = PTStokes(arch, g, dtype=torch.float32, eta_field=eta, G=1.0, dt=0.3, exact=exact)
+        S.tau_old = chmy.TensorField(arch, g, fdt)
+        S.f_V = chmy.VectorField(arch, g, fdt)
+        S.f_P = chmy.Field(arch, g, Cc(), fdt)
+        r2 = np.random.default_rng(3)
+        for f in list(S.V) + list(S.tau) + [S.Pr]:
+            f.data.copy_(torch.from_numpy(r2.uniform(-1, 1, tuple(f.data.shape)).astype(np.float32)))
+        for f in list(S.tau_old) + list(S.f_V) + [S.f_P]:
+            v = torch.from_numpy(r2.uniform(-1, 1, tuple(f.data.shape)).astype(np.float32))
+            f.data.copy_(v.to(torch.bfloat16).to(fdt))
+        S.sync_buffers()
+        for _ in range(3):
+            S.iteration()
+        runs.append([f.data.cpu().numpy() for f in list(S.V) + list(S.tau) + [S.Pr]])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
