@@ -28,8 +28,6 @@ def arch():
     return chmy.Arch(chmy.MI355XBackend())
 
 
-@pytest.mark.parametrize("nd", [2, 3])
-@pytest.mark.parametrize("visc", ["scalar", "field"])
 def _bf16_bits(a):
     """Round-to-nearest-even bf16 bits of float32(a) (numpy), the device rule."""
     u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
@@ -40,6 +38,8 @@ def _devbits(f):
     return f.data.detach().view(torch.int16).cpu().numpy().view(np.uint16)
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("visc", ["scalar", "field"])
 @pytest.mark.parametrize("out", [np.float64, np.float32, "bf16"])
 def test_residual_kernel(arch, nd, visc, out):
     g = chmy.UniformGrid(arch, origin=(-0.5,) * nd, extent=(1.0,) * nd, dims=(37, 12, 9)[:nd] if nd == 3 else (70, 33))
