@@ -1,0 +1,92 @@
+"""The physical time loop of examples/stokes_3d_inc_ve_T.jl (:150-240, 2D
+stokes_2d_inc_ve_T.jl) on the fused PT iteration.
+
+Per time step it: update_old! (T, τ → T_old, τ_old); dt = min(dt_diff,
+dt_adv) with dt_adv = 0.1 min(Δ_D / max|V_D|) / N / 2.1 (∞ while V = 0);
+η_ve = 1/(1/η + 1/(G dt)); then up to niter PT iterations, each the fused
+update_stress! + update_velocity! with the V batch, followed from the second
+time step on by update_thermal_flux!, update_thermal! and the T Neumann
+batch; every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
+boundary vertices, scaled by τsc, lz/psc) decides convergence, and from the
+third step on ncheck = ceil(0.5 n_x).  With exact=True the states are
+bitwise those of the reference's kernel sequence.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .boundary_conditions import Neumann, batch
+from .fields import Field, VectorField
+from .grids import Center, spacing
+from .kernel_launch import Launcher
+from .kernels import maxabs, update_old, update_thermal, update_thermal_flux
+from .stokes import PTStokes
+
+
+class StokesThermal:
+    def __init__(self, arch, grid, *, eta=10.0, G=1.0, lam_rhoCp=None, rhog=None, normal_values=None,
+                 re_m=2.3 * math.pi, r=0.5, exact=False):
+        self.arch, self.grid = arch, grid
+        self.eta, self.G = float(eta), float(G)
+        L = grid.extent
+        tau_sc = self.eta / self.G
+        self.lam = float(lam_rhoCp) if lam_rhoCp is not None else 1e-4 * L[-1] ** 2 / tau_sc
+        self.S = PTStokes(arch, grid, dtype=torch.float64, eta=eta, G=G, dt=1.0, normal_values=normal_values,
+                          re_m=re_m, r=r, rhog=rhog, exact=exact)
+        self.T = Field(arch, grid, Center())
+        self.T_old = Field(arch, grid, Center())
+        self.qT = VectorField(arch, grid)
+        self._Tbc = batch(grid, (self.T, Neumann()))
+        self.launch = Launcher(arch, grid, sync=False)
+        self.scales = dict(tau_sc=tau_sc, lz=L[-1], psc=self.G)
+
+    def apply_bc(self):
+        self.S.apply_bc()
+        self._Tbc.apply()
+        self.S.sync_buffers()
+
+    def dt_rule(self):
+        g = self.grid
+        d = spacing(g)
+        dt_diff = min(d) ** 2 / self.lam / g.nd / 2.1
+        vmax = [maxabs(self.S.V[D]) for D in range(g.nd)]
+        dt_adv = 0.1 * min(d[D] / vmax[D] if vmax[D] > 0 else math.inf for D in range(g.nd)) / g.nd / 2.1
+        return min(dt_diff, dt_adv)
+
+    def time_step(self, it, niter, ncheck, eps):
+        """One physical step (it is 1-based as in the reference); returns
+        (PT iterations, last residual)."""
+        S, g = self.S, self.grid
+        self.launch(self.arch, g, (update_old, (self.T, S.tau, self.T_old, S.tau_old)))
+        dt = self.dt_rule()
+        S.dt = dt
+        S.eta_ve_s = 1.0 / (1.0 / self.eta + 1.0 / (self.G * dt))
+        err = None
+        for q in range(1, niter + 1):
+            check = q % ncheck == 0
+            S.iteration(write_res=check)
+            if it > 1:
+                self.launch(self.arch, g, (update_thermal_flux, (self.qT, self.T, S.V, self.lam, g)))
+                self.launch(self.arch, g, (update_thermal, (self.T, self.T_old, self.qT, dt, g)), bc=self._Tbc)
+            if check:
+                err = S.residual(**self.scales)
+                if all(e < eps for e in err):
+                    return q, err
+                if not all(math.isfinite(e) for e in err):
+                    raise FloatingPointError(f"simulation failed, err = {err}")
+        return niter, err
+
+    def run(self, nt=2, niter=None, ncheck=None, eps=1e-6):
+        """The reference's defaults: niter = 50 n_x, ncheck = 2 n_x, then
+        ceil(0.5 n_x) from the third step on."""
+        nx = self.grid.dims[0]
+        niter = niter or 50 * nx
+        ncheck = ncheck or 2 * nx
+        out = []
+        for it in range(1, nt + 1):
+            if it > 2:
+                ncheck = math.ceil(0.5 * nx)
+            out.append(self.time_step(it, niter, ncheck, eps))
+        return out
