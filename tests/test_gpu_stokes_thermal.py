@@ -102,6 +102,11 @@ def test_time_loop_matches_reference_sequence(arch, nd):
                     break
         iters.append(done)
     assert [r[0] for r in res] == iters, (res, iters)
+    # non-trivial: the buoyant inclusion drives a flow, both steps needed several
+    # residual checks, and the second step moved T
+    assert min(iters) >= 2 * ncheck, iters
+    assert max(abs(float(O.maxabs(f))) for f in Vf) > 1e-5
+    assert not np.array_equal(T.arr, T_arr)
     sl = (slice(1, -1),) * nd     # logical 0..n+2: interior, halos, boundary nodes
     for a, b in zip(list(M.S.V) + [M.S.Pr] + list(M.S.tau) + [M.T], Vf + [Pr] + tau + [T]):
         assert np.array_equal(dev(a)[sl], b.arr[sl])
