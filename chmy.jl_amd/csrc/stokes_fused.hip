@@ -169,8 +169,9 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 #define CHMY_FUSED_WPE 0
 #endif
 
-// XTRA = false compiles out the optional inputs/outputs (forcing, residual
-// fields, rho g) so that the plain iteration keeps its pointers in SGPRs.
+// OPT selects the optional inputs/outputs at compile time so that the common
+// variants keep their pointers in SGPRs: 0 none, 1 forcing only (the
+// correction sweeps), 7 forcing, residual fields and rho g as flagged.
 //
 // Rows: thread row ty sits at y = j = j0 - 1 + ty (j0 = tile * (BY - 1)) and
 // computes the Center stresses and tau_xz at row j, and the y-vertex edge
@@ -178,11 +179,12 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // rows j-1, j and y-vertex edges at rows j, j+1, so rows ty = 1..BY-1 are
 // outputs and only row 0 is a ring.  The last wave also stages row j + 1 of
 // V and eta (LDS row BY).
-template <class T, bool FVISC, bool EXACT, int BY, bool XTRA, bool BF>
+template <class T, bool FVISC, bool EXACT, int BY, int OPT, bool BF>
 __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
-    const bool has_f = XTRA && a.has_f, write_res = XTRA && a.write_res, has_rg = XTRA && a.has_rg;
+    const bool has_f = OPT == 1 || (OPT == 7 && a.has_f);
+    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 7 && a.has_rg;
     constexpr int NR = BY + 1;   // LDS rows of the V and eta planes
     __shared__ T s_vx[NR][kFX], s_vy[2][NR][kFX], s_vz1[NR][kFX];
     __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? NR : 1][kFX], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][kFX];
@@ -434,11 +436,12 @@ __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kFX); }   // v
 
 constexpr int kW2 = 4;   // waves (x-tiles) per block
 
-template <class T, bool FVISC, bool EXACT, bool XTRA, bool BF>
+template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
 __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
-    const bool has_f = XTRA && a.has_f, write_res = XTRA && a.write_res, has_rg = XTRA && a.has_rg;
+    const bool has_f = OPT == 1 || (OPT == 7 && a.has_f);
+    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 7 && a.has_rg;
     const int lane = threadIdx.x;
     const int xt = blockIdx.x * kW2 + threadIdx.y;
     if (xt >= a.ntx) return;                        // whole wave leaves together
@@ -641,12 +644,15 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
         dim3 blk(kFX, kBY3, 1), grd(nb, 1, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
+        const bool fonly = a.has_f && !a.write_res && !a.has_rg;
 #define CHMY_F3(FV, EX)                                                                                   \
     do {                                                                                                  \
-        if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true, std::is_same<T, float>::value>), \
-                                   grd, blk, 0, s, a);                                                    \
-        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, true, false>), grd, blk, 0, s, a); \
-        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, false, false>), grd, blk, 0, s, a);       \
+        constexpr bool F32 = std::is_same<T, float>::value;                                               \
+        if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 1, F32>), grd, blk, 0, s, a); \
+        else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 1, false>), grd, blk, 0, s, a); \
+        else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 7, F32>), grd, blk, 0, s, a);    \
+        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 7, false>), grd, blk, 0, s, a); \
+        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 0, false>), grd, blk, 0, s, a);           \
     } while (0)
         if (fv && exact) CHMY_F3(true, true);
         else if (fv) CHMY_F3(true, false);
@@ -690,12 +696,15 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         dim3 blk(kFX, kW2, 1);
         dim3 grd((unsigned)((a.ntx + kW2 - 1) / kW2), (unsigned)a.nchunks, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
+        const bool fonly = a.has_f && !a.write_res && !a.has_rg;
 #define CHMY_F2(FV, EX)                                                                                   \
     do {                                                                                                  \
-        if (bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true, std::is_same<T, float>::value>), grd, \
-                                   blk, 0, s, a);                                                         \
-        else if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, true, false>), grd, blk, 0, s, a);    \
-        else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, false, false>), grd, blk, 0, s, a);            \
+        constexpr bool F32 = std::is_same<T, float>::value;                                               \
+        if (fonly && bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a);       \
+        else if (fonly) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a);      \
+        else if (bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 7, F32>), grd, blk, 0, s, a);          \
+        else if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 7, false>), grd, blk, 0, s, a);       \
+        else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a);                 \
     } while (0)
         if (fv && exact) CHMY_F2(true, true);
         else if (fv) CHMY_F2(true, false);
