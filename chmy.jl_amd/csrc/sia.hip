@@ -254,7 +254,13 @@ __global__ void __launch_bounds__(256) sia_adj5(SiaF<T> s, SiaW<T> w, DF<T> lo, 
 // so the results are bitwise equal to them.  With blockIdx.z = 1 a launch of
 // the adjoint also advances an independent forward step (the recompute of
 // the previous checkpoint segment), which fills the GPU at small grids.
-constexpr int kSX = 32, kSY = 16, kST = 256;
+#ifndef CHMY_SIA_TX
+#define CHMY_SIA_TX 32
+#endif
+#ifndef CHMY_SIA_TY
+#define CHMY_SIA_TY 8
+#endif
+constexpr int kSX = CHMY_SIA_TX, kSY = CHMY_SIA_TY, kST = 256;
 
 // S = B + H staged in LDS on the tile + R ring (zero outside cells 0..n+1,
 // which no evaluation reads); the formulas are those of SiaF.
