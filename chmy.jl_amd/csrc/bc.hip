@@ -3,7 +3,24 @@
 // sides: side-1 and side-2 writes and reads never alias for fields with at
 // least one cell, and each thread only touches its own normal line, so the
 // result equals the reference's side-1-then-side-2 launches.  Dims run N..1.
+//
+// A whole batch set also runs as ONE launch (bc_set_kernel) that writes each
+// point's final value directly.  Dims run N..1, so the last write to a point
+// p is made by the lowest dim D that writes it (ghost node 0 / m+1, or node
+// 1 / m for Dirichlet on a vertex location).  That write read the inner node
+// p' (p moved to 1 / m along D) as it stood after dims > D, which is again
+// the write of the lowest dim > D writing p', or the untouched input value.
+// Evaluating that chain (at most N steps, each the reference's own fma) per
+// point gives the same bits as the N launches.  Each point is written by the
+// thread of its lowest writing dim only, and the chain's final read is of a
+// point that no dim writes, so no thread reads what another writes.
+#include <algorithm>
+
 #include "common.hpp"
+
+#ifndef CHMY_BC_SET_LAUNCH
+#define CHMY_BC_SET_LAUNCH 1   // 0: one launch per dim
+#endif
 
 namespace chmy {
 
@@ -146,6 +163,202 @@ static int launch_dim(const chmy_grid_t* g, int D, const int* K, const chmy_fiel
     return 0;
 }
 
+// ------------------------------------------------------------ single launch
+constexpr int kSetFields = 8;   // distinct fields in a one-launch batch set
+
+struct BCRule {       // the condition of one (dim, side) of a field
+    int32_t kind;     // -1: none; else CHMY_DIRICHLET / CHMY_NEUMANN
+    int32_t vkind;
+    double value;
+    const void* vp;   // value field logical origin
+    int64_t vst[2];
+};
+
+struct BCSetField {
+    void* p;          // logical (0,0,0)
+    int64_t st[3];
+    int64_t m[3];     // size per dim
+    int32_t vertex[3];
+    int32_t pad_;
+    BCRule r[3][2];
+};
+
+struct BCSetLaunch {
+    int32_t nd, nf;
+    int64_t tn[3];    // n_d + 3
+    double h[3];
+    BCSetField f[kSetFields];
+};
+
+// node written by rule (j, side) of a field, or -1
+__device__ __forceinline__ int64_t bc_node(const BCSetField& F, int j, int side) {
+    const BCRule& r = F.r[j][side];
+    if (r.kind < 0) return -1;
+    const bool dv = r.kind == CHMY_DIRICHLET && F.vertex[j];
+    return side == 0 ? (dv ? 1 : 0) : (dv ? F.m[j] : F.m[j] + 1);
+}
+
+// side (0/1) of the rule of dim j that writes node c, or -1
+__device__ __forceinline__ int bc_writer(const BCSetField& F, int j, int64_t c) {
+    if (c == bc_node(F, j, 0)) return 0;
+    if (c == bc_node(F, j, 1)) return 1;
+    return -1;
+}
+
+template <class T>
+__device__ __forceinline__ T bc_value(const BCRule& r, int j, const int64_t* I) {
+    if (r.vkind == CHMY_VALUE_SCALAR) return T(r.value);
+    if (r.vkind == CHMY_VALUE_FIELD) {
+        int q = 0;
+        int64_t vo = 0;
+        for (int d = 0; d < 3; ++d)
+            if (d != j && q < 2) { vo += I[d] * r.vst[q]; ++q; }
+        return static_cast<const T*>(r.vp)[vo];
+    }
+    return T(0);
+}
+
+// blockIdx.y = (field, dim, side); x threads over the dim's tangential plane
+template <class T>
+__global__ void __launch_bounds__(256) bc_set_kernel(BCSetLaunch L) {
+    const int item = blockIdx.y;
+    const int fi = item / (2 * L.nd), D = (item / 2) % L.nd, sd = item % 2;
+    const BCSetField& F = L.f[fi];
+    const int64_t pD = bc_node(F, D, sd);
+    if (pD < 0) return;
+    int64_t tot = 1;
+    for (int d = 0; d < L.nd; ++d)
+        if (d != D) tot *= L.tn[d];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    int64_t I[3] = {0, 0, 0};
+    {
+        int64_t u = t;
+        for (int d = 0; d < L.nd; ++d)
+            if (d != D) { I[d] = u % L.tn[d]; u /= L.tn[d]; }
+    }
+    I[D] = pD;
+    for (int j = 0; j < D; ++j)
+        if (bc_writer(F, j, I[j]) >= 0) return;      // a lower dim writes this point last
+    const int64_t pofs = I[0] * F.st[0] + I[1] * F.st[1] + I[2] * F.st[2];
+    // the chain: writes of dims D = j0 < j1 < ... applied innermost first
+    int nop = 0;
+    int oj[3], os[3];
+    T ov[3];
+    T v;
+    int j = D, side = sd;
+    int64_t c[3] = {I[0], I[1], I[2]};
+    for (;;) {
+        const BCRule& r = F.r[j][side];
+        const T val = bc_value<T>(r, j, c);
+        if (r.kind == CHMY_DIRICHLET && F.vertex[j]) {
+            v = val;
+            break;
+        }
+        oj[nop] = j;
+        os[nop] = side;
+        ov[nop] = val;
+        ++nop;
+        c[j] = side == 0 ? 1 : F.m[j];
+        int nj = -1, ns = -1;
+        for (int k = j + 1; k < L.nd; ++k) {
+            const int w = bc_writer(F, k, c[k]);
+            if (w >= 0) { nj = k; ns = w; break; }
+        }
+        if (nj < 0) {
+            v = static_cast<const T*>(F.p)[c[0] * F.st[0] + c[1] * F.st[1] + c[2] * F.st[2]];
+            break;
+        }
+        j = nj;
+        side = ns;
+    }
+    for (int q = nop - 1; q >= 0; --q) {
+        const BCRule& r = F.r[oj[q]][os[q]];
+        if (r.kind == CHMY_DIRICHLET) v = fma(T(2), ov[q] - v, v);
+        else v = fma(T(L.h[oj[q]]), os[q] == 0 ? -ov[q] : ov[q], v);
+    }
+    static_cast<T*>(F.p)[pofs] = v;
+}
+
+static bool same_field(const chmy_field_t& a, const chmy_field_t& b) {
+    if (a.nd != b.nd || a.dtype != b.dtype) return false;
+    for (int d = 0; d < 3; ++d)
+        if (a.n[d] != b.n[d] || a.halo[d] != b.halo[d] || a.loc[d] != b.loc[d]) return false;
+    return true;
+}
+
+// Packs the batch set for bc_set_kernel; returns 1 when it qualifies (at most
+// kSetFields distinct fields, each field named once per (dim, side), at least
+// two nodes along every dim it has a condition on, value fields distinct from
+// the batch's fields), 0 when the per-dim launches must run, <0 on error.
+static int pack_set(const chmy_grid_t* g, const int32_t* counts, const chmy_field_t* fields, const chmy_bc_t* bcs,
+                    BCSetLaunch* L, int* dtype) {
+    std::memset(L, 0, sizeof *L);
+    L->nd = g->nd;
+    for (int d = 0; d < 3; ++d) {
+        L->tn[d] = d < g->nd ? g->n[d] + 3 : 1;
+        L->h[d] = d < g->nd ? g->spacing[d] : 0.0;
+    }
+    const chmy_field_t* uniq[kSetFields];
+    int q = 0;
+    *dtype = -1;
+    for (int D = 0; D < g->nd; ++D)
+        for (int sd = 0; sd < 2; ++sd)
+            for (int k = 0; k < counts[2 * D + sd]; ++k, ++q) {
+                const chmy_field_t& f = fields[q];
+                if (*dtype < 0) *dtype = f.dtype;
+                if (f.dtype != CHMY_F64 && f.dtype != CHMY_F32) return 0;
+                BCItem it;
+                CHMY_TRY(fill_item(g, D, &f, &bcs[q], *dtype, &it));
+                int fi = 0;
+                while (fi < L->nf && uniq[fi]->data != f.data) ++fi;
+                if (fi == L->nf) {
+                    if (L->nf == kSetFields) return 0;
+                    uniq[fi] = &f;
+                    BCSetField& F = L->f[fi];
+                    F.p = it.p;
+                    for (int d = 0; d < 3; ++d) {
+                        F.st[d] = it.st[d];
+                        F.m[d] = d < f.nd ? f.n[d] : 1;
+                        F.vertex[d] = d < f.nd ? f.loc[d] : 0;
+                        F.r[d][0].kind = F.r[d][1].kind = -1;
+                    }
+                    ++L->nf;
+                } else if (!same_field(*uniq[fi], f)) {
+                    return 0;
+                }
+                BCSetField& F = L->f[fi];
+                BCRule& r = F.r[D][sd];
+                if (r.kind >= 0 || F.m[D] < 2) return 0;
+                r.kind = it.kind;
+                r.vkind = it.vkind;
+                r.value = it.value;
+                r.vp = it.vp;
+                r.vst[0] = it.vst[0];
+                r.vst[1] = it.vst[1];
+            }
+    for (int a = 0; a < q; ++a)
+        if (bcs[a].vkind == CHMY_VALUE_FIELD)
+            for (int fi = 0; fi < L->nf; ++fi)
+                if (bcs[a].vfield->data == uniq[fi]->data) return 0;
+    return L->nf > 0 ? 1 : 0;
+}
+
+static int launch_set(const BCSetLaunch& L, int dtype, hipStream_t s) {
+    int64_t tot = 1;
+    for (int d = 0; d < L.nd; ++d) {
+        int64_t p = 1;
+        for (int e = 0; e < L.nd; ++e)
+            if (e != d) p *= L.tn[e];
+        tot = std::max(tot, p);
+    }
+    dim3 blk(256), grd((unsigned)((tot + 255) / 256), (unsigned)(L.nf * 2 * L.nd));
+    if (dtype == CHMY_F64) hipLaunchKernelGGL(bc_set_kernel<double>, grd, blk, 0, s, L);
+    else hipLaunchKernelGGL(bc_set_kernel<float>, grd, blk, 0, s, L);
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace chmy
 
 using namespace chmy;
@@ -174,6 +387,14 @@ int chmy_bc_batchset(const chmy_grid_t* g, const int32_t* counts, const chmy_fie
         offs[q + 1] = offs[q] + counts[q];
     }
     if (offs[2 * g->nd] > 0 && (!fields || !bcs)) return set_error(CHMY_E_ARG, "fields/bcs NULL");
+    if (offs[2 * g->nd] == 0) return 0;
+    if (CHMY_BC_SET_LAUNCH) {
+        BCSetLaunch L;
+        int dtype;
+        const int ok = pack_set(g, counts, fields, bcs, &L, &dtype);
+        if (ok < 0) return ok;
+        if (ok) return launch_set(L, dtype, (hipStream_t)stream);
+    }
     for (int D = g->nd - 1; D >= 0; --D) {
         int Ks[2] = {counts[2 * D], counts[2 * D + 1]};
         const chmy_field_t* f[2] = {fields + offs[2 * D], fields + offs[2 * D + 1]};

@@ -434,8 +434,68 @@ __device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, kFX); }     // v
 template <class T>
 __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, kFX); }   // value of lane l+1
 
-constexpr int kW2 = 4;   // waves (x-tiles) per block
+#ifndef CHMY_FUSED_W2
+#define CHMY_FUSED_W2 4
+#endif
+constexpr int kW2 = CHMY_FUSED_W2;   // waves (x-tiles) per block
+#ifndef CHMY_FUSED_YCHUNK2
+#define CHMY_FUSED_YCHUNK2 8
+#endif
+constexpr int kYChunk2 = CHMY_FUSED_YCHUNK2;   // rows per wave (the march is unrolled over them)
 
+// Raw load of a forcing value (kept as bf16 bits until use, so the load can
+// stay in flight across a row of the march) and its conversion.
+template <class T, bool BF>
+using Fraw = typename std::conditional<BF, uint16_t, T>::type;
+template <class T, bool BF>
+__device__ __forceinline__ Fraw<T, BF> ldr(const T* p, uint32_t boff) {
+    if constexpr (BF) return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(p) + (boff >> 1));
+    else return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + boff);
+}
+template <class T, bool BF>
+__device__ __forceinline__ T cvf(Fraw<T, BF> v) {
+    if constexpr (BF) return T(bf16_to_f32(v));
+    else return v;
+}
+
+// What one row j of the y-march reads: Center values at row j, the y-face,
+// vertex and η values at row j+1, and Vx at row j+1 (Vx at row j is carried).
+template <class T, bool BF>
+struct Row2 {
+    T vx1, vy1, P, txx, tyy, txy1, eta1, eml, rg;
+    Fraw<T, BF> oxx, oyy, oxy, fP, fVx, fVy;
+};
+
+template <class T, bool FVISC, int OPT, bool BF>
+__device__ __forceinline__ void load_row2(const Fused2<T>& a, const uint32_t* q, const uint32_t* dy, bool has_f,
+                                          bool has_rg, bool outs, Row2<T, BF>& r) {
+    r.vx1 = ldo(a.Vx, q[L2X] + dy[L2X]);
+    r.vy1 = ldo(a.Vy, q[L2Y] + dy[L2Y]);
+    r.P = ldo(a.Pr, q[L2C]);
+    r.txx = ldo(a.txx, q[L2C]);
+    r.tyy = ldo(a.tyy, q[L2C]);
+    r.txy1 = ldo(a.txy, q[L2XY] + dy[L2XY]);
+    r.oxx = ldr<T, BF>(a.Oxx, q[L2C]);
+    r.oyy = ldr<T, BF>(a.Oyy, q[L2C]);
+    r.oxy = ldr<T, BF>(a.Oxy, q[L2XY] + dy[L2XY]);
+    if constexpr (FVISC) {
+        r.eta1 = ldo(a.eta, q[L2C] + dy[L2C]);
+        r.eml = ldo(a.eml, q[L2C]);
+    }
+    if (has_f) {
+        r.fP = ldr<T, BF>(a.fP, q[L2C]);
+        if (outs) {
+            r.fVx = ldr<T, BF>(a.fVx, q[L2X]);
+            r.fVy = ldr<T, BF>(a.fVy, q[L2Y]);
+        }
+    }
+    if (has_rg && outs) r.rg = ldo(a.rg, q[L2Y]);
+}
+
+// One wave owns 64 x columns (62 outputs) and marches y over a chunk of
+// rows; x neighbours come from shuffles, y neighbours from values carried in
+// registers.  The loads of row j+1 are issued before row j is computed, so a
+// wave always has a row of loads in flight.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
 __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
@@ -447,8 +507,8 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     if (xt >= a.ntx) return;                        // whole wave leaves together
     const int i = xt * (kFX - 2) - 1 + lane;
     const int nb0 = a.n[0] + 1, nb1 = a.n[1] + 1;
-    const int jb = blockIdx.y * a.ychunk;
-    const int je = min(jb + a.ychunk, nb1 + 1);
+    const int jb = blockIdx.y * kYChunk2;
+    const int je = min(jb + kYChunk2, nb1 + 1);
     const bool inx = i >= 0 && i <= nb0;
     const bool ownx = lane >= 1 && lane <= kFX - 2 && inx;
     const int ic = min(i, a.n[0] + 2);              // stored column
@@ -464,16 +524,27 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     T vx_j = ldo(a.Vx, o[L2X]), vy_j = ldo(a.Vy, o[L2Y]);
     T eta_j = FVISC ? ldo(a.eta, o[L2C]) : T(0);
     T P_jm1 = T(0), tyy_jm1 = T(0), txy_j = T(0), eml_jm1 = T(0);
-    for (int j = jb - 1; j < je; ++j) {
+    // Row j: issue the loads of row j+1 into `nxt`, then compute from `cur`.
+    // The march is unrolled over a full chunk; rows j >= je (the last chunk)
+    // are computed and dropped.  The offsets stop at row je - 1, so every load
+    // stays inside the chunk's rows (cached reloads, never used).
+    auto row = [&](int j, const Row2<T, BF>& cur, Row2<T, BF>& nxt) {
         const bool pro = j < jb;
+        const bool live = j < je;
         const bool in_j = inx && j >= 0;
         const bool in_j1 = inx && j + 1 <= nb1;
-        const T vx1 = ldo(a.Vx, o[L2X] + dy[L2X]), vy1 = ldo(a.Vy, o[L2Y] + dy[L2Y]);
-        T P = ldo(a.Pr, o[L2C]), txx = ldo(a.txx, o[L2C]), tyy = ldo(a.tyy, o[L2C]);
-        T txy1 = ldo(a.txy, o[L2XY] + dy[L2XY]);
-        const T oxx = ldf<T, BF>(a.Oxx, o[L2C]), oyy = ldf<T, BF>(a.Oyy, o[L2C]), oxy = ldf<T, BF>(a.Oxy, o[L2XY] + dy[L2XY]);
-        const T eta1 = FVISC ? ldo(a.eta, o[L2C] + dy[L2C]) : T(0);
-        const T eml = FVISC ? ldo(a.eml, o[L2C]) : T(0);
+        const bool more = j + 1 < je;
+        {
+            uint32_t q[NLOC2];
+#pragma unroll
+            for (int c = 0; c < NLOC2; ++c) q[c] = o[c] + (more ? dy[c] : 0u);
+            load_row2<T, FVISC, OPT, BF>(a, q, dy, has_f, has_rg, true, nxt);
+        }
+        __builtin_amdgcn_sched_barrier(0);   // keep the next row's loads ahead of this row's work
+        const T vx1 = cur.vx1, vy1 = cur.vy1;
+        T P = cur.P, txx = cur.txx, tyy = cur.tyy, txy1 = cur.txy1;
+        const T eta1 = FVISC ? cur.eta1 : T(0);
+        const T eml = FVISC ? cur.eml : T(0);
         // Center stresses and Pr at row j
         const T exx = (shdn(vx_j) - vx_j) * ix;
         const T eyy = (vy1 - vy_j) * iy;
@@ -482,7 +553,8 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             typename AR::Loc Lc;
             if constexpr (FVISC) Lc = AR::field(eta_j, K);
             else Lc = AR::scalar(K);
-            const T Pn = AR::pr(P, has_f ? dv - ldf<T, BF>(a.fP, o[L2C]) : dv, Lc, K);
+            const T oxx = cvf<T, BF>(cur.oxx), oyy = cvf<T, BF>(cur.oyy);
+            const T Pn = AR::pr(P, has_f ? dv - cvf<T, BF>(cur.fP) : dv, Lc, K);
             const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
             const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
             P = in_j ? Pn : P;
@@ -497,11 +569,11 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             typename AR::Loc Le;
             if constexpr (FVISC) Le = AR::field(lerp4(etal, eta_j, eta1l, eta1), K);
             else Le = AR::scalar(K);
-            const T n = AR::off(txy1, oxy, exy, Le, K);
+            const T n = AR::off(txy1, cvf<T, BF>(cur.oxy), exy, Le, K);
             txy1 = in_j1 ? n : txy1;
         }
         const T Pl = shup(P), txxl = shup(txx), txyr = shdn(txy_j), emll = FVISC ? shup(eml) : T(0);
-        if (!pro && ownx && in_j) {
+        if (live && !pro && ownx && in_j) {
             sto(a.oPr, o[L2C], P);
             sto(a.otxx, o[L2C], txx);
             sto(a.otyy, o[L2C], tyy);
@@ -509,10 +581,10 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
             if (write_res) sto(a.dV, o[L2C], dv);
             T rx = (-((P - Pl) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy;
             T ry = (-((P - P_jm1) * iy) + (tyy - tyy_jm1) * iy) + (txyr - txy_j) * ix;
-            if (has_rg) ry = ry - ldo(a.rg, o[L2Y]);
+            if (has_rg) ry = ry - cur.rg;
             if (has_f) {
-                rx = rx + ldf<T, BF>(a.fVx, o[L2X]);
-                ry = ry + ldf<T, BF>(a.fVy, o[L2Y]);
+                rx = rx + cvf<T, BF>(cur.fVx);
+                ry = ry + cvf<T, BF>(cur.fVy);
             }
             T ex, ey;
             if constexpr (FVISC) {
@@ -536,14 +608,20 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
         eta_j = eta1;
         eml_jm1 = eml;
 #pragma unroll
-        for (int c = 0; c < NLOC2; ++c) o[c] += dy[c];
-    }
+        for (int c = 0; c < NLOC2; ++c) o[c] += more ? dy[c] : 0u;
+    };
+    // Two register sets used in turn, fully unrolled: no row's values are
+    // copied across a loop edge (a copy would wait for the loads in flight).
+    Row2<T, BF> rr[2];
+    load_row2<T, FVISC, OPT, BF>(a, o, dy, has_f, has_rg, false, rr[0]);   // row jb - 1 writes nothing
+#pragma unroll
+    for (int t = 0; t <= kYChunk2; ++t) row(jb - 1 + t, rr[t & 1], rr[(t + 1) & 1]);
 }
 
 #ifndef CHMY_FUSED_ZCHUNK
 #define CHMY_FUSED_ZCHUNK 16
 #endif
-constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = CHMY_FUSED_ZCHUNK, kYChunk2 = 16;
+constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = CHMY_FUSED_ZCHUNK;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {

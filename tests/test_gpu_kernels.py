@@ -135,6 +135,43 @@ def _ospec(spec, nd, vmap):
 
 @pytest.mark.parametrize("T", TYPES)
 @pytest.mark.parametrize("nd", [1, 2, 3])
+@pytest.mark.parametrize("nfields", [4, 9])
+def test_bc_batchset_equals_per_side_launches(arch, T, nd, nfields):
+    """chmy_bc_batchset (one launch for the whole set when at most 8 distinct
+    fields, else one per dim) equals the reference's sequence of bc! calls,
+    dims N..1, side 1 then side 2 (batch.jl:20-29), bitwise, for a batch that
+    mixes locations and conditions per field."""
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(70 + nd + nfields)
+    names = "xyz"[:nd]
+    D, N = chmy.Dirichlet, chmy.Neumann
+    conds = [D(), N(), D(1.25), N(-0.75)]
+    pairs_a, pairs_b = [], []
+    for q in range(nfields):
+        loc = tuple(Cc() if (q >> d) & 1 == 0 else Vv() for d in range(nd))
+        spec = {}
+        for d, k in enumerate(names):
+            c1, c2 = conds[(q + d) % 4], conds[(q + 2 * d + 1) % 4]
+            spec[k] = (c1, c2) if (q + d) % 3 else c1
+        fa, _ = pair(arch, g, og, loc, T, rng=rng)
+        fb = chmy.Field(arch, g, loc, fa.data.dtype)
+        fb.data.copy_(fa.data)
+        pairs_a.append((fa, spec))
+        pairs_b.append((fb, spec))
+    ba = chmy.batch(g, *pairs_a)
+    bb = chmy.batch(g, *pairs_b)
+    ba.apply()
+    for d in reversed(range(nd)):
+        for s in (1, 2):
+            bb.apply_side(d, s)
+    bits = np.int64 if T == np.float64 else np.int32
+    for q, ((fa, spec), (fb, _)) in enumerate(zip(pairs_a, pairs_b)):
+        assert np.array_equal(dev(fa).view(bits), dev(fb).view(bits)), f"field {q} spec {spec}"
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
 def test_bc_batches(arch, T, nd):
     g = grids(nd)
     og = ogrid(g)
