@@ -112,7 +112,9 @@ int chmy_bc_side(const chmy_grid_t* grid, int dim, int side, int K,
                  const chmy_field_t* fields, const chmy_bc_t* bcs, void* stream);
 /* bc!(arch, grid, batch::BatchSet) (batch.jl:20-29): dims N..1, side 1 then
  * side 2.  counts[2*dim+side-1] fields per batch, fields/bcs concatenated in
- * (dim, side) order.  One launch per dim (both sides are independent). */
+ * (dim, side) order.  One launch for the whole set (each point's final value
+ * evaluated directly, bitwise equal to the per-dim sequence) when it holds at
+ * most 8 distinct fields; otherwise one launch per dim. */
 int chmy_bc_batchset(const chmy_grid_t* grid, const int32_t* counts,
                      const chmy_field_t* fields, const chmy_bc_t* bcs, void* stream);
 
