@@ -160,10 +160,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     return (bid % 8) * per + bid / 8;
 }
 
-// Tuning knobs fixed at build time: rows per block and the minimum waves per
-// SIMD the register allocation must allow (0 = no bound).
-#ifndef CHMY_FUSED_BY
-#define CHMY_FUSED_BY 8
+// Tuning knobs fixed at build time: rows per block (FP64 / FP32 sweeps) and
+// the minimum waves per SIMD the register allocation must allow (0 = no bound).
+#ifndef CHMY_FUSED_BY64
+#define CHMY_FUSED_BY64 12
+#endif
+#ifndef CHMY_FUSED_BY32
+#define CHMY_FUSED_BY32 8
 #endif
 #ifndef CHMY_FUSED_WPE
 #define CHMY_FUSED_WPE 0
@@ -618,10 +621,16 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     for (int t = 0; t <= kYChunk2; ++t) row(jb - 1 + t, rr[t & 1], rr[(t + 1) & 1]);
 }
 
-#ifndef CHMY_FUSED_ZCHUNK
-#define CHMY_FUSED_ZCHUNK 16
+#ifndef CHMY_FUSED_ZCHUNK64
+#define CHMY_FUSED_ZCHUNK64 16
 #endif
-constexpr int kBY3 = CHMY_FUSED_BY, kZChunk = CHMY_FUSED_ZCHUNK;
+#ifndef CHMY_FUSED_ZCHUNK32
+#define CHMY_FUSED_ZCHUNK32 16
+#endif
+template <class T>
+constexpr int kZChunk = std::is_same<T, double>::value ? CHMY_FUSED_ZCHUNK64 : CHMY_FUSED_ZCHUNK32;
+template <class T>
+constexpr int kBY3 = std::is_same<T, double>::value ? CHMY_FUSED_BY64 : CHMY_FUSED_BY32;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
@@ -709,10 +718,10 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.L[LXZ] = loc_stride(tau[4]);
         a.L[LYZ] = loc_stride(tau[5]);
         for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
-        a.kchunk = kZChunk;
+        a.kchunk = kZChunk<T>;
         a.ntx = (int32_t)((g->n[0] + 2 + kFX - 3) / (kFX - 2));
-        a.nty = (int32_t)((g->n[1] + 2 + kBY3 - 2) / (kBY3 - 1));
-        a.nchunks = (int32_t)((g->n[2] + 2 + kZChunk - 1) / kZChunk);
+        a.nty = (int32_t)((g->n[1] + 2 + kBY3<T> - 2) / (kBY3<T> - 1));
+        a.nchunks = (int32_t)((g->n[2] + 2 + kZChunk<T> - 1) / kZChunk<T>);
         a.has_rg = rg != nullptr;
         a.write_res = wres;
         a.ix = T(g->inv_spacing[0]);
@@ -720,17 +729,17 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.iz = T(g->inv_spacing[2]);
         a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
-        dim3 blk(kFX, kBY3, 1), grd(nb, 1, 1);
+        dim3 blk(kFX, kBY3<T>, 1), grd(nb, 1, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
         const bool fonly = a.has_f && !a.write_res && !a.has_rg;
 #define CHMY_F3(FV, EX)                                                                                   \
     do {                                                                                                  \
         constexpr bool F32 = std::is_same<T, float>::value;                                               \
-        if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 1, F32>), grd, blk, 0, s, a); \
-        else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 1, false>), grd, blk, 0, s, a); \
-        else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 7, F32>), grd, blk, 0, s, a);    \
-        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 7, false>), grd, blk, 0, s, a); \
-        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3, 0, false>), grd, blk, 0, s, a);           \
+        if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 1, F32>), grd, blk, 0, s, a); \
+        else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 1, false>), grd, blk, 0, s, a); \
+        else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 7, F32>), grd, blk, 0, s, a);    \
+        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 7, false>), grd, blk, 0, s, a); \
+        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 0, false>), grd, blk, 0, s, a);           \
     } while (0)
         if (fv && exact) CHMY_F3(true, true);
         else if (fv) CHMY_F3(true, false);
