@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
 #define CHMY_FUSED_ZCHUNK64 16
 #endif
 #ifndef CHMY_FUSED_ZCHUNK32
-#define CHMY_FUSED_ZCHUNK32 16
+#define CHMY_FUSED_ZCHUNK32 12
 #endif
 template <class T>
 constexpr int kZChunk = std::is_same<T, double>::value ? CHMY_FUSED_ZCHUNK64 : CHMY_FUSED_ZCHUNK32;
