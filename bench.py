@@ -186,6 +186,71 @@ def time_to_converge(args):
     print(json.dumps(out))
 
 
+def algorithmic_bytes_2d(n, itemsize=8):
+    """Updated {Vx, Vy, Pr, tau_xx, tau_yy, tau_xy} x2 + read {tau_old x3, eta}
+    over the exact interior counts of the 2D staggered fields."""
+    C, Vf, E = n * n, (n + 1) * n, (n + 1) * (n + 1)
+    updated = 2 * Vf + 3 * C + E
+    read = 2 * C + E + C
+    return (2 * updated + read) * itemsize
+
+
+def stokes2d_rates(args):
+    """The 2D fused PT iteration at n^2 (inclusions setup): FP64 and FP32
+    kernel times (HIP events on the launch stream, median of `steps`), and the
+    wall time per sweep of the FP32 + FP64 defect-correction solver (200-sweep
+    cycles with their FP64 residual refresh)."""
+    import torch
+    import chmy
+    from chmy import configs
+    from chmy.stokes import MixedPTStokes, PTStokes, pure_shear_initial, pure_shear_values
+    arch = chmy.Arch(chmy.MI355XBackend())
+    n = args.n
+    g, eta, dt = configs.stokes2d_inclusions(arch, n)
+    out = {"metric": "2D fused PT iteration rate", "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": f"2D VE Stokes PT, {n}^2 cells, 32 inclusions", "grid": [n, n]}}
+    for dt_, name, isz in ((torch.float64, "f64", 8), (torch.float32, "f32", 4)):
+        S = PTStokes(arch, g, dtype=dt_, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(2))
+        pure_shear_initial(g, S.V)
+        S.apply_bc()
+        S.sync_buffers()
+        for _ in range(args.warmup):
+            S.iteration()
+        torch.cuda.synchronize()
+        K = args.steps
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+        t0 = time.perf_counter()
+        for q in range(K):
+            ev[q][0].record()
+            S._iter_call(False)
+            ev[q][1].record()
+            S.apply_bc(S.V2)
+            S._swap()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / K
+        k_ms = sorted(a.elapsed_time(b) for a, b in ev)[K // 2]
+        A = algorithmic_bytes_2d(n, isz)
+        out[name] = {"kernel_ms_median": round(k_ms, 5), "kernel_GBs": round(A / k_ms / 1e6, 1),
+                     "step_ms": round(wall * 1e3, 5), "algorithmic_bytes": A}
+        del S
+    M = MixedPTStokes(arch, g, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(2), inner=200)
+    pure_shear_initial(g, M.x.V)
+    M.x.apply_bc()
+    M.refresh()
+    M.cycle()
+    M.refresh()
+    torch.cuda.synchronize()
+    cycles = max(1, args.steps // 200)
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        M.cycle()
+        M.refresh()
+    torch.cuda.synchronize()
+    out["mixed_sweep_ms"] = round((time.perf_counter() - t0) / (200 * cycles) * 1e3, 5)
+    out["value"] = out["f64"]["kernel_GBs"]
+    print(json.dumps(out))
+
+
 def sia_gradient(args):
     """SIA forward run versus gradient (checkpointed forward + reverse sweep)
     wall time at 512² (300 steps, 30 checkpoints, fixed Δt), and the gradient
@@ -380,6 +445,8 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=8)
     ap.add_argument("--fd-h", type=float, default=1e-4)
     ap.add_argument("--fd-eps", type=float, default=1e-11)
+    ap.add_argument("--stokes2d", action="store_true",
+                    help="2D fused PT iteration rates at --n (default 2047): FP64, FP32, defect-correction sweep")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -395,6 +462,11 @@ def main():
         return
     if args.stokes_adjoint:
         stokes_gradient(args)
+        return
+    if args.stokes2d:
+        if args.n == 255:
+            args.n = 2047
+        stokes2d_rates(args)
         return
     if args.sia:
         if args.n == 255:
