@@ -162,6 +162,12 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 
 // Tuning knobs fixed at build time: rows per block (FP64 / FP32 sweeps) and
 // the minimum waves per SIMD the register allocation must allow (0 = no bound).
+#ifndef CHMY_FUSED_TW64
+#define CHMY_FUSED_TW64 64
+#endif
+#ifndef CHMY_FUSED_TW32
+#define CHMY_FUSED_TW32 64
+#endif
 #ifndef CHMY_FUSED_BY64
 #define CHMY_FUSED_BY64 12
 #endif
@@ -182,23 +188,23 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // rows j-1, j and y-vertex edges at rows j, j+1, so rows ty = 1..BY-1 are
 // outputs and only row 0 is a ring.  The last wave also stages row j + 1 of
 // V and eta (LDS row BY).
-template <class T, bool FVISC, bool EXACT, int BY, int OPT, bool BF>
-__global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
+template <class T, bool FVISC, bool EXACT, int TW, int BY, int OPT, bool BF>
+__global__ void __launch_bounds__(TW * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
     const bool has_f = OPT == 1 || (OPT == 7 && a.has_f);
     const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 7 && a.has_rg;
     constexpr int NR = BY + 1;   // LDS rows of the V and eta planes
-    __shared__ T s_vx[NR][kFX], s_vy[2][NR][kFX], s_vz1[NR][kFX];
-    __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? NR : 1][kFX], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][kFX];
-    __shared__ T s_P[BY][kFX], s_txx[BY][kFX], s_tyy[BY][kFX], s_txy[BY][kFX];
-    __shared__ T s_txz[2][BY][kFX], s_tyz[2][BY][kFX];
+    __shared__ T s_vx[NR][TW], s_vy[2][NR][TW], s_vz1[NR][TW];
+    __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][TW];
+    __shared__ T s_P[BY][TW], s_txx[BY][TW], s_tyy[BY][TW], s_txy[BY][TW];
+    __shared__ T s_txz[2][BY][TW], s_tyz[2][BY][TW];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int nb = a.ntx * a.nty * a.nchunks;
     const int lt = xcd_remap(blockIdx.x, nb);
     const int bx = lt % a.ntx, by = (lt / a.ntx) % a.nty, bz = lt / (a.ntx * a.nty);
-    const int i = bx * (kFX - 2) - 1 + tx;
+    const int i = bx * (TW - 2) - 1 + tx;
     const int j = by * (BY - 1) - 1 + ty;
     const int nb0 = a.n[0] + 1, nb1 = a.n[1] + 1, nb2 = a.n[2] + 1;   // box is [0, n+1]
     const int kb = bz * a.kchunk;
@@ -208,7 +214,7 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
     const bool inx = i >= 0 && i <= nb0;
     const bool inc = inx && j >= 0 && j <= nb1;            // Center / tau_xz column in the box
     const bool ine = inx && j + 1 <= nb1;                  // y-vertex edge row j+1 in the box
-    const bool ownx = tx >= 1 && tx <= kFX - 2;
+    const bool ownx = tx >= 1 && tx <= TW - 2;
     const bool ownc = ownx && ty >= 1 && inc;              // outputs: Center, tau_xz, V at row j
     const bool owne = ownx && ty <= BY - 2 && ine;         // outputs: tau_xy, tau_yz at row j+1
     const bool last = ty == BY - 1;                        // stages row j+1 (wave-uniform)
@@ -291,7 +297,7 @@ __global__ void __launch_bounds__(kFX * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused
         __syncthreads();
 
         // ---------------- B: stresses (computed on every lane, kept in the box)
-        const int tx1 = tx < kFX - 1 ? tx + 1 : tx, txm = tx > 0 ? tx - 1 : tx;
+        const int tx1 = tx < TW - 1 ? tx + 1 : tx, txm = tx > 0 ? tx - 1 : tx;
         const int tym = ty > 0 ? ty - 1 : ty;
         const T exx = (s_vx[ty][tx1] - vx_k) * ix;
         const T eyy = (s_vy[p][ty + 1][tx] - vy_k) * iy;
@@ -621,6 +627,276 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     for (int t = 0; t <= kYChunk2; ++t) row(jb - 1 + t, rr[t & 1], rr[(t + 1) & 1]);
 }
 
+// ------------------------------------------------------------------ 2D, two PT iterations per pass
+// The V batch a 2D free-slip solve applies after every iteration
+// (stokes_2d_inc_ve_T.jl's bc of update_velocity!): Vx Dirichlet on its x
+// nodes 1 / n+1 and Neumann in y, Vy Neumann in x and Dirichlet on its y
+// nodes 1 / n+1, scalar values [side 1, side 2]; h the spacings.
+struct FreeSlip2 {
+    double ax[2], qy[2], by[2], qx[2];
+    double hx, hy;
+};
+
+#ifndef CHMY_FUSED_TB
+#define CHMY_FUSED_TB 1   // 0: chmy_stokes_iterate runs single iterations only
+#endif
+#ifndef CHMY_FUSED_TB_CHUNK
+#define CHMY_FUSED_TB_CHUNK 16
+#endif
+#ifndef CHMY_FUSED_TB_W
+#define CHMY_FUSED_TB_W 4
+#endif
+constexpr int kTBW = CHMY_FUSED_TB_W;       // waves per block
+// output rows per wave; the march is unrolled over them, and the FP64 exact
+// form (31 divisions per point) takes shorter chunks to stay unrolled
+template <class T, bool EXACT>
+constexpr int kTBC = (EXACT && sizeof(T) == 8) ? 8 : CHMY_FUSED_TB_CHUNK;
+
+// What row a of the first iteration reads (Center values at row a, the rest
+// at row a + 1), as fused2d_kernel's row.
+template <class T, bool BF>
+struct In2 {
+    T vx1, vy1, P, txx, tyy, txy1, eta1, eml;
+    Fraw<T, BF> oxx, oyy, oxy, fP, fVx, fVy;
+};
+
+template <class T, bool FVISC, bool BF>
+__device__ __forceinline__ void load_in2(const Fused2<T>& a, const uint32_t* qc, const uint32_t* qn, bool has_f,
+                                         In2<T, BF>& r) {
+    r.vx1 = ldo(a.Vx, qn[L2X]);
+    r.vy1 = ldo(a.Vy, qn[L2Y]);
+    r.P = ldo(a.Pr, qc[L2C]);
+    r.txx = ldo(a.txx, qc[L2C]);
+    r.tyy = ldo(a.tyy, qc[L2C]);
+    r.txy1 = ldo(a.txy, qn[L2XY]);
+    r.oxx = ldr<T, BF>(a.Oxx, qc[L2C]);
+    r.oyy = ldr<T, BF>(a.Oyy, qc[L2C]);
+    r.oxy = ldr<T, BF>(a.Oxy, qn[L2XY]);
+    if constexpr (FVISC) {
+        r.eta1 = ldo(a.eta, qn[L2C]);
+        r.eml = ldo(a.eml, qc[L2C]);
+    }
+    if (has_f) {
+        r.fP = ldr<T, BF>(a.fP, qc[L2C]);
+        r.fVx = ldr<T, BF>(a.fVx, qc[L2X]);
+        r.fVy = ldr<T, BF>(a.fVy, qc[L2Y]);
+    }
+}
+
+// One iteration's state carried along the y-march, and its row step: the
+// expressions of fused2d_kernel's row, with V at row j computed on every lane
+// (the caller keeps or drops it).
+template <class T>
+struct Carry2 {
+    T vx_j, vy_j, eta_j, P_jm1, tyy_jm1, txy_j, eml_jm1;
+};
+
+template <class T, bool FVISC, bool EXACT>
+__device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool in_j1, bool has_f, Carry2<T>& c,
+                                     T vx1, T vy1, T& P, T& txx, T& tyy, T& txy1, T oxx, T oyy, T oxy, T eta1,
+                                     T eml, T fP, T fVx, T fVy, T& vxn, T& vyn, T& txy_prev) {
+    using AR = Arith<T, EXACT>;
+    const T exx = (shdn(c.vx_j) - c.vx_j) * ix;
+    const T eyy = (vy1 - c.vy_j) * iy;
+    const T dv = exx + eyy;
+    {
+        typename AR::Loc Lc;
+        if constexpr (FVISC) Lc = AR::field(c.eta_j, K);
+        else Lc = AR::scalar(K);
+        const T Pn = AR::pr(P, has_f ? dv - fP : dv, Lc, K);
+        const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
+        const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+        P = in_j ? Pn : P;
+        txx = in_j ? txxn : txx;
+        tyy = in_j ? tyyn : tyy;
+    }
+    const T vy1l = shup(vy1);
+    const T etal = FVISC ? shup(c.eta_j) : T(0), eta1l = FVISC ? shup(eta1) : T(0);
+    {
+        const T exy = T(0.5) * ((vx1 - c.vx_j) * iy + (vy1 - vy1l) * ix);
+        typename AR::Loc Le;
+        if constexpr (FVISC) Le = AR::field(lerp4(etal, c.eta_j, eta1l, eta1), K);
+        else Le = AR::scalar(K);
+        const T n = AR::off(txy1, oxy, exy, Le, K);
+        txy1 = in_j1 ? n : txy1;
+    }
+    const T Pl = shup(P), txxl = shup(txx), txyr = shdn(c.txy_j), emll = FVISC ? shup(eml) : T(0);
+    T rx = (-((P - Pl) * ix) + (txx - txxl) * ix) + (txy1 - c.txy_j) * iy;
+    T ry = (-((P - c.P_jm1) * iy) + (tyy - c.tyy_jm1) * iy) + (txyr - c.txy_j) * ix;
+    if (has_f) {
+        rx = rx + fVx;
+        ry = ry + fVy;
+    }
+    T ex, ey;
+    if constexpr (FVISC) {
+        ex = AR::vcoef_field(jmax(emll, eml), K);
+        ey = AR::vcoef_field(jmax(c.eml_jm1, eml), K);
+    } else {
+        ex = ey = AR::vcoef_scalar(K);
+    }
+    vxn = AR::vel(c.vx_j, rx, ex, K);
+    vyn = AR::vel(c.vy_j, ry, ey, K);
+    txy_prev = c.txy_j;
+    c.P_jm1 = P;
+    c.tyy_jm1 = tyy;
+    c.txy_j = txy1;
+    c.vx_j = vx1;
+    c.vy_j = vy1;
+    c.eta_j = eta1;
+    c.eml_jm1 = eml;
+}
+
+// Two PT iterations in one pass: the state is read once, the first iteration
+// (A) lives in registers, the free-slip V batch is applied to its velocities
+// in-kernel, and the second iteration (B) writes the output set.  B's row r
+// needs A's stresses at row r, A's tau_xy at row r+1 and A's batched V at rows
+// r, r+1 (row 0 of Vx copies row 1), so B trails A by two rows; A covers rows
+// jb-2 .. je+1 of a chunk, B rows jb-1 (its first row, nothing stored) .. je-1.
+// In x, A is valid on lanes 1..62 and B on lanes 2..61 (60 outputs per wave).
+// Rows and columns are clamped into the stored range [-1, n+2] for loads;
+// values at clamped positions reach no output.  The batched velocities follow
+// the batch's own order (dims y then x; the x condition writes last): see
+// bc.hip.  Results are bitwise those of two fused2d_kernel launches with the
+// batch between them.
+template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
+__global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, FreeSlip2 fs) {
+    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
+    constexpr bool HF = OPT == 1;
+    const Kc<T>& K = a.k;
+    const int lane = threadIdx.x;
+    const int xt = blockIdx.x * kTBW + threadIdx.y;
+    if (xt >= a.ntx) return;                        // whole wave leaves together
+    const int i = xt * (kFX - 4) - 2 + lane;
+    const int n0 = a.n[0], n1 = a.n[1];
+    const int nb0 = n0 + 1, nb1 = n1 + 1;
+    constexpr int C = kTBC<T, EXACT>;
+    const int jb = blockIdx.y * C;
+    const int je = min(jb + C, nb1 + 1);
+    const bool inx = i >= 0 && i <= nb0;
+    const bool own = lane >= 2 && lane <= kFX - 3 && inx;
+    const int ic = min(max(i, -1), n0 + 2);         // stored column
+    const T ix = a.ix, iy = a.iy;
+    constexpr uint32_t E = sizeof(T);
+    uint32_t b[NLOC2], dy[NLOC2];
+#pragma unroll
+    for (int c = 0; c < NLOC2; ++c) {
+        b[c] = (a.L[c].o0 + (uint32_t)ic - a.L[c].sy) * E;   // row -1
+        dy[c] = a.L[c].sy * E;
+    }
+    auto rowoff = [&](int r, uint32_t* q) {
+        const uint32_t k = (uint32_t)(min(max(r, -1), n1 + 2) + 1);
+#pragma unroll
+        for (int c = 0; c < NLOC2; ++c) q[c] = b[c] + k * dy[c];
+    };
+    // batched V (after the free-slip batch) of A at row q
+    const T hx = T(fs.hx), hy = T(fs.hy);
+    const bool i_in = i >= 0 && i <= n0 + 2;        // inside the tangential range 0..n+2 along x
+    auto vx_bc = [&](int q, T rm, T r0, T rp) -> T {
+        T v = r0;
+        if (q == 0 && i_in) v = fma(hy, -T(fs.qy[0]), rp);
+        else if (q == n1 + 1 && i_in) v = fma(hy, T(fs.qy[1]), rm);
+        if ((i == 1 || i == n0 + 1) && q >= 0 && q <= n1 + 2) v = T(fs.ax[i == 1 ? 0 : 1]);
+        return v;
+    };
+    auto vy_bc = [&](int q, T r0) -> T {
+        T v = r0;
+        if (i_in && q == 1) v = T(fs.by[0]);
+        else if (i_in && q == n1 + 1) v = T(fs.by[1]);
+        const T dn = shdn(v), up = shup(v);
+        const bool q_in = q >= 0 && q <= n1 + 2;
+        if (i == 0 && q_in) return fma(hx, -T(fs.qx[0]), dn);
+        if (i == n0 + 1 && q_in) return fma(hx, T(fs.qx[1]), up);
+        return v;
+    };
+
+    constexpr int S = C + 4;   // A steps (rows jb-2 .. jb+C+1)
+    Carry2<T> A, B;
+    In2<T, BF> in[2];
+    {
+        uint32_t qc[NLOC2], qn[NLOC2];
+        rowoff(jb - 2, qc);
+        rowoff(jb - 1, qn);
+        A.vx_j = ldo(a.Vx, qc[L2X]);
+        A.vy_j = ldo(a.Vy, qc[L2Y]);
+        A.eta_j = FVISC ? ldo(a.eta, qc[L2C]) : T(0);
+        A.P_jm1 = A.tyy_jm1 = A.txy_j = A.eml_jm1 = T(0);
+        load_in2<T, FVISC, BF>(a, qc, qn, HF, in[0]);
+    }
+    B.P_jm1 = B.tyy_jm1 = B.txy_j = B.eml_jm1 = T(0);
+    // A's results by step, as B reads them two steps later
+    T rP[S], rtxx[S], rtyy[S], rtxy1[S], rvx[S], rvy[S];
+    T roxx[S], royy[S], roxy[S], reta1[S], reml[S], rfP[S], rfVx[S], rfVy[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int ra = jb - 2 + t;
+        if (t + 1 < S) {   // the next row's loads go out before this row's work
+            uint32_t qc[NLOC2], qn[NLOC2];
+            rowoff(ra + 1, qc);
+            rowoff(ra + 2, qn);
+            load_in2<T, FVISC, BF>(a, qc, qn, HF, in[(t + 1) & 1]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const In2<T, BF>& cur = in[t & 1];
+        // ---- A, row ra
+        {
+            const bool inA = inx && ra >= 0 && ra <= nb1;
+            const bool inA1 = inx && ra + 1 >= 0 && ra + 1 <= nb1;
+            T P = cur.P, txx = cur.txx, tyy = cur.tyy, txy1 = cur.txy1;
+            const T eta1 = FVISC ? cur.eta1 : T(0), eml = FVISC ? cur.eml : T(0);
+            const T fP = HF ? cvf<T, BF>(cur.fP) : T(0);
+            const T fVx = HF ? cvf<T, BF>(cur.fVx) : T(0), fVy = HF ? cvf<T, BF>(cur.fVy) : T(0);
+            const T oxx = cvf<T, BF>(cur.oxx), oyy = cvf<T, BF>(cur.oyy), oxy = cvf<T, BF>(cur.oxy);
+            const T vx0 = A.vx_j, vy0 = A.vy_j;
+            T vxn, vyn, txy_prev;
+            row2<T, FVISC, EXACT>(K, ix, iy, inA, inA1, HF, A, cur.vx1, cur.vy1, P, txx, tyy, txy1, oxx, oyy, oxy,
+                                  eta1, eml, fP, fVx, fVy, vxn, vyn, txy_prev);
+            const bool keep = inA && t > 0;          // the first step has no velocity
+            rP[t] = P;
+            rtxx[t] = txx;
+            rtyy[t] = tyy;
+            rtxy1[t] = txy1;
+            rvx[t] = keep ? vxn : vx0;
+            rvy[t] = keep ? vyn : vy0;
+            roxx[t] = oxx;
+            royy[t] = oyy;
+            roxy[t] = oxy;
+            reta1[t] = eta1;
+            reml[t] = eml;
+            rfP[t] = fP;
+            rfVx[t] = fVx;
+            rfVy[t] = fVy;
+        }
+        // ---- B, row r = ra - 2
+        if (t >= 3) {
+            const int r = ra - 2;
+            if (t == 3) {
+                B.vx_j = vx_bc(r, rvx[t - 3], rvx[t - 2], rvx[t - 1]);
+                B.vy_j = vy_bc(r, rvy[t - 2]);
+                B.eta_j = reta1[t - 3];
+            }
+            const T vx1 = vx_bc(r + 1, rvx[t - 2], rvx[t - 1], rvx[t]);
+            const T vy1 = vy_bc(r + 1, rvy[t - 1]);
+            const int u = t - 2;
+            T P = rP[u], txx = rtxx[u], tyy = rtyy[u], txy1 = rtxy1[u];
+            const bool inB = inx && r >= 0 && r <= nb1;
+            const bool inB1 = inx && r + 1 >= 0 && r + 1 <= nb1;
+            T vxn, vyn, txy_prev;
+            row2<T, FVISC, EXACT>(K, ix, iy, inB, inB1, HF, B, vx1, vy1, P, txx, tyy, txy1, roxx[u], royy[u],
+                                  roxy[u], reta1[u], reml[u], rfP[u], rfVx[u], rfVy[u], vxn, vyn, txy_prev);
+            if (t >= 4 && r < je && own && inB) {
+                uint32_t q[NLOC2];
+                rowoff(r, q);
+                sto(a.oPr, q[L2C], P);
+                sto(a.otxx, q[L2C], txx);
+                sto(a.otyy, q[L2C], tyy);
+                sto(a.otxy, q[L2XY], txy_prev);
+                sto(a.oVx, q[L2X], vxn);
+                sto(a.oVy, q[L2Y], vyn);
+            }
+        }
+    }
+}
+
 #ifndef CHMY_FUSED_ZCHUNK64
 #define CHMY_FUSED_ZCHUNK64 16
 #endif
@@ -631,6 +907,8 @@ template <class T>
 constexpr int kZChunk = std::is_same<T, double>::value ? CHMY_FUSED_ZCHUNK64 : CHMY_FUSED_ZCHUNK32;
 template <class T>
 constexpr int kBY3 = std::is_same<T, double>::value ? CHMY_FUSED_BY64 : CHMY_FUSED_BY32;
+template <class T>
+constexpr int kTW3 = std::is_same<T, double>::value ? CHMY_FUSED_TW64 : CHMY_FUSED_TW32;
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
@@ -672,7 +950,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
                         const chmy_field_t* rg, const chmy_field_t* fV, const chmy_field_t* fP,
                         const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
-                        hipStream_t s) {
+                        hipStream_t s, const FreeSlip2* tb = nullptr) {
     const bool fv = eta != nullptr;
     const bool bf = told[0].dtype == CHMY_BF16;   // validated: FP32 sweep with bf16 forcing
     // every byte offset a thread forms must fit in 32 bits
@@ -719,7 +997,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.L[LYZ] = loc_stride(tau[5]);
         for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
         a.kchunk = kZChunk<T>;
-        a.ntx = (int32_t)((g->n[0] + 2 + kFX - 3) / (kFX - 2));
+        a.ntx = (int32_t)((g->n[0] + 2 + kTW3<T> - 3) / (kTW3<T> - 2));
         a.nty = (int32_t)((g->n[1] + 2 + kBY3<T> - 2) / (kBY3<T> - 1));
         a.nchunks = (int32_t)((g->n[2] + 2 + kZChunk<T> - 1) / kZChunk<T>);
         a.has_rg = rg != nullptr;
@@ -729,17 +1007,17 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.iz = T(g->inv_spacing[2]);
         a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
-        dim3 blk(kFX, kBY3<T>, 1), grd(nb, 1, 1);
+        dim3 blk(kTW3<T>, kBY3<T>, 1), grd(nb, 1, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
         const bool fonly = a.has_f && !a.write_res && !a.has_rg;
 #define CHMY_F3(FV, EX)                                                                                   \
     do {                                                                                                  \
         constexpr bool F32 = std::is_same<T, float>::value;                                               \
-        if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 1, F32>), grd, blk, 0, s, a); \
-        else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 1, false>), grd, blk, 0, s, a); \
-        else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 7, F32>), grd, blk, 0, s, a);    \
-        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 7, false>), grd, blk, 0, s, a); \
-        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kBY3<T>, 0, false>), grd, blk, 0, s, a);           \
+        if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 1, F32>), grd, blk, 0, s, a); \
+        else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 1, false>), grd, blk, 0, s, a); \
+        else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 7, F32>), grd, blk, 0, s, a);    \
+        else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 7, false>), grd, blk, 0, s, a); \
+        else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 0, false>), grd, blk, 0, s, a);           \
     } while (0)
         if (fv && exact) CHMY_F3(true, true);
         else if (fv) CHMY_F3(true, false);
@@ -777,6 +1055,28 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.iy = T(g->inv_spacing[1]);
         a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
         const int64_t nbx = g->n[0] + 2, nby = g->n[1] + 2;
+        if (tb) {   // two iterations, the free-slip batch between them
+            a.ntx = (int32_t)((nbx + kFX - 5) / (kFX - 4));
+            a.ychunk = exact ? kTBC<T, true> : kTBC<T, false>;
+            a.nchunks = (int32_t)((nby + a.ychunk - 1) / a.ychunk);
+            dim3 blk(kFX, kTBW, 1);
+            dim3 grd((unsigned)((a.ntx + kTBW - 1) / kTBW), (unsigned)a.nchunks, 1);
+            const bool hf = a.has_f;
+#define CHMY_TB2(FV, EX)                                                                                   \
+    do {                                                                                                   \
+        constexpr bool F32 = std::is_same<T, float>::value;                                                \
+        if (hf && bf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a, *tb);   \
+        else if (hf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a, *tb);  \
+        else hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a, *tb);          \
+    } while (0)
+            if (fv && exact) CHMY_TB2(true, true);
+            else if (fv) CHMY_TB2(true, false);
+            else if (exact) CHMY_TB2(false, true);
+            else CHMY_TB2(false, false);
+#undef CHMY_TB2
+            CHMY_HIP(hipGetLastError());
+            return 0;
+        }
         a.ntx = (int32_t)((nbx + kFX - 3) / (kFX - 2));
         a.ychunk = kYChunk2;
         a.nchunks = (int32_t)((nby + kYChunk2 - 1) / kYChunk2);
@@ -809,14 +1109,22 @@ using namespace chmy;
 
 extern "C" {
 
-int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
-                     const chmy_field_t* V_out, const chmy_field_t* tau_out, const chmy_field_t* Pr_out,
-                     const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* eta_ml,
-                     const chmy_field_t* rhog, const chmy_field_t* f_V, const chmy_field_t* f_P,
-                     const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
-                     double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
-                     double nudtau, void* stream) {
+}  // extern "C"
+
+namespace chmy {
+
+// chmy_stokes_iter's checks and launch; `tb` (2D, no residual fields, no rho g)
+// runs two iterations with the free-slip batch between them.
+static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau,
+                            const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
+                            const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                            const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
+                            const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
+                            double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                            double nudtau, void* stream, const FreeSlip2* tb) {
     CHMY_TRY(check_grid(g));
+    if (tb && (g->nd != 2 || rhog || (flags & CHMY_ITER_RESIDUAL)))
+        return set_error(CHMY_E_ARG, "two-iteration pass: 2D, no rho g, no residual fields");
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
     if (flags & ~(CHMY_ITER_RESIDUAL | CHMY_ITER_EXACT)) return set_error(CHMY_E_ARG, "unknown flags %d", flags);
     const int write_residual = (flags & CHMY_ITER_RESIDUAL) != 0;
@@ -876,10 +1184,60 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
     return dt_ == CHMY_F64
                ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                       write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                      (hipStream_t)stream)
+                                      (hipStream_t)stream, tb)
                : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                      write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                     (hipStream_t)stream);
+                                     (hipStream_t)stream, tb);
+}
+
+// The batch names exactly the 2D free-slip conditions with scalar values:
+// one entry per (component, dim, side); Vx Dirichlet in x and Neumann in y,
+// Vy Neumann in x and Dirichlet in y; at least two cells per dim.
+static bool free_slip_2d(const chmy_grid_t* g, const int32_t* counts, const int32_t* comp, const chmy_bc_t* bcs,
+                         FreeSlip2* fs) {
+    if (g->nd != 2 || g->n[0] < 2 || g->n[1] < 2) return false;
+    const chmy_bc_t* rule[2][2][2] = {};   // [component][dim][side]
+    int q = 0;
+    for (int D = 0; D < 2; ++D)
+        for (int sd = 0; sd < 2; ++sd)
+            for (int k = 0; k < counts[2 * D + sd]; ++k, ++q) {
+                const int c = comp[q];
+                if (rule[c][D][sd]) return false;
+                if (bcs[q].vkind != CHMY_VALUE_ZERO && bcs[q].vkind != CHMY_VALUE_SCALAR) return false;
+                rule[c][D][sd] = &bcs[q];
+            }
+    auto val = [](const chmy_bc_t* b) { return b->vkind == CHMY_VALUE_SCALAR ? b->value : 0.0; };
+    for (int c = 0; c < 2; ++c)
+        for (int D = 0; D < 2; ++D)
+            for (int sd = 0; sd < 2; ++sd) {
+                const chmy_bc_t* b = rule[c][D][sd];
+                const int want = c == D ? CHMY_DIRICHLET : CHMY_NEUMANN;
+                if (!b || b->kind != want) return false;
+            }
+    for (int sd = 0; sd < 2; ++sd) {
+        fs->ax[sd] = val(rule[0][0][sd]);
+        fs->qy[sd] = val(rule[0][1][sd]);
+        fs->qx[sd] = val(rule[1][0][sd]);
+        fs->by[sd] = val(rule[1][1][sd]);
+    }
+    fs->hx = g->spacing[0];
+    fs->hy = g->spacing[1];
+    return true;
+}
+
+}  // namespace chmy
+
+extern "C" {
+
+int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
+                     const chmy_field_t* V_out, const chmy_field_t* tau_out, const chmy_field_t* Pr_out,
+                     const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* eta_ml,
+                     const chmy_field_t* rhog, const chmy_field_t* f_V, const chmy_field_t* f_P,
+                     const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
+                     double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                     double nudtau, void* stream) {
+    return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
+                            flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream, nullptr);
 }
 
 // Several PT iterations in one call: each is chmy_stokes_iter from buffer
@@ -919,11 +1277,26 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
         for (int q = 0; q < total; ++q) fset[p][q] = Vs[p][bc_comp[q]];
     const chmy_field_t* T[2] = {tau0, tau1};
     const chmy_field_t* P[2] = {Pr0, Pr1};
+    // 2D free-slip batches run iterations in pairs (one pass, the batch
+    // in-kernel between them); the pair writes the other set, as one iteration
+    // does, so the parity flips once per pair
+    FreeSlip2 fs;
+    const bool tb = CHMY_FUSED_TB && !rhog && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
     int p = *parity_inout;
-    for (int it = 0; it < niter; ++it) {
+    for (int it = 0; it < niter;) {
         const int fl = it == niter - 1 ? flags_last : flags;
-        CHMY_TRY(chmy_stokes_iter(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml, rhog,
-                                  f_V, f_P, divV, r_V, fl, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream));
+        const int fl2 = it + 1 == niter - 1 ? flags_last : flags;
+        if (tb && it + 1 < niter && fl == fl2 && !(fl & CHMY_ITER_RESIDUAL)) {
+            CHMY_TRY(stokes_iter_impl(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml, rhog,
+                                      f_V, f_P, divV, r_V, fl, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
+                                      stream, &fs));
+            it += 2;
+        } else {
+            CHMY_TRY(chmy_stokes_iter(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml,
+                                      rhog, f_V, f_P, divV, r_V, fl, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
+                                      stream));
+            it += 1;
+        }
         CHMY_TRY(chmy_bc_batchset(g, bc_counts, fset[1 - p], bcs, stream));
         p = 1 - p;
     }

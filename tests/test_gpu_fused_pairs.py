@@ -1,0 +1,145 @@
+"""Two PT iterations per pass (2D, the free-slip V batch applied in-kernel
+between them) against two single fused iterations with chmy_bc_batchset
+between and after them, bitwise, on seeded random storage.
+
+chmy_stokes_iterate takes iterations in pairs when the V batch is the 2D
+free-slip batch (Vx Dirichlet in x, Neumann in y; Vy Neumann in x, Dirichlet
+in y; scalar values) and no residual fields or rho g are requested; the
+reference here is the same entry point's single-iteration path, driven one
+iteration at a time (chmy_stokes_iter + chmy_bc_batchset), which
+tests/test_gpu_fused.py pins to the oracle.  Grids cover partial x tiles and
+y chunks, chunks of one row, and the smallest grid the pass takes (2x2).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import chmy
+from chmy import Center as Cc
+from chmy import _lib
+from chmy.kernels import _arr
+from chmy.stokes import pt_params
+
+pytestmark = pytest.mark.gpu
+
+ITER_EXACT = 2
+
+
+@pytest.fixture(scope="module")
+def arch():
+    return chmy.Arch(chmy.MI355XBackend())
+
+
+def _batch(vals):
+    """(counts, comp, conds) in (dim, side) order for the 2D free-slip batch."""
+    (ax1, ax2), (qy1, qy2), (qx1, qx2), (by1, by2) = vals
+    D, N, S = _lib.DIRICHLET, _lib.NEUMANN, _lib.VALUE_SCALAR
+    rows = [(0, D, ax1), (1, N, qx1),      # dim x, side 1
+            (0, D, ax2), (1, N, qx2),      # dim x, side 2
+            (0, N, qy1), (1, D, by1),      # dim y, side 1
+            (0, N, qy2), (1, D, by2)]      # dim y, side 2
+    counts = (C.c_int32 * 6)(2, 2, 2, 2, 0, 0)
+    comp = (C.c_int32 * 8)(*[c for c, _, _ in rows])
+    conds = (_lib.ChmyBC * 8)(*[_lib.ChmyBC(k, S, float(v), None) for _, k, v in rows])
+    return counts, comp, conds
+
+
+class _Set:
+    def __init__(self, arch, g, dt):
+        self.V = chmy.VectorField(arch, g, dt)
+        self.tau = chmy.TensorField(arch, g, dt)
+        self.Pr = chmy.Field(arch, g, Cc(), dt)
+
+    def fields(self):
+        return list(self.V) + list(self.tau) + [self.Pr]
+
+
+def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
+    dt = torch.float64 if T == np.float64 else torch.float32
+    rng = np.random.default_rng(seed)
+    fill = lambda f, lo=-1.0, hi=1.0: f.data.copy_(
+        torch.from_numpy(rng.uniform(lo, hi, tuple(f.data.shape))).to(f.data.dtype))
+    sets = [_Set(arch, g, dt), _Set(arch, g, dt)]
+    for f in sets[0].fields():
+        fill(f)
+    for a, b in zip(sets[1].fields(), sets[0].fields()):
+        a.data.copy_(b.data)
+    fdt = torch.bfloat16 if forcing == "bf16" else dt
+    told = chmy.TensorField(arch, g, fdt)
+    for f in told:
+        fill(f)
+    eta = eml = None
+    if visc == "field":
+        eta, eml = chmy.Field(arch, g, Cc(), dt), chmy.Field(arch, g, Cc(), dt)
+        fill(eta, 1e-3, 3.0)
+        fill(eml, 1e-3, 3.0)
+    fV = fP = None
+    if forcing:
+        fV, fP = chmy.VectorField(arch, g, fdt), chmy.Field(arch, g, Cc(), fdt)
+        for f in list(fV) + [fP]:
+            fill(f, -0.5, 0.5)
+    divV = chmy.Field(arch, g, Cc(), dt)
+    rV = chmy.VectorField(arch, g, dt)
+    counts, comp, conds = _batch(vals)
+    G, dtp, eta_s = 1.0, 0.37, 0.8
+    eve = 1.0 / (1.0 / eta_s + 1.0 / (G * dtp))
+    p = pt_params(g)
+    flags = ITER_EXACT if exact else 0
+    par = C.c_int(0)
+    args = lambda a, b: (C.byref(g.desc()), _arr(list(a.V)), _arr(list(a.tau)), C.byref(a.Pr.desc()),
+                         _arr(list(b.V)), _arr(list(b.tau)), C.byref(b.Pr.desc()), _arr(list(told)),
+                         C.byref(eta.desc()) if eta is not None else None,
+                         C.byref(eml.desc()) if eml is not None else None, None,
+                         _arr(list(fV)) if fV is not None else None,
+                         C.byref(fP.desc()) if fP is not None else None, C.byref(divV.desc()), _arr(list(rV)))
+    scal = (eta_s, eve, G, dtp, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
+    if pairs:
+        _lib.call("chmy_stokes_iterate", *args(sets[0], sets[1]), counts, comp, conds, niter, flags, flags, *scal,
+                  C.byref(par), chmy.current_stream())
+    else:
+        for _ in range(niter):
+            a, b = sets[par.value], sets[1 - par.value]
+            _lib.call("chmy_stokes_iter", *args(a, b), flags, *scal, chmy.current_stream())
+            _lib.call("chmy_bc_batchset", C.byref(g.desc()), counts, _arr([b.V[c] for c in comp]), conds,
+                      chmy.current_stream())
+            par.value = 1 - par.value
+    chmy.synchronize()
+    return [f.data.cpu().numpy() for f in sets[par.value].fields()]
+
+
+SHEAR = ((1.0, -1.0), (0.0, 0.0), (0.0, 0.0), (-1.0, 1.0))
+GENERAL = ((0.7, -1.3), (0.25, -0.6), (-0.4, 0.9), (0.3, -0.2))
+
+
+def _compare(a, b, what):
+    names = ["Vx", "Vy", "txx", "tyy", "txy", "Pr"]
+    for n, x, y in zip(names, a, b):
+        bits = np.int64 if x.dtype == np.float64 else np.int32
+        if not np.array_equal(x.view(bits), y.view(bits)):
+            d = np.nanmax(np.abs(x.astype(np.float64) - y.astype(np.float64)))
+            bad = np.argwhere(x.view(bits) != y.view(bits))
+            raise AssertionError(f"{what}: {n} differs (max |d| {d:.3e}, {len(bad)} points, first {bad[:4].tolist()})")
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("visc", ["field", "scalar"])
+@pytest.mark.parametrize("dims", [(129, 23), (70, 45), (118, 30), (5, 3), (2, 2)])
+def test_pair_pass_equals_single_iterations(arch, T, exact, visc, dims):
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=dims)
+    for vals in (SHEAR, GENERAL):
+        for niter in (2, 5):
+            a = _run(arch, g, T, exact, visc, None, vals, niter, True, 11)
+            b = _run(arch, g, T, exact, visc, None, vals, niter, False, 11)
+            _compare(a, b, f"dims {dims} vals {vals} niter {niter}")
+
+
+@pytest.mark.parametrize("T,forcing", [(np.float64, "same"), (np.float32, "same"), (np.float32, "bf16")])
+@pytest.mark.parametrize("exact", [True, False])
+def test_pair_pass_with_correction_forcing(arch, T, forcing, exact):
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=(70, 45))
+    a = _run(arch, g, T, exact, "field", forcing, GENERAL, 4, True, 12)
+    b = _run(arch, g, T, exact, "field", forcing, GENERAL, 4, False, 12)
+    _compare(a, b, f"forcing {forcing}")
