@@ -641,7 +641,7 @@ struct FreeSlip2 {
 #define CHMY_FUSED_TB 1   // 0: chmy_stokes_iterate runs single iterations only
 #endif
 #ifndef CHMY_FUSED_TB_CHUNK
-#define CHMY_FUSED_TB_CHUNK 16
+#define CHMY_FUSED_TB_CHUNK 8
 #endif
 #ifndef CHMY_FUSED_TB_W
 #define CHMY_FUSED_TB_W 4
