@@ -29,7 +29,7 @@ from .kernels import _arr, _box, maxloc, HipKernel
 from .kernel_launch import Launcher
 
 
-ITER_RESIDUAL, ITER_EXACT = 1, 2
+ITER_RESIDUAL, ITER_EXACT, ITER_PAIRS = 1, 2, 4
 
 
 def pt_params(grid, re_m=2.3 * math.pi, r=0.5):

@@ -897,6 +897,454 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
     }
 }
 
+// ------------------------------------------------------------------ 3D, two PT iterations per pass
+// The V batch of a 3D free-slip solve (stokes_3d_inc_ve_T.jl:188-190): each
+// component Dirichlet on its own dim (vertex nodes 1 / n+1), Neumann on the
+// other two; v[component][dim][side] the scalar value, h the spacings.
+struct FreeSlip3 {
+    double v[3][3][2];
+    double h[3];
+};
+
+#ifndef CHMY_FUSED_TB3_BY64
+#define CHMY_FUSED_TB3_BY64 8
+#endif
+#ifndef CHMY_FUSED_TB3_BY32
+#define CHMY_FUSED_TB3_BY32 12
+#endif
+#ifndef CHMY_FUSED_TB3_ZCHUNK
+#define CHMY_FUSED_TB3_ZCHUNK 16
+#endif
+template <class T>
+constexpr int kTB3Y = std::is_same<T, double>::value ? CHMY_FUSED_TB3_BY64 : CHMY_FUSED_TB3_BY32;
+constexpr int kTB3Z = CHMY_FUSED_TB3_ZCHUNK;
+
+// Two PT iterations in one pass over a 64 x BY tile marching z: the first
+// iteration (A) is fused3d_kernel's step on planes kb-2 .. ke+1 with nothing
+// stored; its velocities get the free-slip batch in-kernel (per point, the
+// batch's own order: z, then y through an LDS exchange of the neighbouring
+// rows, then x through lane shuffles; the reference's fma steps); the second
+// iteration (B) trails A by two planes with its own LDS planes and stores
+// planes kb .. ke-1.  A is exact on lanes 1..62 and rows 1..BY-1 (rows for
+// its velocities; stresses one more), B's outputs are lanes 2..61 and rows
+// 2..BY-2 (Center values, V) and 1..BY-3 (edges at j+1).  The eta and eta_ml
+// planes are 4-plane rings that B reads two planes behind A; B re-reads
+// tau_old and the forcing of its plane.  Rows, columns and planes are
+// clamped into the stored range for loads; clamped values reach no output.
+// Results are bitwise those of two fused3d_kernel launches with the batch
+// between them.
+template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
+__global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
+    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
+    constexpr int BY = kTB3Y<T>, NR = BY + 1, TW = kFX;
+    constexpr bool HF = OPT == 1;
+    using AR = Arith<T, EXACT>;
+    const Kc<T>& K = a.k;
+    __shared__ T s_vx[NR][TW], s_vy[2][NR][TW], s_vz1[NR][TW];
+    __shared__ T s_eta[FVISC ? 4 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 4 : 1][FVISC ? BY : 1][TW];
+    __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW], s_tyz[2][BY][TW];
+    __shared__ T b_vx[NR][TW], b_vy[2][NR][TW], b_vz1[NR][TW];
+    __shared__ T b_P[BY][TW], b_tyy[BY][TW], b_txy[BY][TW], b_tyz[2][BY][TW];
+    __shared__ T s_xc[2][NR][TW];
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int nb = a.ntx * a.nty * a.nchunks;
+    const int lt = xcd_remap(blockIdx.x, nb);
+    const int bx = lt % a.ntx, by = (lt / a.ntx) % a.nty, bz = lt / (a.ntx * a.nty);
+    const int i = bx * (TW - 4) - 2 + tx;
+    const int j = by * (BY - 3) - 2 + ty;
+    const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
+    const int nb0 = n0 + 1, nb1 = n1 + 1, nb2 = n2 + 1;
+    const int kb = bz * a.kchunk;
+    const int ke = min(kb + a.kchunk, nb2 + 1);
+    const int ic = min(max(i, -1), n0 + 2), jc = min(max(j, -1), n1 + 2);
+    const bool inx = i >= 0 && i <= nb0;
+    const bool inc = inx && j >= 0 && j <= nb1;
+    const bool ine = inx && j + 1 >= 0 && j + 1 <= nb1;
+    const bool ownx = tx >= 2 && tx <= TW - 3;
+    const bool ownc = ownx && ty >= 2 && ty <= BY - 2 && inc;
+    const bool owne = ownx && ty >= 1 && ty <= BY - 3 && ine;
+    const bool last = ty == BY - 1;
+    const bool row8 = last && jc + 1 <= n1 + 2;
+    const T ix = a.ix, iy = a.iy, iz = a.iz;
+    constexpr uint32_t E = sizeof(T);
+    uint32_t b0[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
+#pragma unroll
+    for (int c = 0; c < NLOC; ++c) {
+        const LocStride sl = a.L[c];
+        b0[c] = (uint32_t)((int64_t)sl.o0 + ic + (int64_t)jc * sl.sy - sl.sz) * E;   // plane -1
+        dz[c] = sl.sz * E;
+        dy[c] = sl.sy * E;
+        d8[c] = row8 ? dy[c] : 0u;
+    }
+    auto zo = [&](int k, int c) -> uint32_t { return b0[c] + (uint32_t)(min(max(k, -1), n2 + 2) + 1) * dz[c]; };
+    auto ld = [](const T* p, uint32_t off) -> T { return ldo(p, off); };
+    const int tx1 = tx < TW - 1 ? tx + 1 : tx, txm = tx > 0 ? tx - 1 : tx;
+    const int tym = ty > 0 ? ty - 1 : ty, typ = ty + 1;   // typ <= BY: row BY exists in every NR array
+
+    // the batch: after z (own plane ring), y (exchange), x (shuffles)
+    const bool ij_in = i >= 0 && i <= n0 + 2 && j >= 0 && j <= n1 + 2;   // z rule's tangential range
+    const T hx = T(fs.h[0]), hy = T(fs.h[1]), hz = T(fs.h[2]);
+    auto zrule = [&](int c, int q, T rm, T r0, T rp) -> T {
+        if (!ij_in) return r0;
+        if (c == 2) {   // Vz: Dirichlet on its z nodes
+            if (q == 1) return T(fs.v[2][2][0]);
+            if (q == n2 + 1) return T(fs.v[2][2][1]);
+            return r0;
+        }
+        if (q == 0) return fma(hz, -T(fs.v[c][2][0]), rp);
+        if (q == n2 + 1) return fma(hz, T(fs.v[c][2][1]), rm);
+        return r0;
+    };
+    // V after the batch at plane q from the raw first-iteration values at q-1, q, q+1
+    auto batched = [&](int q, const T* rm, const T* r0, const T* rp, T* out) {
+        T az[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) az[c] = zrule(c, q, rm[c], r0[c], rp[c]);
+        s_xc[0][ty][tx] = az[0];
+        s_xc[1][ty][tx] = az[2];
+        __syncthreads();
+        const bool iq_in = i >= 0 && i <= n0 + 2 && q >= 0 && q <= n2 + 2;   // y rule's range
+        const bool jq_in = j >= 0 && j <= n1 + 2 && q >= 0 && q <= n2 + 2;   // x rule's range
+        T ay[3];
+#pragma unroll
+        for (int c = 0; c < 3; c += 2) {   // Vx, Vz: Neumann in y
+            const int s = c == 0 ? 0 : 1;
+            T v = az[c];
+            if (iq_in && j == 0) v = fma(hy, -T(fs.v[c][1][0]), s_xc[s][typ][tx]);
+            else if (iq_in && j == n1 + 1) v = fma(hy, T(fs.v[c][1][1]), s_xc[s][tym][tx]);
+            ay[c] = v;
+        }
+        ay[1] = az[1];   // Vy: Dirichlet on its y nodes
+        if (iq_in && j == 1) ay[1] = T(fs.v[1][1][0]);
+        else if (iq_in && j == n1 + 1) ay[1] = T(fs.v[1][1][1]);
+        // x: Vx Dirichlet on its x nodes, Vy and Vz Neumann from the neighbouring lane
+        out[0] = ay[0];
+        if (jq_in && i == 1) out[0] = T(fs.v[0][0][0]);
+        else if (jq_in && i == n0 + 1) out[0] = T(fs.v[0][0][1]);
+#pragma unroll
+        for (int c = 1; c < 3; ++c) {
+            const T dn = shdn(ay[c]), up = shup(ay[c]);
+            T v = ay[c];
+            if (jq_in && i == 0) v = fma(hx, -T(fs.v[c][0][0]), dn);
+            else if (jq_in && i == n0 + 1) v = fma(hx, T(fs.v[c][0][1]), up);
+            out[c] = v;
+        }
+    };
+
+    // ---- A: carried state at plane kb-2
+    int pa = 0, pb = 0;
+    const int k0 = kb - 2;
+    T Avx = ld(a.Vx, zo(k0, LVX)), Avy = ld(a.Vy, zo(k0, LVY)), Avz = ld(a.Vz, zo(k0, LVZ));
+    T Aeta = T(0), Aeml_m = T(0), AP_m = T(0), Atzz_m = T(0), Atxz = T(0), Atyz = T(0);
+    s_vy[0][ty][tx] = Avy;
+    if constexpr (FVISC) {
+        Aeta = ld(a.eta, zo(k0, LC));
+        s_eta[k0 & 3][ty][tx] = Aeta;
+    }
+    if (last) {
+        s_vy[0][BY][tx] = ld(a.Vy, zo(k0, LVY) + d8[LVY]);
+        if constexpr (FVISC) s_eta[k0 & 3][BY][tx] = ld(a.eta, zo(k0, LC) + d8[LC]);
+    }
+    // ring of A's results (index 2: plane a-2, 1: a-1, 0: a)
+    T rP[3], rtxx[3], rtyy[3], rtzz[3], rtxy[3], rtxz[3], rtyz[3], rv[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        rP[q] = rtxx[q] = rtyy[q] = rtzz[q] = rtxy[q] = rtxz[q] = rtyz[q] = T(0);
+        rv[q][0] = rv[q][1] = rv[q][2] = T(0);
+    }
+    // B's carried state
+    T Bv[3] = {T(0), T(0), T(0)};
+    T BP_m = T(0), Btzz_m = T(0), Btxz = T(0), Btyz = T(0), Beml_m = T(0);
+
+    const int S = ke - kb + 4;
+    for (int t = 0; t < S; ++t) {
+        const int ka = kb - 2 + t;
+        // ================= A: plane ka (fused3d_kernel's step, nothing stored)
+        {
+            const uint32_t oC = zo(ka, LC), oC1 = zo(ka + 1, LC);
+            const T vx_k1 = ld(a.Vx, zo(ka + 1, LVX));
+            const T vy_k1 = ld(a.Vy, zo(ka + 1, LVY));
+            const T vz_k1 = ld(a.Vz, zo(ka + 1, LVZ));
+            T eta_k1 = T(0), eml_k = T(0);
+            if constexpr (FVISC) {
+                eta_k1 = ld(a.eta, oC1);
+                eml_k = ld(a.eml, oC);
+            }
+            T P = ld(a.Pr, oC), txx = ld(a.txx, oC), tyy = ld(a.tyy, oC), tzz = ld(a.tzz, oC);
+            T txy1 = ld(a.txy, zo(ka, LXY) + dy[LXY]);
+            T txz1 = ld(a.txz, zo(ka + 1, LXZ)), tyz1 = ld(a.tyz, zo(ka + 1, LYZ) + dy[LYZ]);
+            const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
+            const T oxy = ldf<T, BF>(a.Oxy, zo(ka, LXY) + dy[LXY]);
+            const T oxz = ldf<T, BF>(a.Oxz, zo(ka + 1, LXZ)), oyz = ldf<T, BF>(a.Oyz, zo(ka + 1, LYZ) + dy[LYZ]);
+            T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
+            if constexpr (HF) {
+                fP = ldf<T, BF>(a.fP, oC);
+                fVx = ldf<T, BF>(a.fVx, zo(ka, LVX));
+                fVy = ldf<T, BF>(a.fVy, zo(ka, LVY));
+                fVz = ldf<T, BF>(a.fVz, zo(ka, LVZ));
+            }
+            T vx8 = T(0), vy8_1 = T(0), vz8_1 = T(0), eta8_1 = T(0);
+            if (last) {
+                vx8 = ld(a.Vx, zo(ka, LVX) + d8[LVX]);
+                vy8_1 = ld(a.Vy, zo(ka + 1, LVY) + d8[LVY]);
+                vz8_1 = ld(a.Vz, zo(ka + 1, LVZ) + d8[LVZ]);
+                if constexpr (FVISC) eta8_1 = ld(a.eta, oC1 + d8[LC]);
+            }
+            const bool in_k = ka >= 0 && ka <= nb2;
+            const bool in_k1 = ka + 1 >= 0 && ka + 1 <= nb2;
+            const int e0 = ka & 3, e1 = (ka + 1) & 3;
+            s_vx[ty][tx] = Avx;
+            s_vy[pa ^ 1][ty][tx] = vy_k1;
+            s_vz1[ty][tx] = vz_k1;
+            if constexpr (FVISC) {
+                s_eta[e1][ty][tx] = eta_k1;
+                s_eml[e0][ty][tx] = eml_k;
+            }
+            if (last) {
+                s_vx[BY][tx] = vx8;
+                s_vy[pa ^ 1][BY][tx] = vy8_1;
+                s_vz1[BY][tx] = vz8_1;
+                if constexpr (FVISC) s_eta[e1][BY][tx] = eta8_1;
+            }
+            __syncthreads();
+            const T exx = (s_vx[ty][tx1] - Avx) * ix;
+            const T eyy = (s_vy[pa][typ][tx] - Avy) * iy;
+            const T ezz = (vz_k1 - Avz) * iz;
+            const T dv = (exx + eyy) + ezz;
+            {
+                typename AR::Loc Lc;
+                if constexpr (FVISC) Lc = AR::field(Aeta, K);
+                else Lc = AR::scalar(K);
+                const T Pn = AR::pr(P, HF ? dv - fP : dv, Lc, K);
+                const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
+                const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+                const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
+                const bool u = inc && in_k;
+                P = u ? Pn : P;
+                txx = u ? txxn : txx;
+                tyy = u ? tyyn : tyy;
+                tzz = u ? tzzn : tzz;
+            }
+            {
+                const T exy = T(0.5) * ((s_vx[typ][tx] - Avx) * iy + (s_vy[pa][typ][tx] - s_vy[pa][typ][txm]) * ix);
+                typename AR::Loc Le;
+                if constexpr (FVISC)
+                    Le = AR::field(lerp4(s_eta[e0][ty][txm], Aeta, s_eta[e0][typ][txm], s_eta[e0][typ][tx]), K);
+                else Le = AR::scalar(K);
+                const T n = AR::off(txy1, oxy, exy, Le, K);
+                txy1 = ine && in_k ? n : txy1;
+            }
+            {
+                const T exz = T(0.5) * ((vx_k1 - Avx) * iz + (vz_k1 - s_vz1[ty][txm]) * ix);
+                const T eyz = T(0.5) * ((s_vy[pa ^ 1][typ][tx] - s_vy[pa][typ][tx]) * iz + (s_vz1[typ][tx] - vz_k1) * iy);
+                typename AR::Loc L4, L5;
+                if constexpr (FVISC) {
+                    L4 = AR::field(lerp4(s_eta[e0][ty][txm], Aeta, s_eta[e1][ty][txm], eta_k1), K);
+                    L5 = AR::field(lerp4(Aeta, s_eta[e0][typ][tx], eta_k1, s_eta[e1][typ][tx]), K);
+                } else {
+                    L4 = L5 = AR::scalar(K);
+                }
+                const T nxz = AR::off(txz1, oxz, exz, L4, K);
+                const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
+                txz1 = inc && in_k1 ? nxz : txz1;
+                tyz1 = ine && in_k1 ? nyz : tyz1;
+            }
+            const T txxl = shup(txx), txzr = shdn(Atxz);   // x neighbours within the row's wave
+            s_P[ty][tx] = P;
+            s_tyy[ty][tx] = tyy;
+            s_txy[ty][tx] = txy1;
+            s_tyz[pa][ty][tx] = tyz1;
+            __syncthreads();
+            const T txy_j = s_txy[tym][tx];
+            T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy) + (txz1 - Atxz) * iz;
+            T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[tym][tx1] - txy_j) * ix) +
+                   (s_tyz[pa][tym][tx] - s_tyz[pa ^ 1][tym][tx]) * iz;
+            T rz = ((-((P - AP_m) * iz) + (tzz - Atzz_m) * iz) + (txzr - Atxz) * ix) +
+                   (Atyz - s_tyz[pa ^ 1][tym][tx]) * iy;
+            if constexpr (HF) {
+                rx = rx + fVx;
+                ry = ry + fVy;
+                rz = rz + fVz;
+            }
+            T ex, ey, ez;
+            if constexpr (FVISC) {
+                ex = AR::vcoef_field(jmax(s_eml[e0][ty][txm], eml_k), K);
+                ey = AR::vcoef_field(jmax(s_eml[e0][tym][tx], eml_k), K);
+                ez = AR::vcoef_field(jmax(Aeml_m, eml_k), K);
+            } else {
+                ex = ey = ez = AR::vcoef_scalar(K);
+            }
+            const bool keep = inc && in_k && t > 0;   // the first plane has no velocity
+            // shift the ring, then plane ka's results go in slot 0
+#pragma unroll
+            for (int q = 2; q > 0; --q) {
+                rP[q] = rP[q - 1];
+                rtxx[q] = rtxx[q - 1];
+                rtyy[q] = rtyy[q - 1];
+                rtzz[q] = rtzz[q - 1];
+                rtxy[q] = rtxy[q - 1];
+                rtxz[q] = rtxz[q - 1];
+                rtyz[q] = rtyz[q - 1];
+                rv[q][0] = rv[q - 1][0];
+                rv[q][1] = rv[q - 1][1];
+                rv[q][2] = rv[q - 1][2];
+            }
+            rP[0] = P;
+            rtxx[0] = txx;
+            rtyy[0] = tyy;
+            rtzz[0] = tzz;
+            rtxy[0] = txy1;
+            rtxz[0] = txz1;
+            rtyz[0] = tyz1;
+            rv[0][0] = keep ? AR::vel(Avx, rx, ex, K) : Avx;
+            rv[0][1] = keep ? AR::vel(Avy, ry, ey, K) : Avy;
+            rv[0][2] = keep ? AR::vel(Avz, rz, ez, K) : Avz;
+            AP_m = P;
+            Atzz_m = tzz;
+            Atxz = txz1;
+            Atyz = tyz1;
+            Avx = vx_k1;
+            Avy = vy_k1;
+            Avz = vz_k1;
+            Aeta = eta_k1;
+            Aeml_m = eml_k;
+            pa ^= 1;
+        }
+        if (t < 3) continue;
+        // ================= B: plane r = ka - 2
+        const int r = ka - 2;
+        T V1[3];
+        if (t == 3) {   // B's first plane: V after the batch at plane r as well
+            batched(r, rv[2], rv[2], rv[1], Bv);   // plane r-1 is never a source here (r < n2+1)
+            __syncthreads();
+        }
+        batched(r + 1, rv[2], rv[1], rv[0], V1);
+        const int f0 = r & 3, f1 = (r + 1) & 3;
+        b_vx[ty][tx] = Bv[0];
+        if (t == 3) b_vy[pb][ty][tx] = Bv[1];
+        b_vy[pb ^ 1][ty][tx] = V1[1];
+        b_vz1[ty][tx] = V1[2];
+        __syncthreads();
+        {
+            const uint32_t oC = zo(r, LC);
+            const T eta_k = FVISC ? s_eta[f0][ty][tx] : T(0), eta_k1 = FVISC ? s_eta[f1][ty][tx] : T(0);
+            const T eml_k = FVISC ? s_eml[f0][ty][tx] : T(0);
+            T P = rP[2], txx = rtxx[2], tyy = rtyy[2], tzz = rtzz[2];
+            T txy1 = rtxy[2], txz1 = rtxz[2], tyz1 = rtyz[2];
+            const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
+            const T oxy = ldf<T, BF>(a.Oxy, zo(r, LXY) + dy[LXY]);
+            const T oxz = ldf<T, BF>(a.Oxz, zo(r + 1, LXZ)), oyz = ldf<T, BF>(a.Oyz, zo(r + 1, LYZ) + dy[LYZ]);
+            T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
+            if constexpr (HF) {
+                fP = ldf<T, BF>(a.fP, oC);
+                fVx = ldf<T, BF>(a.fVx, zo(r, LVX));
+                fVy = ldf<T, BF>(a.fVy, zo(r, LVY));
+                fVz = ldf<T, BF>(a.fVz, zo(r, LVZ));
+            }
+            const bool in_k = r >= 0 && r <= nb2;
+            const bool in_k1 = r + 1 >= 0 && r + 1 <= nb2;
+            const T vx_k = Bv[0], vy_k = Bv[1], vz_k = Bv[2];
+            const T vx_k1 = V1[0], vz_k1 = V1[2];
+            const T exx = (b_vx[ty][tx1] - vx_k) * ix;
+            const T eyy = (b_vy[pb][typ][tx] - vy_k) * iy;
+            const T ezz = (vz_k1 - vz_k) * iz;
+            const T dv = (exx + eyy) + ezz;
+            {
+                typename AR::Loc Lc;
+                if constexpr (FVISC) Lc = AR::field(eta_k, K);
+                else Lc = AR::scalar(K);
+                const T Pn = AR::pr(P, HF ? dv - fP : dv, Lc, K);
+                const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
+                const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
+                const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
+                const bool u = inc && in_k;
+                P = u ? Pn : P;
+                txx = u ? txxn : txx;
+                tyy = u ? tyyn : tyy;
+                tzz = u ? tzzn : tzz;
+            }
+            {
+                const T exy = T(0.5) * ((b_vx[typ][tx] - vx_k) * iy + (b_vy[pb][typ][tx] - b_vy[pb][typ][txm]) * ix);
+                typename AR::Loc Le;
+                if constexpr (FVISC)
+                    Le = AR::field(lerp4(s_eta[f0][ty][txm], eta_k, s_eta[f0][typ][txm], s_eta[f0][typ][tx]), K);
+                else Le = AR::scalar(K);
+                const T n = AR::off(txy1, oxy, exy, Le, K);
+                txy1 = ine && in_k ? n : txy1;
+            }
+            {
+                const T exz = T(0.5) * ((vx_k1 - vx_k) * iz + (vz_k1 - b_vz1[ty][txm]) * ix);
+                const T eyz = T(0.5) * ((b_vy[pb ^ 1][typ][tx] - b_vy[pb][typ][tx]) * iz + (b_vz1[typ][tx] - vz_k1) * iy);
+                typename AR::Loc L4, L5;
+                if constexpr (FVISC) {
+                    L4 = AR::field(lerp4(s_eta[f0][ty][txm], eta_k, s_eta[f1][ty][txm], eta_k1), K);
+                    L5 = AR::field(lerp4(eta_k, s_eta[f0][typ][tx], eta_k1, s_eta[f1][typ][tx]), K);
+                } else {
+                    L4 = L5 = AR::scalar(K);
+                }
+                const T nxz = AR::off(txz1, oxz, exz, L4, K);
+                const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
+                txz1 = inc && in_k1 ? nxz : txz1;
+                tyz1 = ine && in_k1 ? nyz : tyz1;
+            }
+            const bool st = r >= kb && r < ke;
+            if (st) {
+                if (ownc) {
+                    sto(a.oPr, oC, P);
+                    sto(a.otxx, oC, txx);
+                    sto(a.otyy, oC, tyy);
+                    sto(a.otzz, oC, tzz);
+                    sto(a.otxz, zo(r, LXZ), Btxz);
+                }
+                if (owne) {
+                    sto(a.otxy, zo(r, LXY) + dy[LXY], txy1);
+                    sto(a.otyz, zo(r, LYZ) + dy[LYZ], Btyz);
+                }
+            }
+            const T txxl = shup(txx), txzr = shdn(Btxz);
+            b_P[ty][tx] = P;
+            b_tyy[ty][tx] = tyy;
+            b_txy[ty][tx] = txy1;
+            b_tyz[pb][ty][tx] = tyz1;
+            __syncthreads();
+            const T txy_j = b_txy[tym][tx];
+            T rx = ((-((P - b_P[ty][txm]) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy) + (txz1 - Btxz) * iz;
+            T ry = ((-((P - b_P[tym][tx]) * iy) + (tyy - b_tyy[tym][tx]) * iy) + (b_txy[tym][tx1] - txy_j) * ix) +
+                   (b_tyz[pb][tym][tx] - b_tyz[pb ^ 1][tym][tx]) * iz;
+            T rz = ((-((P - BP_m) * iz) + (tzz - Btzz_m) * iz) + (txzr - Btxz) * ix) +
+                   (Btyz - b_tyz[pb ^ 1][tym][tx]) * iy;
+            if constexpr (HF) {
+                rx = rx + fVx;
+                ry = ry + fVy;
+                rz = rz + fVz;
+            }
+            if (st && ownc) {
+                T ex, ey, ez;
+                if constexpr (FVISC) {
+                    ex = AR::vcoef_field(jmax(s_eml[f0][ty][txm], eml_k), K);
+                    ey = AR::vcoef_field(jmax(s_eml[f0][tym][tx], eml_k), K);
+                    ez = AR::vcoef_field(jmax(Beml_m, eml_k), K);
+                } else {
+                    ex = ey = ez = AR::vcoef_scalar(K);
+                }
+                sto(a.oVx, zo(r, LVX), AR::vel(vx_k, rx, ex, K));
+                sto(a.oVy, zo(r, LVY), AR::vel(vy_k, ry, ey, K));
+                sto(a.oVz, zo(r, LVZ), AR::vel(vz_k, rz, ez, K));
+            }
+            BP_m = P;
+            Btzz_m = tzz;
+            Btxz = txz1;
+            Btyz = tyz1;
+            Bv[0] = V1[0];
+            Bv[1] = V1[1];
+            Bv[2] = V1[2];
+            Beml_m = eml_k;
+            pb ^= 1;
+        }
+    }
+}
+
 #ifndef CHMY_FUSED_ZCHUNK64
 #define CHMY_FUSED_ZCHUNK64 16
 #endif
@@ -950,7 +1398,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
                         const chmy_field_t* rg, const chmy_field_t* fV, const chmy_field_t* fP,
                         const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
-                        hipStream_t s, const FreeSlip2* tb = nullptr) {
+                        hipStream_t s, const FreeSlip2* tb = nullptr, const FreeSlip3* tb3 = nullptr) {
     const bool fv = eta != nullptr;
     const bool bf = told[0].dtype == CHMY_BF16;   // validated: FP32 sweep with bf16 forcing
     // every byte offset a thread forms must fit in 32 bits
@@ -996,6 +1444,35 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.L[LXZ] = loc_stride(tau[4]);
         a.L[LYZ] = loc_stride(tau[5]);
         for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
+        if (tb3) {   // two iterations, the free-slip batch between them
+            a.has_rg = 0;
+            a.write_res = 0;
+            a.ix = T(g->inv_spacing[0]);
+            a.iy = T(g->inv_spacing[1]);
+            a.iz = T(g->inv_spacing[2]);
+            a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
+            a.kchunk = kTB3Z;
+            a.ntx = (int32_t)((g->n[0] + 2 + kFX - 5) / (kFX - 4));
+            a.nty = (int32_t)((g->n[1] + 2 + kTB3Y<T> - 4) / (kTB3Y<T> - 3));
+            a.nchunks = (int32_t)((g->n[2] + 2 + kTB3Z - 1) / kTB3Z);
+            const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
+            dim3 blk(kFX, kTB3Y<T>, 1), grd(nb, 1, 1);
+            const bool hf = a.has_f;
+#define CHMY_TB3(FV, EX)                                                                                    \
+    do {                                                                                                    \
+        constexpr bool F32 = std::is_same<T, float>::value;                                                 \
+        if (hf && bf) hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a, *tb3);   \
+        else if (hf) hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a, *tb3);  \
+        else hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a, *tb3);          \
+    } while (0)
+            if (fv && exact) CHMY_TB3(true, true);
+            else if (fv) CHMY_TB3(true, false);
+            else if (exact) CHMY_TB3(false, true);
+            else CHMY_TB3(false, false);
+#undef CHMY_TB3
+            CHMY_HIP(hipGetLastError());
+            return 0;
+        }
         a.kchunk = kZChunk<T>;
         a.ntx = (int32_t)((g->n[0] + 2 + kTW3<T> - 3) / (kTW3<T> - 2));
         a.nty = (int32_t)((g->n[1] + 2 + kBY3<T> - 2) / (kBY3<T> - 1));
@@ -1121,10 +1598,10 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
                             const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
                             const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
                             double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
-                            double nudtau, void* stream, const FreeSlip2* tb) {
+                            double nudtau, void* stream, const FreeSlip2* tb, const FreeSlip3* tb3) {
     CHMY_TRY(check_grid(g));
-    if (tb && (g->nd != 2 || rhog || (flags & CHMY_ITER_RESIDUAL)))
-        return set_error(CHMY_E_ARG, "two-iteration pass: 2D, no rho g, no residual fields");
+    if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || ((tb || tb3) && (rhog || (flags & CHMY_ITER_RESIDUAL))))
+        return set_error(CHMY_E_ARG, "two-iteration pass: free-slip batch of the grid's dims, no rho g, no residual fields");
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
     if (flags & ~(CHMY_ITER_RESIDUAL | CHMY_ITER_EXACT)) return set_error(CHMY_E_ARG, "unknown flags %d", flags);
     const int write_residual = (flags & CHMY_ITER_RESIDUAL) != 0;
@@ -1184,10 +1661,10 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
     return dt_ == CHMY_F64
                ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                       write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                      (hipStream_t)stream, tb)
+                                      (hipStream_t)stream, tb, tb3)
                : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                      write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                     (hipStream_t)stream, tb);
+                                     (hipStream_t)stream, tb, tb3);
 }
 
 // The batch names exactly the 2D free-slip conditions with scalar values:
@@ -1225,6 +1702,33 @@ static bool free_slip_2d(const chmy_grid_t* g, const int32_t* counts, const int3
     return true;
 }
 
+// The batch names exactly the 3D free-slip conditions with scalar values:
+// one entry per (component, dim, side); component c Dirichlet on dim c and
+// Neumann on the other dims; at least two cells per dim.
+static bool free_slip_3d(const chmy_grid_t* g, const int32_t* counts, const int32_t* comp, const chmy_bc_t* bcs,
+                         FreeSlip3* fs) {
+    if (g->nd != 3 || g->n[0] < 2 || g->n[1] < 2 || g->n[2] < 2) return false;
+    const chmy_bc_t* rule[3][3][2] = {};   // [component][dim][side]
+    int q = 0;
+    for (int D = 0; D < 3; ++D)
+        for (int sd = 0; sd < 2; ++sd)
+            for (int k = 0; k < counts[2 * D + sd]; ++k, ++q) {
+                const int c = comp[q];
+                if (rule[c][D][sd]) return false;
+                if (bcs[q].vkind != CHMY_VALUE_ZERO && bcs[q].vkind != CHMY_VALUE_SCALAR) return false;
+                rule[c][D][sd] = &bcs[q];
+            }
+    for (int c = 0; c < 3; ++c)
+        for (int D = 0; D < 3; ++D)
+            for (int sd = 0; sd < 2; ++sd) {
+                const chmy_bc_t* b = rule[c][D][sd];
+                if (!b || b->kind != (c == D ? CHMY_DIRICHLET : CHMY_NEUMANN)) return false;
+                fs->v[c][D][sd] = b->vkind == CHMY_VALUE_SCALAR ? b->value : 0.0;
+            }
+    for (int D = 0; D < 3; ++D) fs->h[D] = g->spacing[D];
+    return true;
+}
+
 }  // namespace chmy
 
 extern "C" {
@@ -1237,7 +1741,7 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                      double nudtau, void* stream) {
     return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
-                            flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream, nullptr);
+                            flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream, nullptr, nullptr);
 }
 
 // Several PT iterations in one call: each is chmy_stokes_iter from buffer
@@ -1280,8 +1784,16 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
     // 2D free-slip batches run iterations in pairs (one pass, the batch
     // in-kernel between them); the pair writes the other set, as one iteration
     // does, so the parity flips once per pair
+    // (3D pairs on request: measured slower than single iterations in FP64
+    // and in the correction sweep at 255^3, faster for plain FP32 sweeps)
+    const bool pairs3 = (flags & CHMY_ITER_PAIRS) != 0;
+    flags &= ~CHMY_ITER_PAIRS;
+    flags_last &= ~CHMY_ITER_PAIRS;
     FreeSlip2 fs;
-    const bool tb = CHMY_FUSED_TB && !rhog && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    FreeSlip3 fs3;
+    const bool tb2 = CHMY_FUSED_TB && !rhog && nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    const bool tb3 = pairs3 && !rhog && nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
+    const bool tb = tb2 || tb3;
     int p = *parity_inout;
     for (int it = 0; it < niter;) {
         const int fl = it == niter - 1 ? flags_last : flags;
@@ -1289,7 +1801,7 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
         if (tb && it + 1 < niter && fl == fl2 && !(fl & CHMY_ITER_RESIDUAL)) {
             CHMY_TRY(stokes_iter_impl(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml, rhog,
                                       f_V, f_P, divV, r_V, fl, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                      stream, &fs));
+                                      stream, tb2 ? &fs : nullptr, tb3 ? &fs3 : nullptr));
             it += 2;
         } else {
             CHMY_TRY(chmy_stokes_iter(g, Vs[p], T[p], P[p], Vs[1 - p], T[1 - p], P[1 - p], tau_old, eta, eta_ml,

@@ -189,7 +189,7 @@ int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const ch
  * the iteration solves for a correction δ driven by a residual R of an f64
  * state: r_V += f_V, and Pr uses (divV - f_P); the stress forcing enters as
  * tau_old = G dt R_tau (see chmy_stokes_residual).  NULL f_V/f_P: plain PT. */
-enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2 };
+enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2, CHMY_ITER_PAIRS = 4 };
 int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                      const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                      const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
@@ -204,7 +204,13 @@ int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_
  * entries per (dim, side) in bc! order, entry q applies bcs[q] to V component
  * bc_comp[q] (at most 64 entries).  `flags` for every iteration, `flags_last`
  * for the last (e.g. CHMY_ITER_RESIDUAL for a convergence check).  On return
- * *parity_inout names the set holding the new state. */
+ * *parity_inout names the set holding the new state.
+ * Build extension: with a free-slip V batch (each component Dirichlet on its
+ * own dim, Neumann on the others, scalar values) and no residual fields or
+ * rho g, 2D iterations run in pairs, one pass each with the batch applied
+ * in-kernel between the two; CHMY_ITER_PAIRS (in `flags`) asks for the same
+ * in 3D.  A pair writes set 1-p like one iteration, so the parity flips once
+ * per pair; results are bitwise those of single iterations. */
 int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const chmy_field_t* tau0,
                         const chmy_field_t* Pr0, const chmy_field_t* V1, const chmy_field_t* tau1,
                         const chmy_field_t* Pr1, const chmy_field_t* tau_old, const chmy_field_t* eta,

@@ -1,10 +1,11 @@
-"""Two PT iterations per pass (2D, the free-slip V batch applied in-kernel
+"""Two PT iterations per pass (2D and 3D, the free-slip V batch applied in-kernel
 between them) against two single fused iterations with chmy_bc_batchset
 between and after them, bitwise, on seeded random storage.
 
-chmy_stokes_iterate takes iterations in pairs when the V batch is the 2D
-free-slip batch (Vx Dirichlet in x, Neumann in y; Vy Neumann in x, Dirichlet
-in y; scalar values) and no residual fields or rho g are requested; the
+chmy_stokes_iterate takes iterations in pairs when the V batch is the
+free-slip batch (each component Dirichlet on its own dim, Neumann on the
+others; scalar values) and no residual fields or rho g are requested (in 3D
+when CHMY_ITER_PAIRS asks for it); the
 reference here is the same entry point's single-iteration path, driven one
 iteration at a time (chmy_stokes_iter + chmy_bc_batchset), which
 tests/test_gpu_fused.py pins to the oracle.  Grids cover partial x tiles and
@@ -24,12 +25,22 @@ from chmy.stokes import pt_params
 
 pytestmark = pytest.mark.gpu
 
-ITER_EXACT = 2
+ITER_EXACT, ITER_PAIRS = 2, 4
 
 
 @pytest.fixture(scope="module")
 def arch():
     return chmy.Arch(chmy.MI355XBackend())
+
+
+def _batch3(v):
+    """(counts, comp, conds) for the 3D free-slip batch, v[component][dim][side]."""
+    D, N, S = _lib.DIRICHLET, _lib.NEUMANN, _lib.VALUE_SCALAR
+    rows = [(c, D if c == d else N, v[c][d][s]) for d in range(3) for s in range(2) for c in range(3)]
+    counts = (C.c_int32 * 6)(*([3] * 6))
+    comp = (C.c_int32 * 18)(*[c for c, _, _ in rows])
+    conds = (_lib.ChmyBC * 18)(*[_lib.ChmyBC(k, S, float(x), None) for _, k, x in rows])
+    return counts, comp, conds
 
 
 def _batch(vals):
@@ -82,7 +93,7 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
             fill(f, -0.5, 0.5)
     divV = chmy.Field(arch, g, Cc(), dt)
     rV = chmy.VectorField(arch, g, dt)
-    counts, comp, conds = _batch(vals)
+    counts, comp, conds = _batch(vals) if g.nd == 2 else _batch3(vals)
     G, dtp, eta_s = 1.0, 0.37, 0.8
     eve = 1.0 / (1.0 / eta_s + 1.0 / (G * dtp))
     p = pt_params(g)
@@ -95,8 +106,9 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
                          _arr(list(fV)) if fV is not None else None,
                          C.byref(fP.desc()) if fP is not None else None, C.byref(divV.desc()), _arr(list(rV)))
     scal = (eta_s, eve, G, dtp, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
-    if pairs:
-        _lib.call("chmy_stokes_iterate", *args(sets[0], sets[1]), counts, comp, conds, niter, flags, flags, *scal,
+    if pairs:   # 2D takes pairs by itself, 3D on request
+        fp = flags | (ITER_PAIRS if g.nd == 3 else 0)
+        _lib.call("chmy_stokes_iterate", *args(sets[0], sets[1]), counts, comp, conds, niter, fp, fp, *scal,
                   C.byref(par), chmy.current_stream())
     else:
         for _ in range(niter):
@@ -114,7 +126,8 @@ GENERAL = ((0.7, -1.3), (0.25, -0.6), (-0.4, 0.9), (0.3, -0.2))
 
 
 def _compare(a, b, what):
-    names = ["Vx", "Vy", "txx", "tyy", "txy", "Pr"]
+    names = ["Vx", "Vy", "txx", "tyy", "txy", "Pr"] if len(a) == 6 else \
+        ["Vx", "Vy", "Vz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "Pr"]
     for n, x, y in zip(names, a, b):
         bits = np.int64 if x.dtype == np.float64 else np.int32
         if not np.array_equal(x.view(bits), y.view(bits)):
@@ -142,4 +155,36 @@ def test_pair_pass_with_correction_forcing(arch, T, forcing, exact):
     g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=(70, 45))
     a = _run(arch, g, T, exact, "field", forcing, GENERAL, 4, True, 12)
     b = _run(arch, g, T, exact, "field", forcing, GENERAL, 4, False, 12)
+    _compare(a, b, f"forcing {forcing}")
+
+
+SHEAR3 = [[[1.0, -1.0], [0.0, 0.0], [0.0, 0.0]],
+          [[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]],
+          [[0.0, 0.0], [0.0, 0.0], [-1.0, 1.0]]]
+GENERAL3 = [[[0.7, -1.3], [0.25, -0.6], [0.1, 0.35]],
+            [[-0.4, 0.9], [0.3, -0.2], [-0.15, 0.05]],
+            [[0.2, 0.45], [-0.3, 0.6], [0.8, -0.5]]]
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("visc", ["field", "scalar"])
+@pytest.mark.parametrize("dims", [(61, 13, 17), (20, 14, 35), (5, 4, 3), (2, 2, 2)])
+def test_pair_pass_3d_equals_single_iterations(arch, T, exact, visc, dims):
+    """3D: x spans two 60-column tiles, y several 5-row tiles, z several
+    16-plane chunks, down to the smallest grid the pass takes."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=dims)
+    for vals in (SHEAR3, GENERAL3):
+        for niter in (2, 3):
+            a = _run(arch, g, T, exact, visc, None, vals, niter, True, 21)
+            b = _run(arch, g, T, exact, visc, None, vals, niter, False, 21)
+            _compare(a, b, f"dims {dims} niter {niter}")
+
+
+@pytest.mark.parametrize("T,forcing", [(np.float64, "same"), (np.float32, "same"), (np.float32, "bf16")])
+@pytest.mark.parametrize("exact", [True, False])
+def test_pair_pass_3d_with_correction_forcing(arch, T, forcing, exact):
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=(61, 13, 17))
+    a = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, True, 22)
+    b = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, False, 22)
     _compare(a, b, f"forcing {forcing}")
