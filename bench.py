@@ -264,6 +264,87 @@ def stokes2d_rates(args):
     print(json.dumps(out))
 
 
+def diffusion_c1(args):
+    """Config 1 (examples/diffusion_2d.jl form, 256^2, 1000 steps, FP64) on the
+    GPU: the reference's launch sequence (compute_q!, update_C! with the
+    Neumann batch, through the Launcher), the fused step (chmy_diffusion_step
+    into a second buffer + the batch, through the Launcher) and the whole loop
+    in one native call (chmy_diffusion_iterate), each checked bitwise against
+    the oracle's 1000 steps, which are also timed on the host (the CPU backend
+    the config names)."""
+    import numpy as np
+    import torch
+    import chmy
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n, steps = 256, 1000
+    arch = chmy.Arch(chmy.MI355XBackend())
+    g = chmy.UniformGrid(arch, origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(n, n))
+    og = O.Grid(g.origin, g.extent, g.dims)
+    rng = np.random.default_rng(1234)
+    xs = og.coords(O.CENTER, 0)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    C0 = np.exp(-(X ** 2 + Y ** 2) / 0.1) + 1e-3 * rng.random((n, n))
+    dt = min(chmy.spacing(g)) ** 2 / 1.0 / 4.1
+    # oracle (host)
+    hC = O.HField(og, O.CENTER)
+    hq = [O.HField(og, (O.VERTEX, O.CENTER)), O.HField(og, (O.CENTER, O.VERTEX))]
+    hC.set_interior(C0)
+    O.bc_apply(og, [(hC, O.Neumann())])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.compute_q(og, hq, hC, 1.0)
+        O.update_C(og, hC, hq, dt)
+        O.bc_apply(og, [(hC, O.Neumann())])
+    t_cpu = time.perf_counter() - t0
+    L = chmy.Launcher(arch, g, sync=False)
+    res = {}
+    for mode in ("reference_launches", "fused_step", "native_loop"):
+        Cf = chmy.Field(arch, g, chmy.Center())
+        C2 = chmy.Field(arch, g, chmy.Center())
+        q = chmy.VectorField(arch, g)
+        chmy.set_(Cf, C0)
+        bcs = chmy.batch(g, (Cf, chmy.Neumann()))
+        bcs2 = chmy.batch(g, (C2, chmy.Neumann()))
+        chmy.bc_(arch, g, bcs)
+        C2.data.copy_(Cf.data)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        if mode == "reference_launches":
+            for _ in range(steps):
+                L(arch, g, (chmy.kernels.compute_q, (q, Cf, 1.0, g)))
+                L(arch, g, (chmy.kernels.update_C, (Cf, q, dt, g)), bc=bcs)
+            out = Cf
+        elif mode == "fused_step":
+            cur, nxt = (Cf, bcs), (C2, bcs2)
+            for _ in range(steps):
+                L(arch, g, (chmy.kernels.diffusion_step, (cur[0], nxt[0], q, 1.0, dt, g)), bc=nxt[1])
+                cur, nxt = nxt, cur
+            out = cur[0]
+        else:   # the whole loop in one native call
+            import ctypes as C
+            from chmy import _lib
+            from chmy.kernels import _arr
+            counts = (C.c_int32 * 6)(1, 1, 1, 1, 0, 0)
+            conds = (_lib.ChmyBC * 4)(*[_lib.ChmyBC(_lib.NEUMANN, _lib.VALUE_ZERO, 0.0, None) for _ in range(4)])
+            par = C.c_int(0)
+            _lib.call("chmy_diffusion_iterate", C.byref(g.desc()), C.byref(Cf.desc()), C.byref(C2.desc()), _arr(q),
+                      counts, conds, 1.0, dt, steps, C.byref(par), chmy.current_stream())
+            out = (Cf, C2)[par.value]
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        bitwise = bool(np.array_equal(out.data.cpu().numpy(), hC.arr))
+        res[mode] = {"ms_1000_steps": round(ms, 3), "bitwise_vs_oracle": bitwise}
+    out = {"metric": "config 1 wall time, 1000 steps", "value": res["native_loop"]["ms_1000_steps"], "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+           "config": {"workload": "2D linear diffusion, 256^2 cells, 1000 steps, Neumann(0)"},
+           "runs": res, "cpu_oracle_ms_1000_steps": round(t_cpu * 1e3, 1),
+           "cpu_cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    print(json.dumps(out))
+
+
 def sia_gradient(args):
     """SIA forward run versus gradient (checkpointed forward + reverse sweep)
     wall time at 512² (300 steps, 30 checkpoints, fixed Δt), and the gradient
@@ -460,6 +541,8 @@ def main():
     ap.add_argument("--fd-eps", type=float, default=1e-11)
     ap.add_argument("--stokes2d", action="store_true",
                     help="2D fused PT iteration rates at --n (default 2047): FP64, FP32, defect-correction sweep")
+    ap.add_argument("--diffusion", action="store_true",
+                    help="config 1: 2D diffusion 256^2 x 1000 steps, reference launches vs fused step, vs the oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -475,6 +558,9 @@ def main():
         return
     if args.stokes_adjoint:
         stokes_gradient(args)
+        return
+    if args.diffusion:
+        diffusion_c1(args)
         return
     if args.stokes2d:
         if args.n == 255:

@@ -61,6 +61,8 @@ SIGNATURES = {
     "chmy_op": [C.c_int, PG, PB, PF, C.c_int, PF, C.c_int, VP],
     "chmy_diffusion_flux": [PG, PB, PF, PF, D, VP],
     "chmy_diffusion_update": [PG, PB, PF, PF, D, VP],
+    "chmy_diffusion_step": [PG, PB, PF, PF, PF, D, D, VP],
+    "chmy_diffusion_iterate": [PG, PF, PF, PF, P(C.c_int32), P(ChmyBC), D, D, C.c_int, P(C.c_int), VP],
     "chmy_stokes_update_old": [PG, PB, PF, PF, PF, PF, VP],
     "chmy_stokes_stress": [PG, PB, PF, PF, PF, PF, PF, PF, D, D, D, D, D, D, VP],
     "chmy_stokes_velocity": [PG, PB, PF, PF, PF, PF, PF, PF, D, D, D, D, VP],

@@ -5,6 +5,7 @@ Each object stands where the reference's `@kernel` function stands in
 the same argument list the reference kernel takes, and it calls the C ABI.
 
   compute_q, update_C              examples/diffusion_2d.jl:8-19
+  diffusion_step                   both in one launch, double-buffered C (extension)
   update_old, update_stress,
   update_velocity, update_thermal_flux,
   update_thermal                   examples/stokes_3d_inc_ve_T.jl:11-77 (2D: stokes_2d_inc_ve_T.jl:11-60)
@@ -57,6 +58,11 @@ def _compute_q(grid, box, q, Cf, chi, g):
 def _update_C(grid, box, Cf, q, dt, g):
     _lib.call("chmy_diffusion_update", C.byref(grid.desc()), C.byref(box), C.byref(Cf.desc()), _arr(q),
               float(dt), current_stream())
+
+
+def _diffusion_step(grid, box, Cf, C_out, q, chi, dt, g):
+    _lib.call("chmy_diffusion_step", C.byref(grid.desc()), C.byref(box), C.byref(Cf.desc()), C.byref(C_out.desc()),
+              _arr(q), float(chi), float(dt), current_stream())
 
 
 def _update_old(grid, box, T, tau, T_old, tau_old):
@@ -113,6 +119,8 @@ def _op(code):
 
 compute_q = HipKernel("compute_q!", _compute_q)
 update_C = HipKernel("update_C!", _update_C)
+# compute_q! + update_C! in one launch into a second buffer (build extension)
+diffusion_step = HipKernel("compute_q! + update_C!", _diffusion_step)
 update_old = HipKernel("update_old!", _update_old)
 update_stress = HipKernel("update_stress!", _update_stress)
 update_velocity = HipKernel("update_velocity!", _update_velocity)

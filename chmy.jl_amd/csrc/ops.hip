@@ -123,6 +123,36 @@ __global__ void __launch_bounds__(256) diffusion_update_kernel(Box3 b, DFL<T> C,
     g_at(C, I) = g_at(C, I) - dt * dv;
 }
 
+// compute_q! and update_C! in one launch (build extension): q at I from C at
+// I, I-e (stored, as compute_q! stores it), q at I+e recomputed the same way
+// when I+e is inside the box, else read from q's storage (what update_C! would
+// read there); C_out[I] = C[I] - Δt divg(q)[I].  C and C_out are distinct
+// buffers, so no thread reads a C value another has advanced.
+template <class T>
+__global__ void __launch_bounds__(256) diffusion_step_kernel(Box3 b, DFL<T> C, DFL<T> Co, DFL<T> q0, DFL<T> q1,
+                                                             DFL<T> q2, int nd, T nchi, T dt, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const T c = g_at(C, I);
+    const int64_t hi[3] = {b.lo[0] + b.len[0], b.lo[1] + b.len[1], b.lo[2] + b.len[2]};
+    const int64_t at[3] = {I.i, I.j, I.k};
+    const T isp[3] = {i0, i1, i2};
+    T dv = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= nd) break;
+        const DFL<T>& qd = d == 0 ? q0 : (d == 1 ? q1 : q2);
+        const T ql = nchi * ((c - g_at(C, sh(I, d, -1))) * isp[d]);
+        g_at(qd, I) = ql;
+        const Ix3 J = sh(I, d, 1);
+        const T qr = at[d] + 1 < hi[d] ? nchi * ((g_at(C, J) - c) * isp[d]) : g_at(qd, J);
+        const T t = (qr - ql) * isp[d];
+        dv = d == 0 ? t : dv + t;
+    }
+    g_at(Co, I) = c - dt * dv;
+}
+
 // ------------------------------------------------------------------ update_old!
 template <class T>
 struct Copy7 {
@@ -313,6 +343,62 @@ int chmy_diffusion_update(const chmy_grid_t* g, const chmy_box_t* box, const chm
     CHMY_TRY(check_vector(g, q, "q", C->dtype));
     return C->dtype == CHMY_F64 ? diffusion_launch<double>(g, box, q, C, dt, false, (hipStream_t)stream)
                                 : diffusion_launch<float>(g, box, q, C, dt, false, (hipStream_t)stream);
+}
+
+int chmy_diffusion_step(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* C,
+                        const chmy_field_t* C_out, const chmy_field_t* q, double chi, double dt, void* stream) {
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_center(g, C, "C", -1));
+    CHMY_TRY(check_center(g, C_out, "C_out", C->dtype));
+    CHMY_TRY(check_vector(g, q, "q", C->dtype));
+    if (C->data == C_out->data) return set_error(CHMY_E_ARG, "C and C_out must be distinct buffers");
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    Box3 b = box3(bx, g->nd);
+    auto launch = [&](auto zero) -> int {
+        using T = decltype(zero);
+        DFL<T> c = lview<T>(*C), co = lview<T>(*C_out);
+        DFL<T> q0 = lview<T>(q[0]);
+        DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
+        dim3 grd = grid_for(b, blk2());
+        hipLaunchKernelGGL(diffusion_step_kernel<T>, grd, blk2(), 0, (hipStream_t)stream, b, c, co, q0, q1, q2,
+                           g->nd, -T(chi), T(dt), T(g->inv_spacing[0]), T(g->inv_spacing[1]),
+                           T(g->inv_spacing[2]));
+        CHMY_HIP(hipGetLastError());
+        return 0;
+    };
+    return C->dtype == CHMY_F64 ? launch(0.0) : launch(0.0f);
+}
+
+int chmy_diffusion_iterate(const chmy_grid_t* g, const chmy_field_t* C0, const chmy_field_t* C1,
+                           const chmy_field_t* q, const int32_t* bc_counts, const chmy_bc_t* bcs, double chi,
+                           double dt, int niter, int* parity_inout, void* stream) {
+    CHMY_TRY(check_grid(g));
+    if (!parity_inout || (*parity_inout != 0 && *parity_inout != 1)) return set_error(CHMY_E_ARG, "parity must be 0/1");
+    if (niter < 0) return set_error(CHMY_E_ARG, "niter < 0");
+    if (!bc_counts || !bcs) return set_error(CHMY_E_ARG, "C batch is NULL");
+    int total = 0;
+    for (int k = 0; k < 2 * g->nd; ++k) {
+        if (bc_counts[k] < 0 || bc_counts[k] > 1) return set_error(CHMY_E_ARG, "one C entry per (dim, side) at most");
+        total += bc_counts[k];
+    }
+    const chmy_field_t* Cs[2] = {C0, C1};
+    chmy_field_t fset[2][6];
+    for (int s = 0; s < 2; ++s)
+        for (int k = 0; k < total; ++k) fset[s][k] = *Cs[s];
+    chmy_box_t box;
+    for (int d = 0; d < 3; ++d) {
+        box.lo[d] = 0;
+        box.len[d] = d < g->nd ? g->n[d] + 2 : 1;
+    }
+    int p = *parity_inout;
+    for (int it = 0; it < niter; ++it) {
+        CHMY_TRY(chmy_diffusion_step(g, &box, Cs[p], Cs[1 - p], q, chi, dt, stream));
+        CHMY_TRY(chmy_bc_batchset(g, bc_counts, fset[1 - p], bcs, stream));
+        p = 1 - p;
+    }
+    *parity_inout = p;
+    return 0;
 }
 
 int chmy_stokes_update_old(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* Tf,

@@ -139,6 +139,21 @@ int chmy_diffusion_flux(const chmy_grid_t* grid, const chmy_box_t* box, const ch
 /* update_C! (examples/diffusion_2d.jl:15-19): C[I] -= Δt divg(q)[I]. */
 int chmy_diffusion_update(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* C,
                           const chmy_field_t* q, double dt, void* stream);
+/* compute_q! then update_C! (examples/diffusion_2d.jl:8-19, the loop body at
+ * :44-48) in one launch over the same box, writing C_out (a second buffer,
+ * distinct from C) instead of updating C in place; q is stored as compute_q!
+ * stores it.  C_out and q equal the two launches' C and q bitwise (build
+ * extension: the reference launches the two kernels). */
+int chmy_diffusion_step(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* C,
+                        const chmy_field_t* C_out, const chmy_field_t* q, double chi, double dt, void* stream);
+/* niter steps of the diffusion loop (examples/diffusion_2d.jl:44-48) in one
+ * call: chmy_diffusion_step over the Launcher box from buffer p =
+ * *parity_inout into 1-p, then the C batch (bc_counts[2*dim+side-1] <= 1
+ * entries, the conditions in bcs, as chmy_bc_batchset) on 1-p.  On return
+ * *parity_inout names the buffer holding C (build extension). */
+int chmy_diffusion_iterate(const chmy_grid_t* grid, const chmy_field_t* C0, const chmy_field_t* C1,
+                           const chmy_field_t* q, const int32_t* bc_counts, const chmy_bc_t* bcs, double chi,
+                           double dt, int niter, int* parity_inout, void* stream);
 
 /* ---------------------------------------------------------------- Stokes + T
  * tau arrays: 2D (xx, yy, xy), 3D (xx, yy, zz, xy, xz, yz) at the locations of

@@ -251,6 +251,82 @@ def test_diffusion_2d_256_1000_steps(arch):
     assert_same(q.x, hq[0], "qx")
 
 
+def test_diffusion_iterate_256_1000_steps(arch):
+    """The same config-1 loop as one native call (chmy_diffusion_iterate:
+    fused step into the other buffer + Neumann batch, 1000 times): every
+    stored value bitwise equal to the oracle's loop."""
+    import ctypes as C
+    from chmy import _lib
+    from chmy.kernels import _arr
+    n, steps = 256, 1000
+    g = chmy.UniformGrid(arch, origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(n, n))
+    og = ogrid(g)
+    rng = np.random.default_rng(1234)
+    xs = og.coords(O.CENTER, 0)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    C0 = np.exp(-(X ** 2 + Y ** 2) / 0.1) + 1e-3 * rng.random((n, n))
+    Cs = [chmy.Field(arch, g, Cc()), chmy.Field(arch, g, Cc())]
+    q = chmy.VectorField(arch, g)
+    hC = O.HField(og, O.CENTER)
+    hq = [O.HField(og, (O.VERTEX, O.CENTER)), O.HField(og, (O.CENTER, O.VERTEX))]
+    chmy.set_(Cs[0], C0)
+    chmy.bc_(arch, g, (Cs[0], chmy.Neumann()))
+    Cs[1].data.copy_(Cs[0].data)
+    hC.set_interior(C0)
+    O.bc_apply(og, [(hC, O.Neumann())])
+    dt = min(chmy.spacing(g)) ** 2 / 1.0 / 4.1
+    counts = (C.c_int32 * 6)(1, 1, 1, 1, 0, 0)
+    bcs = (_lib.ChmyBC * 4)(*[_lib.ChmyBC(_lib.NEUMANN, _lib.VALUE_ZERO, 0.0, None) for _ in range(4)])
+    par = C.c_int(0)
+    _lib.call("chmy_diffusion_iterate", C.byref(g.desc()), C.byref(Cs[0].desc()), C.byref(Cs[1].desc()), _arr(q),
+              counts, bcs, 1.0, dt, steps, C.byref(par), chmy.current_stream())
+    for _ in range(steps):
+        O.compute_q(og, hq, hC, 1.0)
+        O.update_C(og, hC, hq, dt)
+        O.bc_apply(og, [(hC, O.Neumann())])
+    chmy.synchronize()
+    assert par.value == 0
+    assert_same(Cs[par.value], hC, "C after 1000 steps")
+    assert_same(q.x, hq[0], "qx")
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_diffusion_step_equals_two_kernels(arch, T, nd):
+    """chmy_diffusion_step (compute_q! + update_C! in one launch into a second
+    buffer) gives the stored q and the new C of the two launches, bitwise, on
+    random storage over the Launcher box and a sub-box."""
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(40 + nd)
+    Cf, hC = pair(arch, g, og, (Cc(),) * nd, T, rng=rng)
+    qs = [pair(arch, g, og, vloc(D, nd), T, rng=rng) for D in range(nd)]
+    dt = 0.37 * min(chmy.spacing(g)) ** 2
+    for box in (None, ((1,) * nd, tuple(n - 1 for n in g.dims))):
+        Co = chmy.Field(arch, g, Cc(), Cf.data.dtype)
+        q = [chmy.Field(arch, g, f.loc, f.data.dtype) for f, _ in qs]
+        for a, (b, _) in zip(q, qs):
+            a.data.copy_(b.data)
+        lo, ws = ((0,) * nd, tuple(n + 2 for n in g.dims)) if box is None else box
+        chmy.kernels.diffusion_step.launch(g, lo, ws, Cf, Co, q, 1.3, dt, g)
+        # reference: the two launches on copies
+        C2 = chmy.Field(arch, g, Cc(), Cf.data.dtype)
+        C2.data.copy_(Cf.data)
+        q2 = [chmy.Field(arch, g, f.loc, f.data.dtype) for f, _ in qs]
+        for a, (b, _) in zip(q2, qs):
+            a.data.copy_(b.data)
+        chmy.kernels.compute_q.launch(g, lo, ws, q2, C2, 1.3, g)
+        chmy.kernels.update_C.launch(g, lo, ws, C2, q2, dt, g)
+        chmy.synchronize()
+        # C_out holds the new values on the box and nothing elsewhere (zeros)
+        got, want = dev(Co), dev(C2)
+        sl = tuple(slice(1 + l, 1 + l + w) for l, w in zip(reversed(lo), reversed(ws)))
+        bits = np.int64 if T == np.float64 else np.int32
+        assert np.array_equal(got[sl].view(bits), want[sl].view(bits)), f"box {box}"
+        for a, b in zip(q, q2):
+            assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"q box {box}"
+
+
 @pytest.mark.parametrize("ow", [(16, 8), (3, 5)])
 def test_launcher_outer_width_matches(arch, ow):
     """Launcher(outer_width): inner box + 2N slabs with per-side bc
