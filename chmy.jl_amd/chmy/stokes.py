@@ -166,14 +166,16 @@ class PTStokes:
             self._vb = (counts, (C.c_int32 * n)(*comp), (_lib.ChmyBC * n)(*conds), keep)
         return self._vb
 
-    def iterate(self, niter, residual_last=False):
+    def iterate(self, niter, residual_last=False, pairs=False):
         """niter PT iterations in one native call (fused kernel + V batch,
-        buffers alternating); (V, tau, Pr) hold the result afterwards."""
+        buffers alternating); (V, tau, Pr) hold the result afterwards.  2D
+        runs them in pairs (one pass per two iterations); `pairs` asks for
+        the same in 3D (bitwise the same results)."""
         if niter <= 0:
             return
         g = self.grid
         counts, comp, conds, _keep = self._vbatch()
-        base = ITER_EXACT if self.exact else 0
+        base = (ITER_EXACT if self.exact else 0) | (ITER_PAIRS if pairs else 0)
         par = C.c_int(0)
         _lib.call("chmy_stokes_iterate", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
                   _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(self.tau_old),

@@ -934,18 +934,17 @@ constexpr int kTB3Z = CHMY_FUSED_TB3_ZCHUNK;
 // Results are bitwise those of two fused3d_kernel launches with the batch
 // between them.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
-__global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
+__global__ void __launch_bounds__(kFX * kTB3Y<T>, (kTB3Y<T> + 3) / 4) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
     static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
     constexpr int BY = kTB3Y<T>, NR = BY + 1, TW = kFX;
     constexpr bool HF = OPT == 1;
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
     __shared__ T s_vx[NR][TW], s_vy[2][NR][TW], s_vz1[NR][TW];
-    __shared__ T s_eta[FVISC ? 4 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 4 : 1][FVISC ? BY : 1][TW];
+    __shared__ T s_eta[FVISC ? 4 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][TW];
     __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW], s_tyz[2][BY][TW];
     __shared__ T b_vx[NR][TW], b_vy[2][NR][TW], b_vz1[NR][TW];
     __shared__ T b_P[BY][TW], b_tyy[BY][TW], b_txy[BY][TW], b_tyz[2][BY][TW];
-    __shared__ T s_xc[2][NR][TW];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int nb = a.ntx * a.nty * a.nchunks;
@@ -1001,18 +1000,20 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
         T az[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) az[c] = zrule(c, q, rm[c], r0[c], rp[c]);
-        s_xc[0][ty][tx] = az[0];
-        s_xc[1][ty][tx] = az[2];
+        // the exchange goes through B's Pr / tau_yy planes, free until B's stresses
+        b_P[ty][tx] = az[0];
+        b_tyy[ty][tx] = az[2];
         __syncthreads();
+        const int tyx = ty + 1 < BY ? ty + 1 : ty;   // the last row is past every output
         const bool iq_in = i >= 0 && i <= n0 + 2 && q >= 0 && q <= n2 + 2;   // y rule's range
         const bool jq_in = j >= 0 && j <= n1 + 2 && q >= 0 && q <= n2 + 2;   // x rule's range
         T ay[3];
 #pragma unroll
         for (int c = 0; c < 3; c += 2) {   // Vx, Vz: Neumann in y
-            const int s = c == 0 ? 0 : 1;
+            const T(*xc)[TW] = c == 0 ? b_P : b_tyy;
             T v = az[c];
-            if (iq_in && j == 0) v = fma(hy, -T(fs.v[c][1][0]), s_xc[s][typ][tx]);
-            else if (iq_in && j == n1 + 1) v = fma(hy, T(fs.v[c][1][1]), s_xc[s][tym][tx]);
+            if (iq_in && j == 0) v = fma(hy, -T(fs.v[c][1][0]), xc[tyx][tx]);
+            else if (iq_in && j == n1 + 1) v = fma(hy, T(fs.v[c][1][1]), xc[tym][tx]);
             ay[c] = v;
         }
         ay[1] = az[1];   // Vy: Dirichlet on its y nodes
@@ -1099,7 +1100,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             s_vz1[ty][tx] = vz_k1;
             if constexpr (FVISC) {
                 s_eta[e1][ty][tx] = eta_k1;
-                s_eml[e0][ty][tx] = eml_k;
+                s_eml[pa][ty][tx] = eml_k;
             }
             if (last) {
                 s_vx[BY][tx] = vx8;
@@ -1169,8 +1170,8 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             }
             T ex, ey, ez;
             if constexpr (FVISC) {
-                ex = AR::vcoef_field(jmax(s_eml[e0][ty][txm], eml_k), K);
-                ey = AR::vcoef_field(jmax(s_eml[e0][tym][tx], eml_k), K);
+                ex = AR::vcoef_field(jmax(s_eml[pa][ty][txm], eml_k), K);
+                ey = AR::vcoef_field(jmax(s_eml[pa][tym][tx], eml_k), K);
                 ez = AR::vcoef_field(jmax(Aeml_m, eml_k), K);
             } else {
                 ex = ey = ez = AR::vcoef_scalar(K);
@@ -1229,7 +1230,11 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
         {
             const uint32_t oC = zo(r, LC);
             const T eta_k = FVISC ? s_eta[f0][ty][tx] : T(0), eta_k1 = FVISC ? s_eta[f1][ty][tx] : T(0);
-            const T eml_k = FVISC ? s_eml[f0][ty][tx] : T(0);
+            T eml_k = T(0), eml_d = T(0);   // eta_ml at plane r: own, row j-1
+            if constexpr (FVISC) {
+                eml_k = ld(a.eml, oC);
+                eml_d = ld(a.eml, jc > -1 ? oC - dy[LC] : oC);
+            }
             T P = rP[2], txx = rtxx[2], tyy = rtyy[2], tzz = rtzz[2];
             T txy1 = rtxy[2], txz1 = rtxz[2], tyz1 = rtyz[2];
             const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
@@ -1319,11 +1324,12 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 ry = ry + fVy;
                 rz = rz + fVz;
             }
+            const T emll = shup(eml_k);   // every lane takes part in the shuffle
             if (st && ownc) {
                 T ex, ey, ez;
                 if constexpr (FVISC) {
-                    ex = AR::vcoef_field(jmax(s_eml[f0][ty][txm], eml_k), K);
-                    ey = AR::vcoef_field(jmax(s_eml[f0][tym][tx], eml_k), K);
+                    ex = AR::vcoef_field(jmax(emll, eml_k), K);
+                    ey = AR::vcoef_field(jmax(eml_d, eml_k), K);
                     ez = AR::vcoef_field(jmax(Beml_m, eml_k), K);
                 } else {
                     ex = ey = ez = AR::vcoef_scalar(K);
