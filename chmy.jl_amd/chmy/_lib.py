@@ -52,6 +52,9 @@ SIGNATURES = {
     "chmy_device_count": [P(C.c_int)],
     "chmy_malloc_zeros": [P(C.c_void_p), C.c_size_t, VP],
     "chmy_free": [VP],
+    "chmy_memset_zero": [VP, C.c_size_t, VP],
+    "chmy_stream_create": [C.c_int, P(C.c_void_p)],
+    "chmy_stream_destroy": [VP],
     "chmy_memcpy_h2d": [VP, VP, C.c_size_t, VP],
     "chmy_memcpy_d2h": [VP, VP, C.c_size_t, VP],
     "chmy_stream_sync": [VP],

@@ -3,6 +3,8 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace chmy {
@@ -110,6 +112,22 @@ int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name
 
 using namespace chmy;
 
+// Zero `bytes` bytes with vector stores (16 B per thread where aligned, a
+// byte loop for the unaligned head and tail), grid-stride: a fill kernel, so a
+// captured graph records a kernel node rather than a memset node.
+__global__ void __launch_bounds__(256) zero_bytes_kernel(unsigned char* p, size_t bytes) {
+    const size_t head = std::min<size_t>(bytes, (16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15);
+    const size_t nv = (bytes - head) / 16;
+    uint4* v = reinterpret_cast<uint4*>(p + head);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < nv; t += stride) v[t] = make_uint4(0u, 0u, 0u, 0u);
+    if (t0 == 0) {
+        for (size_t b = 0; b < head; ++b) p[b] = 0;
+        for (size_t b = head + nv * 16; b < bytes; ++b) p[b] = 0;
+    }
+}
+
 extern "C" {
 
 const char* chmy_last_error(void) { return g_msg; }
@@ -135,6 +153,32 @@ int chmy_malloc_zeros(void** ptr, size_t bytes, void* stream) {
     if (!ptr) return set_error(CHMY_E_ARG, "ptr is NULL");
     CHMY_HIP(hipMalloc(ptr, bytes));
     CHMY_HIP(hipMemsetAsync(*ptr, 0, bytes, (hipStream_t)stream));
+    return 0;
+}
+
+int chmy_stream_create(int priority, void** stream) {
+    if (!stream) return set_error(CHMY_E_ARG, "stream is NULL");
+    if (priority != 0) return set_error(CHMY_E_ARG, "only the default stream priority (0) is supported");
+    hipStream_t s = nullptr;
+    CHMY_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return 0;
+}
+
+int chmy_stream_destroy(void* stream) {
+    if (!stream) return set_error(CHMY_E_ARG, "stream is NULL");
+    CHMY_HIP(hipStreamDestroy((hipStream_t)stream));
+    return 0;
+}
+
+int chmy_memset_zero(void* ptr, size_t bytes, void* stream) {
+    if (bytes == 0) return 0;
+    if (!ptr) return set_error(CHMY_E_ARG, "ptr is NULL");
+    const size_t nthreads = (bytes + 15) / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((nthreads + 255) / 256, 65535);
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<unsigned char*>(ptr), bytes);
+    CHMY_HIP(hipGetLastError());
     return 0;
 }
 

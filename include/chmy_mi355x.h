@@ -95,6 +95,15 @@ int chmy_device_count(int* count);
 /* KernelAbstractions.zeros(backend, T, dims) (field.jl:59): zero-filled. */
 int chmy_malloc_zeros(void** ptr, size_t bytes, void* stream);
 int chmy_free(void* ptr);
+/* Zero `bytes` bytes at device pointer `ptr` on `stream` with a fill kernel
+ * (KernelAbstractions.zeros' fill, field.jl:59), so it may sit in a captured
+ * HIP graph. */
+int chmy_memset_zero(void* ptr, size_t bytes, void* stream);
+/* A non-blocking stream for a Launcher worker task (KernelLaunch.jl:44,
+ * activate!(...; priority)); only the default priority (0) is supported, so
+ * the binding maps :high to 0. */
+int chmy_stream_create(int priority, void** stream);
+int chmy_stream_destroy(void* stream);
 int chmy_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int chmy_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 /* KernelAbstractions.synchronize(backend) (src/KernelLaunch.jl:117). */

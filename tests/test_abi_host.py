@@ -72,6 +72,35 @@ def test_validation_rejects_misuse_on_host():
     assert rc == -2
 
 
+def test_runtime_helpers_validate_on_host():
+    """Stream and fill helpers reject misuse before any HIP call; the fused
+    diffusion entry points check their buffers like the per-kernel ones."""
+    L = _lib.lib()
+    s = C.c_void_p()
+    assert L.chmy_stream_create(1, C.byref(s)) == -1 and b"priority" in L.chmy_last_error()
+    assert L.chmy_stream_create(0, None) == -1
+    assert L.chmy_stream_destroy(None) == -1
+    assert L.chmy_memset_zero(None, 16, None) == -1
+    assert L.chmy_memset_zero(None, 0, None) == 0        # nothing to do
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(8, 6))
+    f = _lib.ChmyField()
+    f.data, f.nd, f.dtype = 0x1000, 2, _lib.F64
+    for d, n in enumerate((8, 6)):
+        f.n[d], f.halo[d], f.loc[d] = n, 1, 0
+    q = (_lib.ChmyField * 2)()
+    for D in range(2):
+        q[D].data, q[D].nd, q[D].dtype = 0x2000, 2, _lib.F64
+        for d, n in enumerate((8, 6)):
+            q[D].n[d], q[D].halo[d], q[D].loc[d] = n + (d == D), 1, int(d == D)
+    rc = L.chmy_diffusion_step(C.byref(g.desc()), None, C.byref(f), C.byref(f), q, 1.0, 0.1, None)
+    assert rc == -1 and b"distinct" in L.chmy_last_error()
+    par = C.c_int(2)
+    counts = (C.c_int32 * 6)()
+    rc = L.chmy_diffusion_iterate(C.byref(g.desc()), C.byref(f), C.byref(f), q, counts, None, 1.0, 0.1, 1,
+                                  C.byref(par), None)
+    assert rc == -1 and b"parity" in L.chmy_last_error()
+
+
 def test_adjoint_entry_points_validate_on_host():
     """SIA run/reverse and the Stokes gradient pass reject misuse before any
     launch: missing arrays, aliasing, checkpoint counts, wrong locations."""

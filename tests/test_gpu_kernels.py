@@ -251,6 +251,23 @@ def test_diffusion_2d_256_1000_steps(arch):
     assert_same(q.x, hq[0], "qx")
 
 
+def test_stream_and_zero_fill(arch):
+    """chmy_stream_create/destroy and chmy_memset_zero (unaligned head and
+    tail, 16-byte body) on a device buffer."""
+    import ctypes as C
+    import torch
+    from chmy import _lib
+    s = C.c_void_p()
+    _lib.call("chmy_stream_create", 0, C.byref(s))
+    buf = torch.full((4099,), 0x5A, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    _lib.call("chmy_memset_zero", C.c_void_p(buf.data_ptr() + 3), 4000, s)
+    _lib.call("chmy_stream_sync", s)
+    h = buf.cpu().numpy()
+    assert (h[:3] == 0x5A).all() and (h[3:4003] == 0).all() and (h[4003:] == 0x5A).all()
+    _lib.call("chmy_stream_destroy", s)
+
+
 def test_diffusion_iterate_256_1000_steps(arch):
     """The same config-1 loop as one native call (chmy_diffusion_iterate:
     fused step into the other buffer + Neumann batch, 1000 times): every
