@@ -5,14 +5,20 @@
 // Work layout (3D): a block is 64 (x, one wave) x BY (y) threads; thread
 // (tx, ty) owns the logical column (i, j) = (i0-1+tx, j0-1+ty) and marches z
 // over a chunk [kb, ke).  The tile's outputs are tx in [1, 62], ty in
-// [1, BY-2]; the one-cell ring around them recomputes the stresses that the
-// velocity stencil needs (Center values at I-e, edge values at I+e), so an
-// advanced stress is never read back from HBM.  At z-step k a thread computes
-// the Center stresses and Pr at plane k, tau_xy at plane k and tau_xz, tau_yz
-// at z-vertex plane k+1, then V at plane k.  x/y neighbours come from LDS
-// planes; z neighbours are carried in registers.  Two barriers per z-step:
-// LDS planes whose readers and next writers are not separated by a barrier
-// are double-buffered by step parity.
+// [1, BY-1]: lanes 0 and 63 and row 0 recompute the stresses that the
+// velocity stencil needs (Center values at I-e, edge values at I+e), and the
+// last wave also stages row j+1 of V and eta, so an advanced stress is never
+// read back from HBM.  At z-step k a thread computes the Center stresses and
+// Pr at plane k, tau_xy at row j+1, tau_xz, tau_yz at z-vertex plane k+1,
+// then V at plane k.  x/y neighbours come from LDS planes; z neighbours are
+// carried in registers.  Two barriers per z-step: LDS planes whose readers
+// and next writers are not separated by a barrier are double-buffered by
+// step parity.
+//
+// 2D: fused2d_kernel marches y with one wave per 64 columns and no LDS (x
+// neighbours by lane shuffles).  Two-iteration passes (fused2d_tb_kernel,
+// fused3d_tb_kernel) run the step twice per pass with the free-slip V batch
+// applied in-kernel between the iterations; see their comments.
 //
 // Addressing: the 7 staggered locations (Center, Vx, Vy, Vz, xy, xz, yz edges)
 // each have their own padded strides; a thread keeps one 32-bit byte offset per
