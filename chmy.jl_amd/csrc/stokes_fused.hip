@@ -662,13 +662,13 @@ constexpr int kTBC = (EXACT && sizeof(T) == 8) ? 8 : CHMY_FUSED_TB_CHUNK;
 // at row a + 1), as fused2d_kernel's row.
 template <class T, bool BF>
 struct In2 {
-    T vx1, vy1, P, txx, tyy, txy1, eta1, eml;
+    T vx1, vy1, P, txx, tyy, txy1, eta1, eml, rg;
     Fraw<T, BF> oxx, oyy, oxy, fP, fVx, fVy;
 };
 
 template <class T, bool FVISC, bool BF>
 __device__ __forceinline__ void load_in2(const Fused2<T>& a, const uint32_t* qc, const uint32_t* qn, bool has_f,
-                                         In2<T, BF>& r) {
+                                         bool has_rg, In2<T, BF>& r) {
     r.vx1 = ldo(a.Vx, qn[L2X]);
     r.vy1 = ldo(a.Vy, qn[L2Y]);
     r.P = ldo(a.Pr, qc[L2C]);
@@ -687,6 +687,7 @@ __device__ __forceinline__ void load_in2(const Fused2<T>& a, const uint32_t* qc,
         r.fVx = ldr<T, BF>(a.fVx, qc[L2X]);
         r.fVy = ldr<T, BF>(a.fVy, qc[L2Y]);
     }
+    if (has_rg) r.rg = ldo(a.rg, qc[L2Y]);
 }
 
 // One iteration's state carried along the y-march, and its row step: the
@@ -698,9 +699,10 @@ struct Carry2 {
 };
 
 template <class T, bool FVISC, bool EXACT>
-__device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool in_j1, bool has_f, Carry2<T>& c,
-                                     T vx1, T vy1, T& P, T& txx, T& tyy, T& txy1, T oxx, T oyy, T oxy, T eta1,
-                                     T eml, T fP, T fVx, T fVy, T& vxn, T& vyn, T& txy_prev) {
+__device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool in_j1, bool has_f, bool has_rg,
+                                     Carry2<T>& c, T vx1, T vy1, T& P, T& txx, T& tyy, T& txy1, T oxx, T oyy,
+                                     T oxy, T eta1, T eml, T fP, T fVx, T fVy, T rg, T& vxn, T& vyn,
+                                     T& txy_prev) {
     using AR = Arith<T, EXACT>;
     const T exx = (shdn(c.vx_j) - c.vx_j) * ix;
     const T eyy = (vy1 - c.vy_j) * iy;
@@ -729,6 +731,7 @@ __device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool
     const T Pl = shup(P), txxl = shup(txx), txyr = shdn(c.txy_j), emll = FVISC ? shup(eml) : T(0);
     T rx = (-((P - Pl) * ix) + (txx - txxl) * ix) + (txy1 - c.txy_j) * iy;
     T ry = (-((P - c.P_jm1) * iy) + (tyy - c.tyy_jm1) * iy) + (txyr - c.txy_j) * ix;
+    if (has_rg) ry = ry - rg;
     if (has_f) {
         rx = rx + fVx;
         ry = ry + fVy;
@@ -764,9 +767,9 @@ __device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool
 // the batch's own order (dims y then x; the x condition writes last): see
 // bc.hip.  Results are bitwise those of two fused2d_kernel launches with the
 // batch between them.
-template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
+template <class T, bool FVISC, bool EXACT, int OPT, bool BF, bool RG>
 __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, FreeSlip2 fs) {
-    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
+    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only (rho g by RG)");
     constexpr bool HF = OPT == 1;
     const Kc<T>& K = a.k;
     const int lane = threadIdx.x;
@@ -826,12 +829,12 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
         A.vy_j = ldo(a.Vy, qc[L2Y]);
         A.eta_j = FVISC ? ldo(a.eta, qc[L2C]) : T(0);
         A.P_jm1 = A.tyy_jm1 = A.txy_j = A.eml_jm1 = T(0);
-        load_in2<T, FVISC, BF>(a, qc, qn, HF, in[0]);
+        load_in2<T, FVISC, BF>(a, qc, qn, HF, RG, in[0]);
     }
     B.P_jm1 = B.tyy_jm1 = B.txy_j = B.eml_jm1 = T(0);
     // A's results by step, as B reads them two steps later
     T rP[S], rtxx[S], rtyy[S], rtxy1[S], rvx[S], rvy[S];
-    T roxx[S], royy[S], roxy[S], reta1[S], reml[S], rfP[S], rfVx[S], rfVy[S];
+    T roxx[S], royy[S], roxy[S], reta1[S], reml[S], rfP[S], rfVx[S], rfVy[S], rrg[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) {
         const int ra = jb - 2 + t;
@@ -839,7 +842,7 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
             uint32_t qc[NLOC2], qn[NLOC2];
             rowoff(ra + 1, qc);
             rowoff(ra + 2, qn);
-            load_in2<T, FVISC, BF>(a, qc, qn, HF, in[(t + 1) & 1]);
+            load_in2<T, FVISC, BF>(a, qc, qn, HF, RG, in[(t + 1) & 1]);
         }
         __builtin_amdgcn_sched_barrier(0);
         const In2<T, BF>& cur = in[t & 1];
@@ -852,10 +855,11 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
             const T fP = HF ? cvf<T, BF>(cur.fP) : T(0);
             const T fVx = HF ? cvf<T, BF>(cur.fVx) : T(0), fVy = HF ? cvf<T, BF>(cur.fVy) : T(0);
             const T oxx = cvf<T, BF>(cur.oxx), oyy = cvf<T, BF>(cur.oyy), oxy = cvf<T, BF>(cur.oxy);
+            const T rg = RG ? cur.rg : T(0);
             const T vx0 = A.vx_j, vy0 = A.vy_j;
             T vxn, vyn, txy_prev;
-            row2<T, FVISC, EXACT>(K, ix, iy, inA, inA1, HF, A, cur.vx1, cur.vy1, P, txx, tyy, txy1, oxx, oyy, oxy,
-                                  eta1, eml, fP, fVx, fVy, vxn, vyn, txy_prev);
+            row2<T, FVISC, EXACT>(K, ix, iy, inA, inA1, HF, RG, A, cur.vx1, cur.vy1, P, txx, tyy, txy1, oxx, oyy,
+                                  oxy, eta1, eml, fP, fVx, fVy, rg, vxn, vyn, txy_prev);
             const bool keep = inA && t > 0;          // the first step has no velocity
             rP[t] = P;
             rtxx[t] = txx;
@@ -871,6 +875,7 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
             rfP[t] = fP;
             rfVx[t] = fVx;
             rfVy[t] = fVy;
+            rrg[t] = rg;
         }
         // ---- B, row r = ra - 2
         if (t >= 3) {
@@ -887,8 +892,8 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
             const bool inB = inx && r >= 0 && r <= nb1;
             const bool inB1 = inx && r + 1 >= 0 && r + 1 <= nb1;
             T vxn, vyn, txy_prev;
-            row2<T, FVISC, EXACT>(K, ix, iy, inB, inB1, HF, B, vx1, vy1, P, txx, tyy, txy1, roxx[u], royy[u],
-                                  roxy[u], reta1[u], reml[u], rfP[u], rfVx[u], rfVy[u], vxn, vyn, txy_prev);
+            row2<T, FVISC, EXACT>(K, ix, iy, inB, inB1, HF, RG, B, vx1, vy1, P, txx, tyy, txy1, roxx[u], royy[u],
+                                  roxy[u], reta1[u], reml[u], rfP[u], rfVx[u], rfVy[u], rrg[u], vxn, vyn, txy_prev);
             if (t >= 4 && r < je && own && inB) {
                 uint32_t q[NLOC2];
                 rowoff(r, q);
@@ -1551,18 +1556,25 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
             dim3 blk(kFX, kTBW, 1);
             dim3 grd((unsigned)((a.ntx + kTBW - 1) / kTBW), (unsigned)a.nchunks, 1);
             const bool hf = a.has_f;
-#define CHMY_TB2(FV, EX)                                                                                   \
-    do {                                                                                                   \
-        constexpr bool F32 = std::is_same<T, float>::value;                                                \
-        if (hf && bf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a, *tb);   \
-        else if (hf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a, *tb);  \
-        else hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a, *tb);          \
+            const bool hr = rg != nullptr;
+#define CHMY_TB2R(FV, EX, RG)                                                                                 \
+    do {                                                                                                      \
+        constexpr bool F32 = std::is_same<T, float>::value;                                                   \
+        if (hf && bf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, F32, RG>), grd, blk, 0, s, a, *tb);  \
+        else if (hf) hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 1, false, RG>), grd, blk, 0, s, a, *tb); \
+        else hipLaunchKernelGGL((fused2d_tb_kernel<T, FV, EX, 0, false, RG>), grd, blk, 0, s, a, *tb);         \
+    } while (0)
+#define CHMY_TB2(FV, EX)                          \
+    do {                                          \
+        if (hr) CHMY_TB2R(FV, EX, true);          \
+        else CHMY_TB2R(FV, EX, false);            \
     } while (0)
             if (fv && exact) CHMY_TB2(true, true);
             else if (fv) CHMY_TB2(true, false);
             else if (exact) CHMY_TB2(false, true);
             else CHMY_TB2(false, false);
 #undef CHMY_TB2
+#undef CHMY_TB2R
             CHMY_HIP(hipGetLastError());
             return 0;
         }
@@ -1602,8 +1614,9 @@ extern "C" {
 
 namespace chmy {
 
-// chmy_stokes_iter's checks and launch; `tb` (2D, no residual fields, no rho g)
-// runs two iterations with the free-slip batch between them.
+// chmy_stokes_iter's checks and launch; `tb` (2D, no residual fields) or
+// `tb3` (3D, no residual fields, no rho g) runs two iterations with the
+// free-slip batch between them.
 static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau,
                             const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                             const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
@@ -1612,8 +1625,9 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
                             double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                             double nudtau, void* stream, const FreeSlip2* tb, const FreeSlip3* tb3) {
     CHMY_TRY(check_grid(g));
-    if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || ((tb || tb3) && (rhog || (flags & CHMY_ITER_RESIDUAL))))
-        return set_error(CHMY_E_ARG, "two-iteration pass: free-slip batch of the grid's dims, no rho g, no residual fields");
+    if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || (tb3 && rhog) || ((tb || tb3) && (flags & CHMY_ITER_RESIDUAL)))
+        return set_error(CHMY_E_ARG, "two-iteration pass: free-slip batch of the grid's dims, no residual fields "
+                                     "(and no rho g in 3D)");
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
     if (flags & ~(CHMY_ITER_RESIDUAL | CHMY_ITER_EXACT)) return set_error(CHMY_E_ARG, "unknown flags %d", flags);
     const int write_residual = (flags & CHMY_ITER_RESIDUAL) != 0;
@@ -1803,7 +1817,7 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
     flags_last &= ~CHMY_ITER_PAIRS;
     FreeSlip2 fs;
     FreeSlip3 fs3;
-    const bool tb2 = CHMY_FUSED_TB && !rhog && nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    const bool tb2 = CHMY_FUSED_TB && nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
     const bool tb3 = pairs3 && !rhog && nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
     const bool tb = tb2 || tb3;
     int p = *parity_inout;

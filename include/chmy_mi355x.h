@@ -230,10 +230,10 @@ int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_
  * for the last (e.g. CHMY_ITER_RESIDUAL for a convergence check).  On return
  * *parity_inout names the set holding the new state.
  * Build extension: with a free-slip V batch (each component Dirichlet on its
- * own dim, Neumann on the others, scalar values) and no residual fields or
- * rho g, 2D iterations run in pairs, one pass each with the batch applied
- * in-kernel between the two; CHMY_ITER_PAIRS (in `flags`) asks for the same
- * in 3D.  A pair writes set 1-p like one iteration, so the parity flips once
+ * own dim, Neumann on the others, scalar values) and no residual fields,
+ * 2D iterations run in pairs, one pass each with the batch applied in-kernel
+ * between the two; CHMY_ITER_PAIRS (in `flags`) asks for the same in 3D
+ * (without rho g).  A pair writes set 1-p like one iteration, so the parity flips once
  * per pair; results are bitwise those of single iterations. */
 int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const chmy_field_t* tau0,
                         const chmy_field_t* Pr0, const chmy_field_t* V1, const chmy_field_t* tau1,

@@ -67,7 +67,7 @@ class _Set:
         return list(self.V) + list(self.tau) + [self.Pr]
 
 
-def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
+def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed, rhog=False):
     dt = torch.float64 if T == np.float64 else torch.float32
     rng = np.random.default_rng(seed)
     fill = lambda f, lo=-1.0, hi=1.0: f.data.copy_(
@@ -91,6 +91,11 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
         fV, fP = chmy.VectorField(arch, g, fdt), chmy.Field(arch, g, Cc(), fdt)
         for f in list(fV) + [fP]:
             fill(f, -0.5, 0.5)
+    rg = None
+    if rhog:   # rho g at the last component's location (update_velocity!'s -rho g)
+        from chmy.fields import vector_location
+        rg = chmy.Field(arch, g, vector_location(g.nd - 1, g.nd), dt)
+        fill(rg, -0.5, 0.5)
     divV = chmy.Field(arch, g, Cc(), dt)
     rV = chmy.VectorField(arch, g, dt)
     counts, comp, conds = _batch(vals) if g.nd == 2 else _batch3(vals)
@@ -102,7 +107,8 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed):
     args = lambda a, b: (C.byref(g.desc()), _arr(list(a.V)), _arr(list(a.tau)), C.byref(a.Pr.desc()),
                          _arr(list(b.V)), _arr(list(b.tau)), C.byref(b.Pr.desc()), _arr(list(told)),
                          C.byref(eta.desc()) if eta is not None else None,
-                         C.byref(eml.desc()) if eml is not None else None, None,
+                         C.byref(eml.desc()) if eml is not None else None,
+                         C.byref(rg.desc()) if rg is not None else None,
                          _arr(list(fV)) if fV is not None else None,
                          C.byref(fP.desc()) if fP is not None else None, C.byref(divV.desc()), _arr(list(rV)))
     scal = (eta_s, eve, G, dtp, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
@@ -188,3 +194,15 @@ def test_pair_pass_3d_with_correction_forcing(arch, T, forcing, exact):
     a = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, True, 22)
     b = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, False, 22)
     _compare(a, b, f"forcing {forcing}")
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("forcing", [None, "same"])
+def test_pair_pass_2d_with_rho_g(arch, T, exact, forcing):
+    """rho g (examples/stokes_2d_inc_ve_T.jl passes it to update_velocity!)
+    with and without correction forcing."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=(70, 45))
+    a = _run(arch, g, T, exact, "field", forcing, GENERAL, 5, True, 31, rhog=True)
+    b = _run(arch, g, T, exact, "field", forcing, GENERAL, 5, False, 31, rhog=True)
+    _compare(a, b, f"rho g, forcing {forcing}")
