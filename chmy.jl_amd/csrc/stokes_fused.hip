@@ -186,7 +186,8 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 
 // OPT selects the optional inputs/outputs at compile time so that the common
 // variants keep their pointers in SGPRs: 0 none, 1 forcing only (the
-// correction sweeps), 7 forcing, residual fields and rho g as flagged.
+// correction sweeps), 2 rho g only (the reference examples' buoyancy), 7
+// forcing, residual fields and rho g as flagged.
 //
 // Rows: thread row ty sits at y = j = j0 - 1 + ty (j0 = tile * (BY - 1)) and
 // computes the Center stresses and tau_xz at row j, and the y-vertex edge
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(TW * BY, CHMY_FUSED_WPE) fused3d_kernel(Fused3
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
     const bool has_f = OPT == 1 || (OPT == 7 && a.has_f);
-    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 7 && a.has_rg;
+    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 2 || (OPT == 7 && a.has_rg);
     constexpr int NR = BY + 1;   // LDS rows of the V and eta planes
     __shared__ T s_vx[NR][TW], s_vy[2][NR][TW], s_vz1[NR][TW];
     __shared__ T s_eta[FVISC ? 2 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][TW];
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(kFX * kW2) fused2d_kernel(Fused2<T> a) {
     using AR = Arith<T, EXACT>;
     const Kc<T>& K = a.k;
     const bool has_f = OPT == 1 || (OPT == 7 && a.has_f);
-    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 7 && a.has_rg;
+    const bool write_res = OPT == 7 && a.write_res, has_rg = OPT == 2 || (OPT == 7 && a.has_rg);
     const int lane = threadIdx.x;
     const int xt = blockIdx.x * kW2 + threadIdx.y;
     if (xt >= a.ntx) return;                        // whole wave leaves together
@@ -1504,12 +1505,14 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         dim3 blk(kTW3<T>, kBY3<T>, 1), grd(nb, 1, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
         const bool fonly = a.has_f && !a.write_res && !a.has_rg;
+        const bool gonly = a.has_rg && !a.has_f && !a.write_res;
 #define CHMY_F3(FV, EX)                                                                                   \
     do {                                                                                                  \
         constexpr bool F32 = std::is_same<T, float>::value;                                               \
         if (fonly && bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 1, F32>), grd, blk, 0, s, a); \
         else if (fonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 1, false>), grd, blk, 0, s, a); \
         else if (bf) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 7, F32>), grd, blk, 0, s, a);    \
+        else if (gonly) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 2, false>), grd, blk, 0, s, a); \
         else if (xtra) hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 7, false>), grd, blk, 0, s, a); \
         else hipLaunchKernelGGL((fused3d_kernel<T, FV, EX, kTW3<T>, kBY3<T>, 0, false>), grd, blk, 0, s, a);           \
     } while (0)
@@ -1585,12 +1588,14 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         dim3 grd((unsigned)((a.ntx + kW2 - 1) / kW2), (unsigned)a.nchunks, 1);
         const bool xtra = a.has_f || a.write_res || a.has_rg;
         const bool fonly = a.has_f && !a.write_res && !a.has_rg;
+        const bool gonly = a.has_rg && !a.has_f && !a.write_res;
 #define CHMY_F2(FV, EX)                                                                                   \
     do {                                                                                                  \
         constexpr bool F32 = std::is_same<T, float>::value;                                               \
         if (fonly && bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a);       \
         else if (fonly) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a);      \
         else if (bf) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 7, F32>), grd, blk, 0, s, a);          \
+        else if (gonly) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 2, false>), grd, blk, 0, s, a);      \
         else if (xtra) hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 7, false>), grd, blk, 0, s, a);       \
         else hipLaunchKernelGGL((fused2d_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a);                 \
     } while (0)
