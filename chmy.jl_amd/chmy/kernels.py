@@ -9,6 +9,7 @@ the same argument list the reference kernel takes, and it calls the C ABI.
   update_old, update_stress,
   update_velocity, update_thermal_flux,
   update_thermal                   examples/stokes_3d_inc_ve_T.jl:11-77 (2D: stokes_2d_inc_ve_T.jl:11-60)
+  thermal_step                     the last two in one launch, double-buffered T (extension)
   partial, lerp, divg, vmag, left,
   right                            GridOperators as used in test/test_grid_operators.jl
 """
@@ -102,6 +103,11 @@ def _update_thermal(grid, box, T, T_old, qT, dt, g):
               C.byref(T_old.desc()), _arr(qT), float(dt), current_stream())
 
 
+def _thermal_step(grid, box, T, T_out, T_old, qT, V, lam, dt, g):
+    _lib.call("chmy_thermal_step", C.byref(grid.desc()), C.byref(box), C.byref(T.desc()), C.byref(T_out.desc()),
+              C.byref(T_old.desc()), _arr(qT), _arr(V), float(lam), float(dt), current_stream())
+
+
 def _maxloc(grid, box, dst, src):
     _lib.call("chmy_maxloc", C.byref(grid.desc()), C.byref(box), C.byref(dst.desc()), C.byref(src.desc()),
               current_stream())
@@ -127,6 +133,7 @@ update_velocity = HipKernel("update_velocity!", _update_velocity)
 update_velocity_visc = HipKernel("update_velocity! (variable viscosity)", _update_velocity_visc)
 update_thermal_flux = HipKernel("update_thermal_flux!", _update_thermal_flux)
 update_thermal = HipKernel("update_thermal!", _update_thermal)
+thermal_step = HipKernel("update_thermal_flux! + update_thermal!", _thermal_step)
 maxloc = HipKernel("maxloc", _maxloc)
 # generic operators: (dst, src[, dim], grid)
 partial = HipKernel("∂", _op(_lib.OP_PARTIAL))

@@ -6,7 +6,8 @@ dt_adv) with dt_adv = 0.1 min(Δ_D / max|V_D|) / N / 2.1 (∞ while V = 0);
 η_ve = 1/(1/η + 1/(G dt)); then up to niter PT iterations, each the fused
 update_stress! + update_velocity! with the V batch, followed from the second
 time step on by update_thermal_flux!, update_thermal! and the T Neumann
-batch; every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
+batch (one launch into the second of two T buffers, then the batch on it:
+the same values as the two launches in place); every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
 boundary vertices, scaled by τsc, lz/psc) decides convergence, and from the
 third step on ncheck = ceil(0.5 n_x).  With exact=True the states are
 bitwise those of the reference's kernel sequence.
@@ -21,7 +22,7 @@ from .boundary_conditions import Neumann, batch
 from .fields import Field, VectorField
 from .grids import Center, spacing
 from .kernel_launch import Launcher
-from .kernels import maxabs, update_old, update_thermal, update_thermal_flux
+from .kernels import maxabs, thermal_step, update_old
 from .stokes import PTStokes
 
 
@@ -35,16 +36,23 @@ class StokesThermal:
         self.lam = float(lam_rhoCp) if lam_rhoCp is not None else 1e-4 * L[-1] ** 2 / tau_sc
         self.S = PTStokes(arch, grid, dtype=torch.float64, eta=eta, G=G, dt=1.0, normal_values=normal_values,
                           re_m=re_m, r=r, rhog=rhog, exact=exact)
-        self.T = Field(arch, grid, Center())
+        # T double-buffered: thermal_step reads one buffer and writes the other
+        self._Tbuf = [Field(arch, grid, Center()), Field(arch, grid, Center())]
+        self._Tbcs = [batch(grid, (f, Neumann())) for f in self._Tbuf]
+        self._tp = 0
         self.T_old = Field(arch, grid, Center())
         self.qT = VectorField(arch, grid)
-        self._Tbc = batch(grid, (self.T, Neumann()))
         self.launch = Launcher(arch, grid, sync=False)
         self.scales = dict(tau_sc=tau_sc, lz=L[-1], psc=self.G)
 
+    @property
+    def T(self):
+        """The current temperature Field."""
+        return self._Tbuf[self._tp]
+
     def apply_bc(self):
         self.S.apply_bc()
-        self._Tbc.apply()
+        self._Tbcs[self._tp].apply()
         self.S.sync_buffers()
 
     def dt_rule(self):
@@ -64,12 +72,16 @@ class StokesThermal:
         S.dt = dt
         S.eta_ve_s = 1.0 / (1.0 / self.eta + 1.0 / (self.G * dt))
         err = None
+        if it > 1:   # both buffers start equal, so points no launch writes agree
+            self._Tbuf[1 - self._tp].data.copy_(self.T.data)
         for q in range(1, niter + 1):
             check = q % ncheck == 0
             S.iteration(write_res=check)
             if it > 1:
-                self.launch(self.arch, g, (update_thermal_flux, (self.qT, self.T, S.V, self.lam, g)))
-                self.launch(self.arch, g, (update_thermal, (self.T, self.T_old, self.qT, dt, g)), bc=self._Tbc)
+                src, dst = self._tp, 1 - self._tp
+                self.launch(self.arch, g, (thermal_step, (self._Tbuf[src], self._Tbuf[dst], self.T_old, self.qT,
+                                                          S.V, self.lam, dt, g)), bc=self._Tbcs[dst])
+                self._tp = dst
             if check:
                 err = S.residual(**self.scales)
                 if all(e < eps for e in err):

@@ -176,17 +176,59 @@ __global__ void __launch_bounds__(256) thermal_flux_kernel(Box3 b, DFL<T> Tf, DF
     bool ok;
     Ix3 I = box_index(b, ok);
     if (!ok) return;
-    const DFL<T>* V[3] = {&V0, &V1, &V2};
-    const DFL<T>* q[3] = {&q0, &q1, &q2};
     const T isp[3] = {i0, i1, i2};
-    T tc = g_at(Tf, I);
-    for (int d = 0; d < nd; ++d) {
+    const T tc = g_at(Tf, I);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {   // constant indices once unrolled: no stack arrays
+        if (d >= nd) break;
+        const DFL<T>& Vd = d == 0 ? V0 : (d == 1 ? V1 : V2);
+        const DFL<T>& qd = d == 0 ? q0 : (d == 1 ? q1 : q2);
         // qT_D = ((-λ ∂_D T) + max(V_D,0) left_D T) + min(V_D,0) right_D T
         // (stokes_3d_inc_ve_T.jl:120-128); T Center: left = T[I-e], right = T[I]
-        T tl = g_at(Tf, sh(I, d, -1));
-        T v = g_at(*V[d], I);
-        g_at(*q[d], I) = (nlam * ((tc - tl) * isp[d]) + jmax0(v) * tl) + jmin0(v) * tc;
+        const T tl = g_at(Tf, sh(I, d, -1));
+        const T v = g_at(Vd, I);
+        g_at(qd, I) = (nlam * ((tc - tl) * isp[d]) + jmax0(v) * tl) + jmin0(v) * tc;
     }
+}
+
+// update_thermal_flux! and update_thermal! in one launch (build extension):
+// qT at I from T at I, I-e and V at I (stored, as update_thermal_flux! stores
+// it), qT at I+e recomputed the same way when I+e is inside the box, else
+// read from qT's storage; T_out[I] = T_old[I] - Δt divg(qT)[I].  T and T_out
+// are distinct buffers.
+template <class T>
+__global__ void __launch_bounds__(256) thermal_step_kernel(Box3 b, DFL<T> Tf, DFL<T> Tout, DFL<T> To, DFL<T> V0,
+                                                           DFL<T> V1, DFL<T> V2, DFL<T> q0, DFL<T> q1, DFL<T> q2,
+                                                           int nd, T nlam, T dt, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const int64_t hi[3] = {b.lo[0] + b.len[0], b.lo[1] + b.len[1], b.lo[2] + b.len[2]};
+    const int64_t at[3] = {I.i, I.j, I.k};
+    const T isp[3] = {i0, i1, i2};
+    const T tc = g_at(Tf, I);
+    T dv = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= nd) break;
+        const DFL<T>& Vd = d == 0 ? V0 : (d == 1 ? V1 : V2);
+        const DFL<T>& qd = d == 0 ? q0 : (d == 1 ? q1 : q2);
+        const T tl = g_at(Tf, sh(I, d, -1));
+        const T v = g_at(Vd, I);
+        const T ql = (nlam * ((tc - tl) * isp[d]) + jmax0(v) * tl) + jmin0(v) * tc;
+        g_at(qd, I) = ql;
+        const Ix3 J = sh(I, d, 1);
+        T qr;
+        if (at[d] + 1 < hi[d]) {
+            const T tr = g_at(Tf, J), vr = g_at(Vd, J);
+            qr = (nlam * ((tr - tc) * isp[d]) + jmax0(vr) * tc) + jmin0(vr) * tr;
+        } else {
+            qr = g_at(qd, J);
+        }
+        const T t = (qr - ql) * isp[d];
+        dv = d == 0 ? t : dv + t;
+    }
+    g_at(Tout, I) = g_at(To, I) - dt * dv;
 }
 
 template <class T>
@@ -440,6 +482,35 @@ int chmy_thermal_update(const chmy_grid_t* g, const chmy_box_t* box, const chmy_
     CHMY_TRY(check_vector(g, qT, "qT", Tf->dtype));
     return Tf->dtype == CHMY_F64 ? thermal_launch<double>(g, bx, qT, Tf, To, dt, false, (hipStream_t)stream)
                                  : thermal_launch<float>(g, bx, qT, Tf, To, dt, false, (hipStream_t)stream);
+}
+
+int chmy_thermal_step(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* Tf, const chmy_field_t* T_out,
+                      const chmy_field_t* To, const chmy_field_t* qT, const chmy_field_t* V, double lam, double dt,
+                      void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_center(g, Tf, "T", -1));
+    CHMY_TRY(check_center(g, T_out, "T_out", Tf->dtype));
+    CHMY_TRY(check_center(g, To, "T_old", Tf->dtype));
+    CHMY_TRY(check_vector(g, qT, "qT", Tf->dtype));
+    CHMY_TRY(check_vector(g, V, "V", Tf->dtype));
+    if (Tf->data == T_out->data) return set_error(CHMY_E_ARG, "T and T_out must be distinct buffers");
+    Box3 b = box3(bx, g->nd);
+    auto launch = [&](auto zero) -> int {
+        using T = decltype(zero);
+        DFL<T> t = lview<T>(*Tf), to = lview<T>(*T_out), o = lview<T>(*To);
+        DFL<T> v0 = lview<T>(V[0]), q0 = lview<T>(qT[0]);
+        DFL<T> v1 = g->nd > 1 ? lview<T>(V[1]) : v0, v2 = g->nd > 2 ? lview<T>(V[2]) : v0;
+        DFL<T> q1 = g->nd > 1 ? lview<T>(qT[1]) : q0, q2 = g->nd > 2 ? lview<T>(qT[2]) : q0;
+        dim3 grd = grid_for(b, blk2());
+        hipLaunchKernelGGL(thermal_step_kernel<T>, grd, blk2(), 0, (hipStream_t)stream, b, t, to, o, v0, v1, v2, q0,
+                           q1, q2, g->nd, -T(lam), T(dt), T(g->inv_spacing[0]), T(g->inv_spacing[1]),
+                           T(g->inv_spacing[2]));
+        CHMY_HIP(hipGetLastError());
+        return 0;
+    };
+    return Tf->dtype == CHMY_F64 ? launch(0.0) : launch(0.0f);
 }
 
 int chmy_maxloc(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, const chmy_field_t* src,

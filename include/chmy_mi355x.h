@@ -196,6 +196,14 @@ int chmy_thermal_flux(const chmy_grid_t* grid, const chmy_box_t* box, const chmy
 /* update_thermal! (examples/stokes_3d_inc_ve_T.jl:73-77). */
 int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* T,
                         const chmy_field_t* T_old, const chmy_field_t* qT, double dt, void* stream);
+/* update_thermal_flux! then update_thermal! (examples/stokes_3d_inc_ve_T.jl:
+ * 166-167) in one launch over the same box, writing T_out (a second buffer,
+ * distinct from T) instead of T; qT is stored as update_thermal_flux! stores
+ * it.  T_out and qT equal the two launches' T and qT bitwise (build
+ * extension). */
+int chmy_thermal_step(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* T,
+                      const chmy_field_t* T_out, const chmy_field_t* T_old, const chmy_field_t* qT,
+                      const chmy_field_t* V, double lambda_rhoCp, double dt, void* stream);
 
 /* One fused PT iteration = update_stress! then update_velocity! over the
  * Launcher box (examples/stokes_3d_inc_ve_T.jl:222-223), double-buffered:

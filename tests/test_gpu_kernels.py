@@ -344,6 +344,45 @@ def test_diffusion_step_equals_two_kernels(arch, T, nd):
             assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"q box {box}"
 
 
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_thermal_step_equals_two_kernels(arch, T, nd):
+    """chmy_thermal_step (update_thermal_flux! + update_thermal! in one launch
+    into a second T buffer) gives the stored qT and the new T of the two
+    launches, bitwise, on random storage with signed V (both upwind branches)
+    over the Launcher box and a sub-box."""
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(50 + nd)
+    Tf, _ = pair(arch, g, og, (Cc(),) * nd, T, rng=rng)
+    To, _ = pair(arch, g, og, (Cc(),) * nd, T, rng=rng)
+    V = [pair(arch, g, og, vloc(D, nd), T, rng=rng)[0] for D in range(nd)]
+    qs = [pair(arch, g, og, vloc(D, nd), T, rng=rng)[0] for D in range(nd)]
+    lam, dt = 0.7, 0.21 * min(chmy.spacing(g)) ** 2
+    bits = np.int64 if T == np.float64 else np.int32
+    for box in (None, ((1,) * nd, tuple(n - 1 for n in g.dims))):
+        Tn = chmy.Field(arch, g, Cc(), Tf.data.dtype)
+        q = [chmy.Field(arch, g, f.loc, f.data.dtype) for f in qs]
+        for a, b in zip(q, qs):
+            a.data.copy_(b.data)
+        lo, ws = ((0,) * nd, tuple(n + 2 for n in g.dims)) if box is None else box
+        chmy.kernels.thermal_step.launch(g, lo, ws, Tf, Tn, To, q, V, lam, dt, g)
+        T2 = chmy.Field(arch, g, Cc(), Tf.data.dtype)
+        T2.data.copy_(Tf.data)
+        q2 = [chmy.Field(arch, g, f.loc, f.data.dtype) for f in qs]
+        for a, b in zip(q2, qs):
+            a.data.copy_(b.data)
+        chmy.kernels.update_thermal_flux.launch(g, lo, ws, q2, T2, V, lam, g)
+        chmy.kernels.update_thermal.launch(g, lo, ws, T2, To, q2, dt, g)
+        chmy.synchronize()
+        sl = tuple(slice(1 + l, 1 + l + w) for l, w in zip(reversed(lo), reversed(ws)))
+        assert np.array_equal(dev(Tn)[sl].view(bits), dev(T2)[sl].view(bits)), f"box {box}"
+        for a, b in zip(q, q2):
+            assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"qT box {box}"
+    with pytest.raises(chmy.ChmyError):
+        chmy.kernels.thermal_step.launch(g, lo, ws, Tf, Tf, To, q, V, lam, dt, g)
+
+
 @pytest.mark.parametrize("ow", [(16, 8), (3, 5)])
 def test_launcher_outer_width_matches(arch, ow):
     """Launcher(outer_width): inner box + 2N slabs with per-side bc
