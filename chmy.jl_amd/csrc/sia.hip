@@ -504,18 +504,38 @@ __device__ __forceinline__ void sia_adj_block(const SiaAdj<T>& p, int64_t nx, in
 
 constexpr int kAdjLds = (kSX + 8) * (kSY + 8) + (kSX + 6) * (kSY + 6) + (kSX + 4) * (kSY + 4) + 2 * (kSX + 2) * (kSY + 2) + kSX * kSY;
 
-template <class T>
+// NP > 0: the Glen exponent as a compile-time constant (it must equal the
+// run-time one), so that every power's product loop unrolls; the products
+// and their order are the same as with NP = 0.
+template <class T, int NP>
 __global__ void __launch_bounds__(kST) sia_step_tile(SiaStep<T> f, int64_t nx, int64_t ny) {
     __shared__ T lds[(kSX + 4) * (kSY + 4) + (kSX + 2) * (kSY + 2)];
+    if constexpr (NP > 0) f.s.n = NP;
     sia_step_block(f, nx, ny, lds);
 }
 
 // blockIdx.z == 0: adjoint tile; blockIdx.z == 1: forward tile of an independent step
-template <class T>
+template <class T, int NP>
 __global__ void __launch_bounds__(kST) sia_adj_tile(SiaAdj<T> a, SiaStep<T> f, int64_t nx, int64_t ny) {
     __shared__ T lds[kAdjLds];
+    if constexpr (NP > 0) {
+        a.s.n = NP;
+        f.s.n = NP;
+    }
     if (blockIdx.z == 0) sia_adj_block(a, nx, ny, lds);
     else sia_step_block(f, nx, ny, lds);
+}
+
+// n = 3 (Glen's law as sia.jl uses it) takes the unrolled kernels
+static void launch_step_tile(dim3 grd, hipStream_t st, const SiaStep<double>& f, int64_t nx, int64_t ny) {
+    if (f.s.n == 3) hipLaunchKernelGGL((sia_step_tile<double, 3>), grd, dim3(kST), 0, st, f, nx, ny);
+    else hipLaunchKernelGGL((sia_step_tile<double, 0>), grd, dim3(kST), 0, st, f, nx, ny);
+}
+
+static void launch_adj_tile(dim3 grd, hipStream_t st, const SiaAdj<double>& a, const SiaStep<double>& f, int64_t nx,
+                            int64_t ny) {
+    if (a.s.n == 3) hipLaunchKernelGGL((sia_adj_tile<double, 3>), grd, dim3(kST), 0, st, a, f, nx, ny);
+    else hipLaunchKernelGGL((sia_adj_tile<double, 0>), grd, dim3(kST), 0, st, a, f, nx, ny);
 }
 
 template <class T>
@@ -654,8 +674,7 @@ int chmy_sia_step_adjoint(const chmy_grid_t* g, const chmy_field_t* H, const chm
     SiaPar p{As, npow, mb_a, mb_b, mb_c, dt};
     if (!work) {   // one tiled pass
         SiaAdj<double> a = sia_adjointer(g, H, lam_in, lam_out, A ? grad_logA : nullptr, B, A, p, dt);
-        hipLaunchKernelGGL(sia_adj_tile<double>, sia_tiles(g->n[0], g->n[1], 1), dim3(kST), 0, (hipStream_t)stream,
-                           a, SiaStep<double>{}, g->n[0], g->n[1]);
+        launch_adj_tile(sia_tiles(g->n[0], g->n[1], 1), (hipStream_t)stream, a, SiaStep<double>{}, g->n[0], g->n[1]);
         CHMY_HIP(hipGetLastError());
         return 0;
     }
@@ -696,8 +715,7 @@ int chmy_sia_run(const chmy_grid_t* g, const chmy_field_t* states, int nstates, 
     hipStream_t st = (hipStream_t)stream;
     for (int q = 0; q + 1 < nstates; ++q) {
         SiaStep<double> f = sia_stepper(g, &states[q], &states[q + 1], B, A, p, dts[q]);
-        hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(g->n[0], g->n[1], 1), dim3(kST), 0, st, f, g->n[0],
-                           g->n[1]);
+        launch_step_tile(sia_tiles(g->n[0], g->n[1], 1), st, f, g->n[0], g->n[1]);
     }
     CHMY_HIP(hipGetLastError());
     return 0;
@@ -731,7 +749,7 @@ int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, 
     for (int q = 1; q < len(nseg - 1); ++q) {
         SiaStep<double> f = sia_stepper(g, state(nseg - 1, q - 1), state(nseg - 1, q), B, A, p,
                                         dts[(nseg - 1) * stride + q - 1]);
-        hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(nx, ny, 1), dim3(kST), 0, st, f, nx, ny);
+        launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
     }
     int slot = 0;   // lam_in of the next adjoint step: 0 = lam_a, 1 = lam_b
     for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -743,7 +761,7 @@ int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, 
             SiaStep<double> f{};
             if (fw) f = sia_stepper(g, state(sg - 1, r), state(sg - 1, r + 1), B, A, p, dts[(sg - 1) * stride + r]);
             if (r >= L) {
-                hipLaunchKernelGGL(sia_step_tile<double>, sia_tiles(nx, ny, 1), dim3(kST), 0, st, f, nx, ny);
+                launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
                 continue;
             }
             const int q = L - 1 - r;
@@ -751,7 +769,7 @@ int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, 
             const chmy_field_t* lo = slot ? lam_a : lam_b;
             SiaAdj<double> a = sia_adjointer(g, state(sg, q), li, lo, A ? grad_logA : nullptr, B, A, p,
                                              dts[sg * stride + q]);
-            hipLaunchKernelGGL(sia_adj_tile<double>, sia_tiles(nx, ny, fw ? 2 : 1), dim3(kST), 0, st, a, f, nx, ny);
+            launch_adj_tile(sia_tiles(nx, ny, fw ? 2 : 1), st, a, f, nx, ny);
             slot ^= 1;
         }
     }
