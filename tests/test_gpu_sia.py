@@ -84,14 +84,16 @@ def test_sia_fused_step_equals_reference_sequence(arch):
     assert dmax == O.sia_Dmax(og, hH, hB, A=hA, As=0.5, npow=3)
 
 
-def test_sia_run_equals_step_and_neumann_sequence(arch):
+@pytest.mark.parametrize("npow", [3, 2, 4])
+def test_sia_run_equals_step_and_neumann_sequence(arch, npow):
     """chmy_sia_run (step + Neumann batch in one tiled launch, Δt per step)
     gives the same bits as chmy_sia_step followed by the Neumann batch, on a
-    grid that is not a multiple of the tile (45 x 21), for 15 steps."""
+    grid that is not a multiple of the tile (45 x 21), for 15 steps; n = 3
+    takes the tile kernel compiled for n = 3, other exponents the generic one."""
     n = (45, 21)
     g, og, H, hH, B, hB, A, hA = _setup(arch, n=n, seed=7)
-    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
-    T = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=npow, mb=(0.5, 0.3, 0.25))
+    T = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=npow, mb=(0.5, 0.3, 0.25))
     S.reset(hH.arr)
     T.reset(hH.arr)
     dts = [2e-3 * (1 + 0.1 * k) for k in range(15)]
@@ -113,12 +115,14 @@ def test_sia_run_equals_step_and_neumann_sequence(arch):
         S.H.swap(S.H2)
 
 
+@pytest.mark.parametrize("npow", [3, 2, 4])
 @pytest.mark.parametrize("n", [(45, 21), (64, 32)])
-def test_sia_adjoint_tiled_equals_five_pass(arch, n):
+def test_sia_adjoint_tiled_equals_five_pass(arch, n, npow):
     """The tiled one-launch adjoint step and the five gather passes give the
-    same bits (λ and the log A gradient), interior and tile edges included."""
+    same bits (λ and the log A gradient), interior and tile edges included,
+    for the n = 3 instantiation and the generic exponent."""
     g, og, H, hH, B, hB, A, hA = _setup(arch, n=n, seed=11)
-    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
+    S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=npow, mb=(0.5, 0.3, 0.25))
     rng = np.random.default_rng(2)
     lam0 = torch.from_numpy(rng.standard_normal((n[1], n[0])).T.copy()).to(S.lam.data.device)
     out = []
