@@ -494,6 +494,31 @@ def stokes_gradient(args):
     print(json.dumps(res))
 
 
+def pairs_rate(S, args, isz):
+    """3D PT iterations two per pass (chmy_stokes_iterate with the pairs flag:
+    fused3d_tb_kernel, then the V batch); one step = one pass = two
+    iterations.  Prints one JSON line with the time per iteration."""
+    import torch
+    n = args.n
+    K = max(1, args.steps // 2)
+    for _ in range(max(1, args.warmup // 2)):
+        S.iterate(2, pairs=True)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(K):
+        S.iterate(2, pairs=True)
+    t1.record()
+    torch.cuda.synchronize()
+    ms_pass = t0.elapsed_time(t1) / K
+    A = algorithmic_bytes(n, isz)
+    print(json.dumps({"metric": "T_eff GB/s, two PT iterations per pass", "value": round(2 * A / (ms_pass * 1e-3) / 1e9, 2),
+                      "unit": "GB/s", "passes": K, "ms_per_pass": round(ms_pass, 4),
+                      "ms_per_iteration": round(ms_pass / 2, 4), "dtype": args.dtype,
+                      "config": {"workload": f"3D PT viscoelastic Stokes, {n}^3 cells, fused3d_tb_kernel + V batch",
+                                 "algorithmic_bytes_per_iteration": A}}))
+
+
 def copy_calibration(args):
     """A plain device copy of known size (1 GiB read + 1 GiB written per
     launch): the reference point for scaling HBM counter bytes."""
@@ -516,9 +541,10 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", default="fused", choices=["fused", "unfused"],
+    ap.add_argument("--mode", default="fused", choices=["fused", "unfused", "pairs"],
                     help="fused: one-pass double-buffered iteration; unfused: the reference's "
-                         "update_stress! + update_velocity! kernels")
+                         "update_stress! + update_velocity! kernels; pairs: two iterations per pass "
+                         "(the free-slip batch in-kernel between them), timed per pass")
     ap.add_argument("--ttc", choices=["2d", "3d"], default=None,
                     help="time-to-convergence comparison instead of the T_eff line")
     ap.add_argument("--eps", type=float, default=1e-8)
@@ -579,6 +605,9 @@ def main():
     n = args.n
     S = setup(n, dtype)
     print(f"# setup done: {n}^3 {args.dtype}", file=sys.stderr, flush=True)
+    if args.mode == "pairs":
+        pairs_rate(S, args, isz)
+        return
     fused = args.mode == "fused"
     for _ in range(args.warmup):
         S.iteration() if fused else S.iteration_unfused()
