@@ -1,5 +1,6 @@
-"""Parity at the workloads' full sizes, through properties that do not depend
-on size: two formulations of the same iteration give the same bits.
+"""Parity at the workloads' full sizes: one exact fused iteration at 255³
+against the oracle on seeded random storage, and properties that do not
+depend on size (two formulations of the same iteration give the same bits):
 
 - 3D PT Stokes at 255³ (spherical inclusion, η = 1e-3 inside, times exp(-T)),
   FP64: the fused exact iteration (one launch + the V batch) against the
@@ -11,7 +12,7 @@ on size: two formulations of the same iteration give the same bits.
   FP32: the two-iteration pass with the in-kernel free-slip batch against
   single fused iterations with the batch launched between them.
 
-Each compares the whole padded storage bitwise on the device."""
+Each compares the whole padded storage bitwise."""
 import pytest
 import torch
 
@@ -113,3 +114,57 @@ def test_3d_255_correction_sweep_pairs_equal_single(arch):
     assert float(Ms[0].d.V[0].data.abs().max()) > 0
     for k, (a, b) in enumerate(zip(_state(Ms[0].d), _state(Ms[1].d))):
         _bitwise(a, b, f"255^3 FP32 correction sweep pairs vs single, field {k}")
+
+
+def test_3d_255_fused_exact_equals_oracle(arch):
+    """One fused exact iteration at 255³ FP64 on seeded random storage (every
+    field, η and η_ml in [1e-3, 3]) against the oracle's update_stress! and
+    update_velocity! on the same storage: V, τ, Pr, ∇V and r_V bitwise."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle as O
+    from _helpers import assert_same, ogrid, pair, vloc
+    from chmy import Center as Cc
+    from chmy import _lib
+    from chmy.kernels import _arr
+    g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(255,) * 3)
+    og = ogrid(g)
+    rng = np.random.default_rng(255)
+    T = np.float64
+    tl = list(chmy.tensor_locations(3).values())
+    V = [pair(arch, g, og, vloc(D, 3), T, rng=rng) for D in range(3)]
+    tau = [pair(arch, g, og, l, T, rng=rng) for l in tl]
+    Pr = pair(arch, g, og, Cc(), T, rng=rng)
+    told = [pair(arch, g, og, l, T, rng=rng) for l in tl]
+    eta = pair(arch, g, og, Cc(), T, rng=rng, lo=1e-3, hi=3.0)
+    eml = pair(arch, g, og, Cc(), T, rng=rng, lo=1e-3, hi=3.0)
+    rg = pair(arch, g, og, vloc(2, 3), T, rng=rng)
+    dV = pair(arch, g, og, Cc(), T, fill=0.0)
+    rV = [pair(arch, g, og, vloc(D, 3), T, fill=0.0) for D in range(3)]
+    Vo = [chmy.Field(arch, g, f.loc, f.dtype) for f, _ in V]
+    tauo = [chmy.Field(arch, g, f.loc, f.dtype) for f, _ in tau]
+    Pro = chmy.Field(arch, g, Cc(), Pr[0].dtype)
+    for a, (b, _) in zip(Vo + tauo + [Pro], V + tau + [Pr]):
+        a.data.copy_(b.data)
+    G, dt, eta_s = 1.0, 0.37, 0.8
+    eve = 1.0 / (1.0 / eta_s + 1.0 / (G * dt))
+    p = O.pt_params(og)
+    _lib.call("chmy_stokes_iter", C.byref(g.desc()), _arr([a for a, _ in V]), _arr([a for a, _ in tau]),
+              C.byref(Pr[0].desc()), _arr(Vo), _arr(tauo), C.byref(Pro.desc()), _arr([a for a, _ in told]),
+              C.byref(eta[0].desc()), C.byref(eml[0].desc()), C.byref(rg[0].desc()), None, None,
+              C.byref(dV[0].desc()), _arr([a for a, _ in rV]), 1 | 2, eta_s, eve, G, dt,
+              p["dtau_Pr"], p["dtau_r"], p["nudtau"], chmy.current_stream())
+    chmy.synchronize()
+    H = lambda L: [b for _, b in L]
+    O.update_stress(og, H(tau), Pr[1], dV[1], H(V), H(told), eta[1], eve, G, dt, p["dtau_Pr"], p["dtau_r"])
+    O.update_velocity(og, H(V), H(rV), Pr[1], H(tau), rg[1], eve, p["nudtau"], eta_ml=eml[1], G=G, dt=dt)
+    for D, (f, (_, h)) in enumerate(zip(Vo, V)):
+        assert_same(f, h, f"V[{D}]")
+    for c, (f, (_, h)) in enumerate(zip(tauo, tau)):
+        assert_same(f, h, f"tau[{c}]")
+    assert_same(Pro, Pr[1], "Pr")
+    assert_same(dV[0], dV[1], "divV")
+    for D in range(3):
+        assert_same(rV[D][0], rV[D][1], f"r_V[{D}]")
