@@ -3,6 +3,8 @@
 // one logical index I of the box; blocks are one wave64 wide along x (the
 // contiguous dim) so loads and stores coalesce.  These are bandwidth-bound
 // streaming kernels; the fused Stokes iteration lives in stokes.hip.
+#include <cstring>
+
 #include "common.hpp"
 
 namespace chmy {
@@ -128,29 +130,103 @@ __global__ void __launch_bounds__(256) diffusion_update_kernel(Box3 b, DFL<T> C,
 // when I+e is inside the box, else read from q's storage (what update_C! would
 // read there); C_out[I] = C[I] - Δt divg(q)[I].  C and C_out are distinct
 // buffers, so no thread reads a C value another has advanced.
+// The step's new C at I (box b): q_D at I (stored when store_q, as
+// compute_q! stores it), q_D at I+e recomputed when I+e is in the box, else
+// read from q's storage; C[I] - Δt divg(q)[I].
 template <class T>
-__global__ void __launch_bounds__(256) diffusion_step_kernel(Box3 b, DFL<T> C, DFL<T> Co, DFL<T> q0, DFL<T> q1,
-                                                             DFL<T> q2, int nd, T nchi, T dt, T i0, T i1, T i2) {
-    bool ok;
-    Ix3 I = box_index(b, ok);
-    if (!ok) return;
+__device__ __forceinline__ T diffusion_value(const Box3& b, const DFL<T>& C, const DFL<T>& q0, const DFL<T>& q1,
+                                             const DFL<T>& q2, int nd, T nchi, T dt, const T* isp, Ix3 I,
+                                             bool store_q) {
     const T c = g_at(C, I);
     const int64_t hi[3] = {b.lo[0] + b.len[0], b.lo[1] + b.len[1], b.lo[2] + b.len[2]};
     const int64_t at[3] = {I.i, I.j, I.k};
-    const T isp[3] = {i0, i1, i2};
     T dv = T(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (d >= nd) break;
         const DFL<T>& qd = d == 0 ? q0 : (d == 1 ? q1 : q2);
         const T ql = nchi * ((c - g_at(C, sh(I, d, -1))) * isp[d]);
-        g_at(qd, I) = ql;
+        if (store_q) g_at(qd, I) = ql;
         const Ix3 J = sh(I, d, 1);
         const T qr = at[d] + 1 < hi[d] ? nchi * ((g_at(C, J) - c) * isp[d]) : g_at(qd, J);
         const T t = (qr - ql) * isp[d];
         dv = d == 0 ? t : dv + t;
     }
-    g_at(Co, I) = c - dt * dv;
+    return c - dt * dv;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) diffusion_step_kernel(Box3 b, DFL<T> C, DFL<T> Co, DFL<T> q0, DFL<T> q1,
+                                                             DFL<T> q2, int nd, T nchi, T dt, T i0, T i1, T i2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const T isp[3] = {i0, i1, i2};
+    g_at(Co, I) = diffusion_value(b, C, q0, q1, q2, nd, nchi, dt, isp, I, true);
+}
+
+// One scalar rule per (dim, side) of a Center field's batch.
+struct SideRules {
+    int32_t kind[3][2];   // CHMY_DIRICHLET / CHMY_NEUMANN
+    double val[3][2];
+    double h[3];          // spacings (the Neumann step)
+};
+
+// The step and the C batch in one launch, when every (dim, side) carries one
+// scalar rule: one thread per point of [0, n+2]^N, which holds everything the
+// step (box [0, n+1]^N) and the batch (ghost planes 0 and n+1, tangential
+// 0..n+2) write.  A point that no dim writes keeps the step's value (or is
+// left alone outside the box).  A point that some dims write takes the
+// batch's final value, the chain bc_set_kernel evaluates: its inner point Q
+// (every ghost coordinate moved inward) as the step leaves it (recomputed
+// here when Q is in the box, else read as stored: no thread writes it), then
+// each writing dim's rule from the highest to the lowest, the batch's own fma
+// steps.  q is stored on the box as the step stores it.
+template <class T>
+__global__ void __launch_bounds__(256) diffusion_step_bc_kernel(Box3 box, Box3 reg, DFL<T> C, DFL<T> Co, DFL<T> q0,
+                                                                DFL<T> q1, DFL<T> q2, int nd, T nchi, T dt, T i0,
+                                                                T i1, T i2, SideRules r) {
+    bool ok;
+    const Ix3 P = box_index(reg, ok);
+    if (!ok) return;
+    const T isp[3] = {i0, i1, i2};
+    const int64_t p[3] = {P.i, P.j, P.k};
+    int64_t m[3], qc[3];
+    int gs[3];
+    bool in_box = true, ghost = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        m[d] = box.len[d] - 2;   // cells: the box is [0, n+1]
+        qc[d] = p[d];
+        gs[d] = -1;
+        if (d >= nd) continue;
+        in_box = in_box && p[d] <= m[d] + 1;
+        if (p[d] == 0 || p[d] == m[d] + 1) {
+            gs[d] = p[d] == 0 ? 0 : 1;
+            qc[d] = p[d] == 0 ? 1 : m[d];
+            ghost = true;
+        }
+    }
+    T v = T(0);
+    if (in_box) v = diffusion_value(box, C, q0, q1, q2, nd, nchi, dt, isp, P, true);
+    if (!ghost) {
+        if (in_box) g_at(Co, P) = v;
+        return;
+    }
+    const Ix3 Q{qc[0], qc[1], qc[2]};
+    bool q_in = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (d < nd) q_in = q_in && qc[d] <= m[d] + 1;
+    T w = q_in ? diffusion_value(box, C, q0, q1, q2, nd, nchi, dt, isp, Q, false) : g_at(Co, Q);
+#pragma unroll
+    for (int d = 2; d >= 0; --d) {
+        if (d >= nd || gs[d] < 0) continue;
+        const T val = T(r.val[d][gs[d]]);
+        if (r.kind[d][gs[d]] == CHMY_DIRICHLET) w = fma(T(2), val - w, w);
+        else w = fma(T(r.h[d]), gs[d] == 0 ? -val : val, w);
+    }
+    g_at(Co, P) = w;
 }
 
 // ------------------------------------------------------------------ update_old!
@@ -434,6 +510,46 @@ int chmy_diffusion_iterate(const chmy_grid_t* g, const chmy_field_t* C0, const c
         box.len[d] = d < g->nd ? g->n[d] + 2 : 1;
     }
     int p = *parity_inout;
+    // one scalar rule on every (dim, side): the batch rides in the step's launch
+    bool fold = total == 2 * g->nd;
+    SideRules rules;
+    std::memset(&rules, 0, sizeof rules);
+    for (int k = 0; fold && k < total; ++k) {
+        const chmy_bc_t& c = bcs[k];
+        fold = (c.kind == CHMY_DIRICHLET || c.kind == CHMY_NEUMANN) && c.vkind != CHMY_VALUE_FIELD;
+        rules.kind[k / 2][k % 2] = c.kind;
+        rules.val[k / 2][k % 2] = c.vkind == CHMY_VALUE_SCALAR ? c.value : 0.0;
+    }
+    if (fold && niter > 0) {
+        for (int d = 0; d < 3; ++d) rules.h[d] = g->spacing[d];
+        CHMY_TRY(check_center(g, C0, "C0", -1));
+        CHMY_TRY(check_center(g, C1, "C1", C0->dtype));
+        CHMY_TRY(check_vector(g, q, "q", C0->dtype));
+        if (C0->data == C1->data) return set_error(CHMY_E_ARG, "C0 and C1 must be distinct buffers");
+        chmy_box_t bx;
+        CHMY_TRY(check_box(g, &box, &bx));
+        const Box3 b = box3(bx, g->nd);
+        Box3 reg = b;
+        for (int d = 0; d < g->nd; ++d) reg.len[d] = b.len[d] + 1;   // [0, n+2]
+        auto run = [&](auto zero) -> int {
+            using T = decltype(zero);
+            DFL<T> c[2] = {lview<T>(*C0), lview<T>(*C1)};
+            DFL<T> q0 = lview<T>(q[0]);
+            DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
+            const dim3 grd = grid_for(reg, blk2());
+            for (int it = 0; it < niter; ++it) {
+                hipLaunchKernelGGL(diffusion_step_bc_kernel<T>, grd, blk2(), 0, (hipStream_t)stream, b, reg, c[p],
+                                   c[1 - p], q0, q1, q2, g->nd, -T(chi), T(dt), T(g->inv_spacing[0]),
+                                   T(g->inv_spacing[1]), T(g->inv_spacing[2]), rules);
+                p = 1 - p;
+            }
+            CHMY_HIP(hipGetLastError());
+            return 0;
+        };
+        CHMY_TRY(C0->dtype == CHMY_F64 ? run(0.0) : run(0.0f));
+        *parity_inout = p;
+        return 0;
+    }
     for (int it = 0; it < niter; ++it) {
         CHMY_TRY(chmy_diffusion_step(g, &box, Cs[p], Cs[1 - p], q, chi, dt, stream));
         CHMY_TRY(chmy_bc_batchset(g, bc_counts, fset[1 - p], bcs, stream));

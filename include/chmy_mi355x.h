@@ -158,7 +158,9 @@ int chmy_diffusion_step(const chmy_grid_t* grid, const chmy_box_t* box, const ch
 /* niter steps of the diffusion loop (examples/diffusion_2d.jl:44-48) in one
  * call: chmy_diffusion_step over the Launcher box from buffer p =
  * *parity_inout into 1-p, then the C batch (bc_counts[2*dim+side-1] <= 1
- * entries, the conditions in bcs, as chmy_bc_batchset) on 1-p.  On return
+ * entries, the conditions in bcs, as chmy_bc_batchset) on 1-p.  When every
+ * (dim, side) has one rule with a scalar (or no) value, the batch runs inside
+ * the step's launch (one launch per step, same bits).  On return
  * *parity_inout names the buffer holding C (build extension). */
 int chmy_diffusion_iterate(const chmy_grid_t* grid, const chmy_field_t* C0, const chmy_field_t* C1,
                            const chmy_field_t* q, const int32_t* bc_counts, const chmy_bc_t* bcs, double chi,

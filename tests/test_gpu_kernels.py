@@ -8,11 +8,12 @@ sites.
 """
 import numpy as np
 import pytest
+import torch
 
 import chmy
 from chmy import Center as Cc, Vertex as Vv
 import oracle as O
-from _helpers import assert_same, dev, ogrid, pair, vloc
+from _helpers import NP2TORCH, assert_same, dev, ogrid, pair, vloc
 
 pytestmark = pytest.mark.gpu
 
@@ -305,6 +306,63 @@ def test_diffusion_iterate_256_1000_steps(arch):
     assert par.value == 0
     assert_same(Cs[par.value], hC, "C after 1000 steps")
     assert_same(q.x, hq[0], "qx")
+
+
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("nd", [1, 2, 3])
+@pytest.mark.parametrize("mix", ["neumann0", "mixed"])
+def test_diffusion_iterate_folded_batch_equals_step_and_batchset(arch, T, nd, mix):
+    """chmy_diffusion_iterate with one scalar rule on every (dim, side) runs
+    the batch inside the step's launch; 5 steps of it equal chmy_diffusion_step
+    + chmy_bc_batchset per step bitwise (both buffers and q), on random storage
+    (signed zeros included), for zero-flux Neumann and for a mix of Dirichlet
+    and Neumann with non-zero values."""
+    import ctypes as C
+    from chmy import _lib
+    from chmy.kernels import _arr
+    g = grids(nd)
+    og = ogrid(g)
+    rng = np.random.default_rng(60 + nd)
+    if mix == "neumann0":
+        rules = [(_lib.NEUMANN, _lib.VALUE_ZERO, 0.0)] * (2 * nd)
+    else:
+        pool = [(_lib.DIRICHLET, _lib.VALUE_SCALAR, 0.7), (_lib.NEUMANN, _lib.VALUE_SCALAR, -1.3),
+                (_lib.DIRICHLET, _lib.VALUE_ZERO, 0.0), (_lib.NEUMANN, _lib.VALUE_SCALAR, 0.25),
+                (_lib.NEUMANN, _lib.VALUE_ZERO, 0.0), (_lib.DIRICHLET, _lib.VALUE_SCALAR, -2.0)]
+        rules = pool[:2 * nd]
+    counts = (C.c_int32 * 6)(*([1] * (2 * nd) + [0] * (6 - 2 * nd)))
+    bcs = (_lib.ChmyBC * (2 * nd))(*[_lib.ChmyBC(k, vk, v, None) for k, vk, v in rules])
+    sets = []
+    for _ in range(2):
+        Cs = [chmy.Field(arch, g, Cc(), NP2TORCH[T]), chmy.Field(arch, g, Cc(), NP2TORCH[T])]
+        q = [chmy.Field(arch, g, vloc(D, nd), NP2TORCH[T]) for D in range(nd)]
+        sets.append((Cs, q))
+    for f in sets[0][0] + sets[0][1]:
+        a = rng.uniform(-1.0, 1.0, tuple(f.data.shape))
+        a[a < -0.95] = -0.0   # signed zeros meet the batch's fma steps
+        f.data.copy_(torch.from_numpy(a).to(f.data.dtype))
+    for a, b in zip(sets[1][0] + sets[1][1], sets[0][0] + sets[0][1]):
+        a.data.copy_(b.data)
+    dt = 0.2 * min(chmy.spacing(g)) ** 2
+    steps = 5
+    par = C.c_int(0)
+    Cs, q = sets[0]
+    _lib.call("chmy_diffusion_iterate", C.byref(g.desc()), C.byref(Cs[0].desc()), C.byref(Cs[1].desc()), _arr(q),
+              counts, bcs, 0.9, dt, steps, C.byref(par), chmy.current_stream())
+    Cr, qr = sets[1]
+    box = chmy.kernels._box((0,) * nd, tuple(n + 2 for n in g.dims))
+    p = 0
+    for _ in range(steps):
+        _lib.call("chmy_diffusion_step", C.byref(g.desc()), C.byref(box), C.byref(Cr[p].desc()),
+                  C.byref(Cr[1 - p].desc()), _arr(qr), 0.9, dt, chmy.current_stream())
+        _lib.call("chmy_bc_batchset", C.byref(g.desc()), counts, _arr([Cr[1 - p]] * (2 * nd)), bcs,
+                  chmy.current_stream())
+        p = 1 - p
+    chmy.synchronize()
+    assert par.value == p
+    bits = np.int64 if T == np.float64 else np.int32
+    for k, (a, b) in enumerate(zip(Cs + q, Cr + qr)):
+        assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"array {k}"
 
 
 @pytest.mark.parametrize("T", TYPES)
