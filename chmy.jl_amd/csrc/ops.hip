@@ -516,7 +516,8 @@ int chmy_diffusion_iterate(const chmy_grid_t* g, const chmy_field_t* C0, const c
     std::memset(&rules, 0, sizeof rules);
     for (int k = 0; fold && k < total; ++k) {
         const chmy_bc_t& c = bcs[k];
-        fold = (c.kind == CHMY_DIRICHLET || c.kind == CHMY_NEUMANN) && c.vkind != CHMY_VALUE_FIELD;
+        fold = (c.kind == CHMY_DIRICHLET || c.kind == CHMY_NEUMANN) &&
+               (c.vkind == CHMY_VALUE_ZERO || c.vkind == CHMY_VALUE_SCALAR);   // else the batch set validates
         rules.kind[k / 2][k % 2] = c.kind;
         rules.val[k / 2][k % 2] = c.vkind == CHMY_VALUE_SCALAR ? c.value : 0.0;
     }
