@@ -336,6 +336,18 @@ int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckp
                      int npow, double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot,
                      void* stream);
 
+/* ---------------------------------------------------------------- measurement */
+/* Measurement aid, no counterpart in the reference (its callers: bench.py's
+ * access-pattern ceilings).  Reads nr FP64 arrays in[] and writes nw arrays
+ * out[] (18 and 10, the fused iteration's counts, or 1 and 1), each the
+ * Center storage of an n^3 grid ((n+4)^3
+ * elements, n <= 1000), over the Launcher box [0, n+1]^3 with one sum per
+ * point, in pattern 0 (linear, x fastest), 1 (the fused 3D iteration's tiles:
+ * 64 x by lanes, 62 x (by-1) outputs, z-march of kchunk planes plus one, XCD
+ * remap when xcd != 0) or 2 (the same tiles, one plane per block). */
+int chmy_probe_stream(int pattern, const double* const* in, int nr, double* const* out, int nw, int64_t n, int by,
+                      int kchunk, int xcd, void* stream);
+
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),
  * exact (order-independent); result in f64 written to *out_dev (device).
