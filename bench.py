@@ -540,7 +540,8 @@ def pattern_probe(args, nr=18, nw=10, reps=20):
     rows = {}
     cases = [("linear", 0, 12, 16, 1), ("tiles 64x12, 16-plane z-march", 1, 12, 16, 1),
              ("tiles 64x12, 8-plane z-march", 1, 12, 8, 1), ("tiles 64x12, 16-plane z-march, no XCD remap", 1, 12, 16, 0),
-             ("tiles 64x12, one plane per block", 2, 12, 1, 1)]
+             ("tiles 64x12, one plane per block", 2, 12, 1, 1),
+             ("tiles 64x12, one plane per block, z neighbours re-read", 3, 12, 1, 1)]
     for name, pat, by, kc, xcd in cases:
         call = lambda: _lib.call("chmy_probe_stream", pat, pin, nr, pout, nw, n, by, kc, xcd, current_stream())
         for _ in range(3):

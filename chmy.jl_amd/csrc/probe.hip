@@ -100,13 +100,40 @@ __global__ void __launch_bounds__(64 * BY) probe_plane(ProbeArrays a, ProbeGeo g
     if (own) probe_store<NW>(a, o, s);
 }
 
+// One plane per block as pattern 2, but each point also reads every input at
+// plane k-1 and the first three inputs at plane k+1: the z neighbours a fused
+// kernel without a z-march would re-read (2 NR + 3 reads per point).
+template <int BY, int NR, int NW>
+__global__ void __launch_bounds__(64 * BY) probe_plane_halo(ProbeArrays a, ProbeGeo g) {
+    const int nb = g.ntx * g.nty * (int)g.m;
+    const int t = probe_tile(blockIdx.x, nb, g.xcd);
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty;
+    const int64_t z = t / (g.ntx * g.nty);
+    const int64_t x = (int64_t)tx * 62 + threadIdx.x - 1, y = (int64_t)ty * (BY - 1) + threadIdx.y - 1;
+    const bool own = threadIdx.x >= 1 && threadIdx.x <= 62 && threadIdx.y >= 1 && x < g.m && y < g.m;
+    const int64_t xc = min(max(x, (int64_t)-1), g.m), yc = min(max(y, (int64_t)-1), g.m);
+    const int64_t o = probe_off(g, xc, yc, z);
+    double v[2 * NR + 3];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        v[q] = a.r[q][o];
+        v[NR + q] = a.r[q][o - g.pxy];   // plane z-1 >= -1
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[2 * NR + q] = a.r[q % NR][o + g.pxy];   // plane z+1 <= n+2
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 2 * NR + 3; ++q) s += v[q];
+    if (own) probe_store<NW>(a, o, s);
+}
+
 }  // namespace chmy
 
 using namespace chmy;
 
 extern "C" int chmy_probe_stream(int pattern, const double* const* in, int nr, double* const* out, int nw, int64_t n,
                                  int by, int kchunk, int xcd, void* stream) {
-    if (pattern < 0 || pattern > 2) return set_error(CHMY_E_ARG, "probe pattern %d not in 0..2", pattern);
+    if (pattern < 0 || pattern > 3) return set_error(CHMY_E_ARG, "probe pattern %d not in 0..3", pattern);
     const bool fused = nr == 18 && nw == 10, copy = nr == 1 && nw == 1;
     if (!in || !out || !(fused || copy))
         return set_error(CHMY_E_ARG, "probe: 18 inputs and 10 outputs (the fused iteration) or 1 and 1 (a copy)");
@@ -147,7 +174,8 @@ extern "C" int chmy_probe_stream(int pattern, const double* const* in, int nr, d
 #define CHMY_PROBE(BY)                                                                                  \
     if (by == BY) {                                                                                     \
         if (pattern == 1) hipLaunchKernelGGL((probe_tiled<BY, NR, NW>), dim3(nb), blk, 0, s, a, g);     \
-        else hipLaunchKernelGGL((probe_plane<BY, NR, NW>), dim3(nb), blk, 0, s, a, g);                 \
+        else if (pattern == 2) hipLaunchKernelGGL((probe_plane<BY, NR, NW>), dim3(nb), blk, 0, s, a, g); \
+        else hipLaunchKernelGGL((probe_plane_halo<BY, NR, NW>), dim3(nb), blk, 0, s, a, g);            \
     }
         CHMY_PROBE(4) CHMY_PROBE(8) CHMY_PROBE(12) CHMY_PROBE(16)
 #undef CHMY_PROBE

@@ -344,7 +344,9 @@ int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckp
  * elements, n <= 1000), over the Launcher box [0, n+1]^3 with one sum per
  * point, in pattern 0 (linear, x fastest), 1 (the fused 3D iteration's tiles:
  * 64 x by lanes, 62 x (by-1) outputs, z-march of kchunk planes plus one, XCD
- * remap when xcd != 0) or 2 (the same tiles, one plane per block). */
+ * remap when xcd != 0), 2 (the same tiles, one plane per block) or 3 (as 2,
+ * each point also reading every input at plane k-1 and the first three at
+ * k+1: the z neighbours a kernel without a z-march re-reads). */
 int chmy_probe_stream(int pattern, const double* const* in, int nr, double* const* out, int nw, int64_t n, int by,
                       int kchunk, int xcd, void* stream);
 

@@ -108,7 +108,7 @@ def test_probe_validates_on_host():
     L = _lib.lib()
     ins = (C.c_void_p * 18)(*([0x1000] * 18))
     outs = (C.c_void_p * 10)(*([0x2000] * 10))
-    cases = [((3, ins, 18, outs, 10, 255, 12, 16, 1), b"pattern"),
+    cases = [((4, ins, 18, outs, 10, 255, 12, 16, 1), b"pattern"),
              ((1, ins, 17, outs, 10, 255, 12, 16, 1), b"18 inputs"),
              ((1, ins, 18, outs, 9, 255, 12, 16, 1), b"18 inputs"),
              ((1, ins, 18, outs, 10, 1001, 12, 16, 1), b"not in 1..1000"),
