@@ -265,13 +265,13 @@ def stokes2d_rates(args):
 
 
 def diffusion_c1(args):
-    """Config 1 (examples/diffusion_2d.jl form, 256^2, 1000 steps, FP64) on the
+    """2D linear diffusion (examples/diffusion_2d.jl form, 256^2, 1000 steps, FP64) on the
     GPU: the reference's launch sequence (compute_q!, update_C! with the
     Neumann batch, through the Launcher), the fused step (chmy_diffusion_step
     into a second buffer + the batch, through the Launcher) and the whole loop
     in one native call (chmy_diffusion_iterate), each checked bitwise against
-    the oracle's 1000 steps, which are also timed on the host (the CPU backend
-    the config names)."""
+    the oracle's 1000 steps, which are also timed on the host (the reference
+    runs this workload on its CPU backend)."""
     import numpy as np
     import torch
     import chmy
@@ -337,7 +337,7 @@ def diffusion_c1(args):
         ms = e0.elapsed_time(e1)
         bitwise = bool(np.array_equal(out.data.cpu().numpy(), hC.arr))
         res[mode] = {"ms_1000_steps": round(ms, 3), "bitwise_vs_oracle": bitwise}
-    out = {"metric": "config 1 wall time, 1000 steps", "value": res["native_loop"]["ms_1000_steps"], "unit": "ms",
+    out = {"metric": "2D diffusion 256^2 wall time, 1000 steps", "value": res["native_loop"]["ms_1000_steps"], "unit": "ms",
            "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
            "config": {"workload": "2D linear diffusion, 256^2 cells, 1000 steps, Neumann(0)"},
            "runs": res, "cpu_oracle_ms_1000_steps": round(t_cpu * 1e3, 1),
@@ -623,7 +623,7 @@ def main():
     ap.add_argument("--stokes2d", action="store_true",
                     help="2D fused PT iteration rates at --n (default 2047): FP64, FP32, defect-correction sweep")
     ap.add_argument("--diffusion", action="store_true",
-                    help="config 1: 2D diffusion 256^2 x 1000 steps, reference launches vs fused step, vs the oracle")
+                    help="2D diffusion 256^2 x 1000 steps: reference launches vs fused step vs native loop, vs the oracle")
     ap.add_argument("--pattern-probe", action="store_true",
                     help="the fused 3D kernel's access pattern alone (18 reads, 10 writes per point, no stencil "
                          "work) against linear order and one plane per block, at --n")

@@ -270,7 +270,7 @@ def test_stream_and_zero_fill(arch):
 
 
 def test_diffusion_iterate_256_1000_steps(arch):
-    """The same config-1 loop as one native call (chmy_diffusion_iterate:
+    """The same 256² x 1000-step diffusion loop as one native call (chmy_diffusion_iterate:
     fused step into the other buffer + Neumann batch, 1000 times): every
     stored value bitwise equal to the oracle's loop."""
     import ctypes as C
