@@ -238,3 +238,32 @@ def test_launcher_boxes():
             o, w = L.outer_offset(D, S), L.outer_worksize(D)
             cover[o[0]:o[0] + w[0], o[1]:o[1] + w[1]] += 1
     assert np.all(cover == 1)
+
+
+def test_error_messages_are_per_thread():
+    """chmy_last_error is thread-local: threads that fail different calls at
+    the same time each read their own message."""
+    import threading
+    L = _lib.lib()
+    barrier = threading.Barrier(2)
+    seen = {}
+
+    def fail_stream():
+        barrier.wait()
+        for _ in range(200):
+            assert L.chmy_stream_create(3, None) == -1
+            seen.setdefault("stream", set()).add(L.chmy_last_error())
+
+    def fail_grid():
+        barrier.wait()
+        for _ in range(200):
+            assert L.chmy_diffusion_flux(None, None, None, None, 1.0, None) == -1
+            seen.setdefault("grid", set()).add(L.chmy_last_error())
+
+    ts = [threading.Thread(target=fail_stream), threading.Thread(target=fail_grid)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert len(seen["stream"]) == 1 and b"stream is NULL" in next(iter(seen["stream"]))
+    assert len(seen["grid"]) == 1 and b"grid is NULL" in next(iter(seen["grid"]))
