@@ -11,8 +11,8 @@ from . import _lib, kernels
 from ._lib import ChmyError
 from .architectures import (Arch, MI355XBackend, SingleDeviceArchitecture, activate, current_stream,
                             get_backend, get_device, heuristic_groupsize, set_device, synchronize)
-from .boundary_conditions import (BatchSet, Dirichlet, EmptyBatch, FieldBatch, FirstOrderBC, Neumann, batch,
-                                  bc_)
+from .boundary_conditions import (BatchSet, BoundaryFunction, Dirichlet, EmptyBatch, FieldBatch, FirstOrderBC,
+                                  Neumann, batch, bc_)
 from .fields import (Field, TensorField, VectorField, halo, interior, location, set_, set_storage,
                      tensor_locations, vector_location)
 from .grids import Center, UniformGrid, Vertex, coord, flip, inv_spacing, spacing
@@ -20,7 +20,7 @@ from .kernel_launch import Launcher, Offset
 
 __all__ = [
     "Arch", "MI355XBackend", "SingleDeviceArchitecture", "activate", "current_stream", "get_backend",
-    "get_device", "heuristic_groupsize", "set_device", "synchronize", "BatchSet", "Dirichlet", "EmptyBatch",
+    "get_device", "heuristic_groupsize", "set_device", "synchronize", "BatchSet", "BoundaryFunction", "Dirichlet", "EmptyBatch",
     "FieldBatch", "FirstOrderBC", "Neumann", "batch", "bc_", "Field", "TensorField", "VectorField", "halo",
     "interior", "location", "set_", "set_storage", "tensor_locations", "vector_location", "Center",
     "UniformGrid", "Vertex", "coord", "flip", "inv_spacing", "spacing", "Launcher", "Offset", "kernels",

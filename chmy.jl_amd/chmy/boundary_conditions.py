@@ -1,4 +1,5 @@
-"""bc! batches (src/BoundaryConditions/batch.jl, first_order_boundary_condition.jl).
+"""bc! batches (src/BoundaryConditions/batch.jl, first_order_boundary_condition.jl,
+boundary_function.jl).
 
 A per-field condition is a FirstOrderBC (all dims and sides) or a dict
 {"x": bc | (bc_left, bc_right), ...} (the reference's NamedTuple form; missing
@@ -6,14 +7,24 @@ dims are `nothing`).  batch() normalises a list of (field, condition) pairs
 into one FieldBatch per (dim, side), pruning fields without a condition
 (batch.jl:72-157).  bc_() applies the set on the device: dims N..1, side 1 then
 side 2, fields in batch order (batch.jl:20-29).
+
+A condition's value is None (zero), a number, a Field over the tangential
+indices, or a BoundaryFunction (a plain function is taken as a continuous one
+over the reduced dimensions, boundary_function.jl:97-101).  batch() evaluates
+a BoundaryFunction once per (field, dim, side) on the host, at the points
+first_order_boundary_condition.jl:42-84 evaluates it, into a value Field over
+the tangential range 0..n+2 that the device batch reads.
 """
 from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _lib
 from .architectures import current_stream
-from .fields import Field
+from .fields import Field, set_storage
+from .grids import Center, UniformGrid, coord, flip
 
 
 class FirstOrderBC:
@@ -44,12 +55,80 @@ class FirstOrderBC:
         return b
 
 
+class BoundaryFunction:
+    """BoundaryFunction(fun; discrete, parameters, reduce_dims)
+    (boundary_function.jl:57-95).  Continuous: fun(coordinates..., parameters...)
+    of the evaluation point, without the boundary's own dim when reduce_dims.
+    Discrete: fun(grid, loc, dim, indices..., parameters...) with Chmy's logical
+    indices (interior from 1), likewise reduced."""
+
+    def __init__(self, fun, discrete=False, parameters=None, reduce_dims=True):
+        self.fun = fun
+        self.discrete = bool(discrete)
+        self.parameters = () if parameters is None else tuple(parameters)
+        self.reduce_dims = bool(reduce_dims)
+
+    def __call__(self, grid, loc, dim, I):
+        """The value at logical index I (all dims) for location tuple loc
+        (boundary_function.jl:37-43)."""
+        keep = [d for d in range(grid.nd) if not (self.reduce_dims and d == dim)]
+        if self.discrete:
+            return self.fun(grid, loc, dim, *[I[d] for d in keep], *self.parameters)
+        return self.fun(*[coord(grid, loc[d], d, I[d]) for d in keep], *self.parameters)
+
+    def __repr__(self):
+        return f"BoundaryFunction({getattr(self.fun, '__name__', self.fun)})"
+
+
+def _as_value(value):
+    if value is None or isinstance(value, (Field, BoundaryFunction)) or not callable(value):
+        return value
+    return BoundaryFunction(value)   # FirstOrderBC{Kind}(f::Function) (boundary_function.jl:97-101)
+
+
 def Dirichlet(value=None):
-    return FirstOrderBC(_lib.DIRICHLET, value)
+    return FirstOrderBC(_lib.DIRICHLET, _as_value(value))
 
 
 def Neumann(value=None):
-    return FirstOrderBC(_lib.NEUMANN, value)
+    return FirstOrderBC(_lib.NEUMANN, _as_value(value))
+
+
+def _materialise(grid, f, d, side, cond):
+    """A BoundaryFunction condition of field f on dim d, side (0 or 1) as a
+    condition with a value Field (or a number in 1D).  The point and location
+    are those of first_order_boundary_condition.jl: a Vertex-located Dirichlet
+    is evaluated at the boundary node (1 or m) with f's location (:46-50), a
+    Center-located Dirichlet at the ghost (0 or m+1) with Center (:56-68), a
+    Neumann at the ghost with the flipped location (:76-84); tangential indices
+    run over 0..n+2."""
+    bf = cond.value
+    nd = grid.nd
+    m = f.dims[d]
+    vertex = f.loc[d].code == _lib.VERTEX
+    if cond.kind == _lib.DIRICHLET and vertex:
+        loc, normal = f.loc, (1 if side == 0 else m)
+    elif cond.kind == _lib.DIRICHLET:
+        loc, normal = (Center(),) * nd, (0 if side == 0 else m + 1)
+    else:
+        loc, normal = tuple(flip(l) for l in f.loc), (0 if side == 0 else m + 1)
+    if nd == 1:
+        return FirstOrderBC(cond.kind, float(bf(grid, loc, d, (normal,))))
+    tang = [t for t in range(nd) if t != d]
+    tg = UniformGrid(origin=[grid.origin[t] for t in tang], extent=[grid.extent[t] for t in tang],
+                     dims=[grid.dims[t] for t in tang])
+    vf = Field(None, tg, Center(), f.dtype, device=f.data.device)
+    arr = np.zeros(tuple(reversed(vf.pad)))
+    ranges = [range(0, grid.dims[t] + 3) for t in tang]
+    for idx in np.ndindex(*[len(r) for r in ranges]):
+        I = [0] * nd
+        I[d] = normal
+        for q, t in enumerate(tang):
+            I[t] = ranges[q][idx[q]]
+        # logical tangential index i sits at storage i + 1 of the value field
+        arr[tuple(I[t] + 1 for t in reversed(tang))] = bf(grid, loc, d, I)
+    set_storage(vf, arr)
+    return FirstOrderBC(cond.kind, vf)
 
 
 class EmptyBatch:
@@ -160,7 +239,12 @@ def batch(grid, *pairs, exchange=None):
     fields to exchange with neighbouring ranks; on a single-device Bounded
     grid it is a no-op (batch.jl:103-105)."""
     names = grid.axes_names()
-    regs = [(f, _regularise(grid.nd, names, spec)) for f, spec in pairs]
+    regs = []
+    for f, spec in pairs:
+        r = _regularise(grid.nd, names, spec)
+        r = [tuple(_materialise(grid, f, d, s, c) if c is not None and isinstance(c.value, BoundaryFunction) else c
+                   for s, c in enumerate(r[d])) for d in range(grid.nd)]
+        regs.append((f, r))
     per_dim = []
     for d in range(grid.nd):
         sides = []

@@ -267,3 +267,43 @@ def test_error_messages_are_per_thread():
         t.join()
     assert len(seen["stream"]) == 1 and b"stream is NULL" in next(iter(seen["stream"]))
     assert len(seen["grid"]) == 1 and b"grid is NULL" in next(iter(seen["grid"]))
+
+
+def test_boundary_function_points_follow_the_reference():
+    """batch() evaluates a BoundaryFunction at the points and locations
+    first_order_boundary_condition.jl uses (host only, no device): a
+    Vertex-located Dirichlet at the boundary nodes 1 and m with the field's
+    location (:46-50), a Center-located Dirichlet at the ghosts 0 and m+1 with
+    Center (:56-68), a Neumann at the ghosts with the flipped location
+    (:76-84), over the tangential range 0..n+2."""
+    from chmy import Center as Cc, Vertex as Vv
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 2.0), dims=(4, 5))
+    calls = []
+
+    def rec(grid, loc, dim, *I):
+        calls.append((tuple(type(l).__name__[0] for l in loc), dim, I))
+        return 0.0
+
+    bf = chmy.BoundaryFunction(rec, discrete=True, reduce_dims=False)
+    vx = chmy.Field(None, g, (Vv(), Cc()))
+    chmy.batch(g, (vx, {"x": chmy.Dirichlet(bf), "y": chmy.Neumann(bf)}))
+    x_calls = [c for c in calls if c[1] == 0]
+    y_calls = [c for c in calls if c[1] == 1]
+    assert {c[0] for c in x_calls} == {("V", "C")}
+    assert sorted({c[2][0] for c in x_calls}) == [1, 5]               # boundary nodes 1, m = n+1
+    assert sorted({c[2][1] for c in x_calls}) == list(range(0, 8))    # tangential 0..n_y+2
+    assert {c[0] for c in y_calls} == {("C", "V")}                     # flipped
+    assert sorted({c[2][1] for c in y_calls}) == [0, 6]               # ghosts 0, m+1
+    assert sorted({c[2][0] for c in y_calls}) == list(range(0, 7))
+    calls.clear()
+    cf = chmy.Field(None, g, Cc())
+    chmy.batch(g, (cf, {"y": chmy.Dirichlet(bf)}))
+    assert {c[0] for c in calls} == {("C", "C")} and sorted({c[2][1] for c in calls}) == [0, 6]
+    # reduced dims drop the boundary's own dim; a continuous function gets coordinates
+    got = []
+    red = chmy.BoundaryFunction(lambda y, s: got.append(y) or s * y, parameters=(2.0,))
+    b = chmy.batch(g, (cf, {"x": chmy.Dirichlet(red)}))
+    ys = g.coords(Cc(), 1, first=0, count=8)
+    assert np.array_equal(np.array(got[:8]), ys)
+    v = b[0][0].conditions[0].value
+    assert np.array_equal(v.data.cpu().numpy()[1:9], 2.0 * ys)
