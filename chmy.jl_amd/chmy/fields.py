@@ -104,7 +104,11 @@ def set_(f, value, grid=None, *, parameters=(), discrete=False):
     A function is evaluated on the host with the grid's fma coordinates
     (continuous: fun(x, y, z, *parameters)) or logical indices (discrete:
     fun(grid, loc, ix, iy, iz, *parameters)), vectorised over numpy arrays,
-    and uploaded."""
+    and uploaded.  A VectorField/TensorField sets every component (field.jl:144-145)."""
+    if isinstance(f, tuple):
+        for c in f:
+            set_(c, value, grid, parameters=parameters, discrete=discrete)
+        return
     it = f.interior()
     if callable(value):
         if grid is None:

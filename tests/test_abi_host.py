@@ -307,3 +307,14 @@ def test_boundary_function_points_follow_the_reference():
     assert np.array_equal(np.array(got[:8]), ys)
     v = b[0][0].conditions[0].value
     assert np.array_equal(v.data.cpu().numpy()[1:9], 2.0 * ys)
+
+
+def test_set_on_a_vector_field_sets_every_component():
+    """set!(fs::NamedTuple, args...) (field.jl:144-145): each component is set
+    with its own location's coordinates."""
+    g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(4, 3))
+    V = chmy.VectorField(None, g)
+    chmy.set_(V, lambda x, y: x + 10.0 * y, g)
+    for D, f in enumerate(V):
+        xs = np.meshgrid(*[g.coords(f.loc, d) for d in range(2)], indexing="ij")
+        assert np.array_equal(f.interior().numpy(), xs[0] + 10.0 * xs[1]), D
