@@ -168,3 +168,20 @@ def test_3d_255_fused_exact_equals_oracle(arch):
     assert_same(dV[0], dV[1], "divV")
     for D in range(3):
         assert_same(rV[D][0], rV[D][1], f"r_V[{D}]")
+
+
+def test_3d_255_fast_form_tracks_exact_form(arch):
+    """200 PT iterations at 255³ FP64 (inclusion workload, V batch between
+    iterations): the fast form (7 divisions per point) stays within 1e-11 of
+    the exact form relative to each field's max|value|."""
+    g, eta, dt = configs.stokes3d_inclusion(arch, 255)
+    A = _solver(arch, g, eta, dt, 3, exact=True)
+    B = _solver(arch, g, eta, dt, 3, exact=False)
+    A.iterate(200)
+    B.iterate(200)
+    chmy.synchronize()
+    for k, (a, b) in enumerate(zip(_state(A), _state(B))):
+        scale = float(a.data.abs().max())
+        assert scale > 0, k
+        d = float((a.data - b.data).abs().max())
+        assert d <= 1e-11 * scale, (k, d, scale)
