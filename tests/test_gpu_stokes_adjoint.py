@@ -102,3 +102,46 @@ def test_stokes_gradient_mixed_precision_matches_f64(arch):
     a, b = g64.xyz().cpu().numpy(), gm.xyz().cpu().numpy()
     assert abs(J - Jm) <= 1e-8 * abs(J)
     assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("dims", [(12, 10), (8, 7, 6), (24, 22, 20)])
+def test_stokes_gradient_matches_cpu_reference_gradient(arch, dims):
+    """The whole gradient on the GPU (forward solve, misfit, adjoint solve,
+    gradient pass; f64, residual < 1e-12) against the same pipeline run by the
+    oracle on the CPU (its own update_stress!/update_velocity!/bc! solves and
+    orc_stokes_grad_eta) on the same log-viscosity and observation: J to 1e-9
+    and the gradient to 1e-7 relative to its max."""
+    from test_oracle_stokes_adjoint import Problem
+    g, la0, A = _problem(arch, dims)
+    nd = len(dims)
+    pert = smooth_log_field(g, 0.3, seed=7)
+    A.set_logeta(la0 + pert)
+    A.solve_forward(eps=1e-12)
+    V_obs = A.observe()
+    A.set_logeta(la0)
+    J, grad = A.J_and_gradient(V_obs, eps=1e-12)
+    gpu = grad.xyz().cpu().numpy()
+    # the CPU reference: the oracle's solves on the same storage arrays
+    P = Problem(dims)
+    normal = pure_shear_values(nd)
+    P.set_logeta(la0 + pert)
+    P.solve(normal)
+    Vobs = [P.top(P.V[D]).copy() for D in range(nd - 1)]
+    P.set_logeta(la0)
+    P.solve(normal)
+    Jc = sum(0.5 * float(np.sum((P.top(P.V[D]) - Vobs[D]) ** 2)) for D in range(nd - 1))
+    tau_f = [O.HField(P.g, l) for l in P.tl]
+    for a, b in zip(tau_f, P.tau):
+        a.arr[...] = b.arr
+    for f in P.fV:
+        f.arr[...] = 0.0
+    for D in range(nd - 1):
+        P.top(P.fV[D])[...] = -(P.top(P.V[D]) - Vobs[D])
+    P.solve({}, forcing=True)
+    gc = O.HField(P.g, O.CENTER)
+    O.stokes_grad_eta(P.g, tau_f, P.tau, P.eta, gc, logscale=True)
+    cpu = np.transpose(gc.arr, tuple(reversed(range(nd))))
+    assert J > 0 and abs(J - Jc) <= 1e-9 * abs(Jc), (J, Jc)
+    scale = np.max(np.abs(cpu))
+    assert scale > 0
+    assert np.max(np.abs(gpu - cpu)) <= 1e-7 * scale, np.max(np.abs(gpu - cpu)) / scale
