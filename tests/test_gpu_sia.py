@@ -254,3 +254,51 @@ def test_sia_gradient_matches_central_differences(arch):
         fd = (Jp - Jm) / (2 * h)
         ad = float(np.sum(gn * v[2:-2, 2:-2].T))
         assert abs(fd - ad) <= 1e-5 * abs(fd) + 1e-12 * J, (fd, ad)
+
+
+def test_sia_gradient_matches_cpu_reference(arch):
+    """The GPU gradient (forward with checkpoints, reverse sweep) against the
+    CPU reference: J and central differences of J from the oracle's forward
+    (update_q!, q Dirichlet, update_H!, H Neumann; sia.jl:71-81) for the same
+    Δt sequence, along random directions of log A."""
+    n = 48
+    g = chmy.UniformGrid(arch, origin=(0.0, 0.0), extent=(250e3, 250e3), dims=(n, n))
+    og = ogrid(g)
+    B = bed_bumps(g, seed=4, hmin=1500.0, hmax=2500.0)
+    logA0 = np.full((n + 4, n + 4), math.log(2.8e-6))
+    mb = (0.01, 1500.0, 2.0)
+    S = SIA(arch, g, B, logA=logA0 + smooth_field(g, 0.3, seed=9), npow=3, mb=mb, dt_max=1.0)
+    S.reset()
+    nsteps = 40
+    dts = S.forward(nsteps)
+    H_obs = S.H.interior().cpu().numpy().copy()        # (x, y)
+    S.set_logA(logA0)
+    J, grad = S.gradient(None, S.H.interior().new_tensor(H_obs), nsteps, dts, ncheck=5)
+    gn = grad.cpu().numpy()
+    hB = O.HField(og, O.CENTER)
+    hB.arr[...] = B
+
+    def J_cpu(logA):
+        hA = O.HField(og, O.CENTER)
+        hA.arr[...] = np.exp(logA)
+        hH = O.HField(og, O.CENTER)
+        hq = [O.HField(og, tuple(l.code for l in vloc(D, 2))) for D in range(2)]
+        for dt in dts:
+            O.sia_flux(og, hq, hH, hB, A=hA, As=0.0, npow=3)
+            O.bc_apply(og, [(hq[0], O.Dirichlet()), (hq[1], O.Dirichlet())])
+            Hn = hH.copy()
+            O.sia_update(og, hH, Hn, hq, hB, dt, As=0.0, npow=3, mb=mb)
+            O.bc_apply(og, [(Hn, O.Neumann())])
+            hH = Hn
+        d = hH.arr[2:-2, 2:-2].T - H_obs
+        return 0.5 * float(np.sum(d * d))
+
+    Jc = J_cpu(logA0)
+    assert J > 0 and abs(J - Jc) <= 1e-12 * Jc, (J, Jc)
+    rng = np.random.default_rng(2)
+    h = 1e-4
+    for _ in range(4):
+        v = rng.standard_normal((n + 4, n + 4))
+        fd = (J_cpu(logA0 + h * v) - J_cpu(logA0 - h * v)) / (2 * h)
+        ad = float(np.sum(gn * v[2:-2, 2:-2].T))
+        assert abs(fd - ad) <= 1e-5 * abs(fd) + 1e-12 * J, (fd, ad)
