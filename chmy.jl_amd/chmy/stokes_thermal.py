@@ -7,7 +7,9 @@ dt_adv) with dt_adv = 0.1 min(Δ_D / max|V_D|) / N / 2.1 (∞ while V = 0);
 update_stress! + update_velocity! with the V batch, followed from the second
 time step on by update_thermal_flux!, update_thermal! and the T Neumann
 batch (one launch into the second of two T buffers, then the batch on it:
-the same values as the two launches in place); every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
+the same values as the two launches in place; in the first step, with no
+thermal update, the iterations between checks run in one native call);
+every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
 boundary vertices, scaled by τsc, lz/psc) decides convergence, and from the
 third step on ncheck = ceil(0.5 n_x).  With exact=True the states are
 bitwise those of the reference's kernel sequence.
@@ -72,16 +74,32 @@ class StokesThermal:
         S.dt = dt
         S.eta_ve_s = 1.0 / (1.0 / self.eta + 1.0 / (self.G * dt))
         err = None
-        if it > 1:   # both buffers start equal, so points no launch writes agree
-            self._Tbuf[1 - self._tp].data.copy_(self.T.data)
+        if it == 1:
+            # no thermal update: the iterations between checks run in one
+            # native call (in 2D two per pass), with the residual fields
+            # written by the last one
+            q = 0
+            while q < niter:
+                k = min(ncheck - q % ncheck, niter - q)
+                check = (q + k) % ncheck == 0
+                S.iterate(k, residual_last=check)
+                q += k
+                if check:
+                    err = S.residual(**self.scales)
+                    if all(e < eps for e in err):
+                        return q, err
+                    if not all(math.isfinite(e) for e in err):
+                        raise FloatingPointError(f"simulation failed, err = {err}")
+            return niter, err
+        # both T buffers start equal, so points no launch writes agree
+        self._Tbuf[1 - self._tp].data.copy_(self.T.data)
         for q in range(1, niter + 1):
             check = q % ncheck == 0
             S.iteration(write_res=check)
-            if it > 1:
-                src, dst = self._tp, 1 - self._tp
-                self.launch(self.arch, g, (thermal_step, (self._Tbuf[src], self._Tbuf[dst], self.T_old, self.qT,
-                                                          S.V, self.lam, dt, g)), bc=self._Tbcs[dst])
-                self._tp = dst
+            src, dst = self._tp, 1 - self._tp
+            self.launch(self.arch, g, (thermal_step, (self._Tbuf[src], self._Tbuf[dst], self.T_old, self.qT,
+                                                      S.V, self.lam, dt, g)), bc=self._Tbcs[dst])
+            self._tp = dst
             if check:
                 err = S.residual(**self.scales)
                 if all(e < eps for e in err):
