@@ -641,6 +641,8 @@ def main():
         pattern_probe(args)
         return
     if args.ttc:
+        if args.ttc == "2d" and args.n == 255:
+            args.n = 2047   # the 2D workload's grid
         time_to_converge(args)
         return
     if args.stokes_adjoint:
