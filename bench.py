@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: pseudo-transient 3D viscoelastic Stokes at 255³, FP64, one MI355X.
 
-A step is one PT iteration (examples/stokes_3d_inc_ve_T.jl:222-223): the fused
+A step is one PT iteration (examples/stokes_3d_inc_ve_T.jl:164-165): the fused
 stress+velocity kernel over the Launcher box plus the free-slip V boundary
 batch, on fields resident in HBM.  Reported value: T_eff = algorithmic bytes
 per iteration / time per iteration (GB/s), with algorithmic bytes = 27 array
@@ -60,7 +60,7 @@ def setup(n, dtype):
     from chmy import configs
     from chmy.stokes import PTStokes, pure_shear_initial, pure_shear_values
     arch = chmy.Arch(chmy.MI355XBackend())
-    g, eta, dt = configs.stokes3d_inclusion(arch, n)   # dt_diff (stokes_3d_inc_ve_T.jl:215), λ_ρCp = 1e-4
+    g, eta, dt = configs.stokes3d_inclusion(arch, n)   # dt_diff (stokes_3d_inc_ve_T.jl:157), λ_ρCp = 1e-4
     S = PTStokes(arch, g, dtype=dtype, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(3))
     pure_shear_initial(g, S.V)
     S.apply_bc()
@@ -519,74 +519,6 @@ def pairs_rate(S, args, isz):
                                  "algorithmic_bytes_per_iteration": A}}))
 
 
-def pattern_probe(args, nr=18, nw=10, reps=20):
-    """Streaming ceilings of access patterns (chmy_probe_stream): nr FP64
-    reads and nw writes per point of the Launcher box of an n^3 grid (the
-    fused 3D iteration's 17 reads + eta_ml and 10 writes), with no stencil
-    work, in linear order, in the fused kernel's 64 x 12 tiles with its
-    16-plane z-march (and 8-plane), and one plane per block."""
-    import ctypes as C
-    import torch
-    from chmy import _lib
-    from chmy.architectures import current_stream
-    n = args.n
-    m = n + 2
-    size = (n + 4) ** 3
-    ins = [torch.zeros(size, dtype=torch.float64, device="cuda") for _ in range(nr)]
-    outs = [torch.zeros(size, dtype=torch.float64, device="cuda") for _ in range(nw)]
-    pin = (C.c_void_p * nr)(*[t.data_ptr() for t in ins])
-    pout = (C.c_void_p * nw)(*[t.data_ptr() for t in outs])
-    nominal = (nr + nw) * m ** 3 * 8
-    rows = {}
-    cases = [("linear", 0, 12, 16, 1), ("tiles 64x12, 16-plane z-march", 1, 12, 16, 1),
-             ("tiles 64x12, 8-plane z-march", 1, 12, 8, 1), ("tiles 64x12, 16-plane z-march, no XCD remap", 1, 12, 16, 0),
-             ("tiles 64x12, one plane per block", 2, 12, 1, 1),
-             ("tiles 64x12, one plane per block, z neighbours re-read", 3, 12, 1, 1)]
-    for name, pat, by, kc, xcd in cases:
-        call = lambda: _lib.call("chmy_probe_stream", pat, pin, nr, pout, nw, n, by, kc, xcd, current_stream())
-        for _ in range(3):
-            call()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            call()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        rows[name] = {"ms": round(ms, 4), "GBs_nominal": round(nominal / (ms * 1e-3) / 1e9, 1)}
-        print(f"# {name}: {ms:.4f} ms", file=sys.stderr, flush=True)
-    # a plain copy of one array in linear order, for scale
-    pin1, pout1 = (C.c_void_p * 1)(ins[0].data_ptr()), (C.c_void_p * 1)(outs[0].data_ptr())
-    call = lambda: _lib.call("chmy_probe_stream", 0, pin1, 1, pout1, 1, n, 12, 16, 1, current_stream())
-    for _ in range(3):
-        call()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        call()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    rows["copy of one array, linear"] = {"ms": round(ms, 4), "GBs_nominal": round(2 * m ** 3 * 8 / (ms * 1e-3) / 1e9, 1)}
-    print(json.dumps({"metric": "access-pattern streaming ceiling", "n": n, "reads": nr, "writes": nw,
-                      "nominal_bytes": nominal, "patterns": rows}))
-
-
-def copy_calibration(args):
-    """A plain device copy of known size (1 GiB read + 1 GiB written per
-    launch): the reference point for scaling HBM counter bytes."""
-    import torch
-    nbytes = 1 << 30
-    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(args.steps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": args.steps}))
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -624,22 +556,11 @@ def main():
                     help="2D fused PT iteration rates at --n (default 2047): FP64, FP32, defect-correction sweep")
     ap.add_argument("--diffusion", action="store_true",
                     help="2D diffusion 256^2 x 1000 steps: reference launches vs fused step vs native loop, vs the oracle")
-    ap.add_argument("--pattern-probe", action="store_true",
-                    help="the fused 3D kernel's access pattern alone (18 reads, 10 writes per point, no stencil "
-                         "work) against linear order and one plane per block, at --n")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is out of scope)", file=sys.stderr)
         sys.exit(2)
-    if args.calibrate_copy:
-        copy_calibration(args)
-        return
-    if args.pattern_probe:
-        pattern_probe(args)
-        return
     if args.ttc:
         if args.ttc == "2d" and args.n == 255:
             args.n = 2047   # the 2D workload's grid

@@ -73,8 +73,6 @@ SIGNATURES = {
     "chmy_thermal_flux": [PG, PB, PF, PF, PF, D, VP],
     "chmy_thermal_update": [PG, PB, PF, PF, PF, D, VP],
     "chmy_thermal_step": [PG, PB, PF, PF, PF, PF, PF, D, D, VP],
-    "chmy_probe_stream": [C.c_int, P(C.c_void_p), C.c_int, P(C.c_void_p), C.c_int, C.c_int64, C.c_int, C.c_int,
-                          C.c_int, VP],
     "chmy_stokes_iter": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, C.c_int, D, D, D, D, D, D, D,
                          VP],
     "chmy_stokes_iterate": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32),

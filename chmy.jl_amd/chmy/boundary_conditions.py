@@ -10,7 +10,7 @@ side 2, fields in batch order (batch.jl:20-29).
 
 A condition's value is None (zero), a number, a Field over the tangential
 indices, or a BoundaryFunction (a plain function is taken as a continuous one
-over the reduced dimensions, boundary_function.jl:97-101).  batch() evaluates
+over the reduced dimensions, boundary_function.jl:78-81).  batch() evaluates
 a BoundaryFunction once per (field, dim, side) on the host, at the points
 first_order_boundary_condition.jl:42-84 evaluates it, into a value Field over
 the tangential range 0..n+2 that the device batch reads.
@@ -57,7 +57,7 @@ class FirstOrderBC:
 
 class BoundaryFunction:
     """BoundaryFunction(fun; discrete, parameters, reduce_dims)
-    (boundary_function.jl:57-95).  Continuous: fun(coordinates..., parameters...)
+    (boundary_function.jl:46-75).  Continuous: fun(coordinates..., parameters...)
     of the evaluation point, without the boundary's own dim when reduce_dims.
     Discrete: fun(grid, loc, dim, indices..., parameters...) with Chmy's logical
     indices (interior from 1), likewise reduced."""
@@ -70,7 +70,7 @@ class BoundaryFunction:
 
     def __call__(self, grid, loc, dim, I):
         """The value at logical index I (all dims) for location tuple loc
-        (boundary_function.jl:37-43)."""
+        (boundary_function.jl:34-44)."""
         keep = [d for d in range(grid.nd) if not (self.reduce_dims and d == dim)]
         if self.discrete:
             return self.fun(grid, loc, dim, *[I[d] for d in keep], *self.parameters)
@@ -83,7 +83,7 @@ class BoundaryFunction:
 def _as_value(value):
     if value is None or isinstance(value, (Field, BoundaryFunction)) or not callable(value):
         return value
-    return BoundaryFunction(value)   # FirstOrderBC{Kind}(f::Function) (boundary_function.jl:97-101)
+    return BoundaryFunction(value)   # FirstOrderBC{Kind}(f::Function) (boundary_function.jl:78-81)
 
 
 def Dirichlet(value=None):

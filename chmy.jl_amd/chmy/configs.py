@@ -3,7 +3,7 @@
 - stokes3d_inclusion: 3D VE Stokes, [-0.5,0.5]³, background η = 1, sphere of
   radius 0.1 with η = 1e-3, times exp(-T), T = 0.1·exp(-|x|²/0.05²), G = 1,
   dt from the reference's diffusion limit dt = Δ²/λ/N/2.1 with λ = 1e-4
-  (examples/stokes_3d_inc_ve_T.jl:151,215), pure-shear free-slip walls ±1.
+  (examples/stokes_3d_inc_ve_T.jl:93,157), pure-shear free-slip walls ±1.
 - stokes2d_inclusions: 2D VE Stokes, [-0.5,0.5]², 32 circular inclusions at
   seeded positions (centres U[-0.5,0.5]², radii U[0.01,0.04], log10 η
   U[-3,3]; later inclusions overwrite earlier ones), background η = 1,

@@ -1,6 +1,6 @@
 """Pseudo-transient viscoelastic Stokes solver on the MI355X library.
 
-The time/PT loop of examples/stokes_3d_inc_ve_T.jl:210-239 (2D:
+The time/PT loop of examples/stokes_3d_inc_ve_T.jl:152-184 (2D:
 stokes_2d_inc_ve_T.jl) for one physical time step, driven from the host:
 each PT iteration is one fused stress+velocity kernel (chmy_stokes_iter,
 double-buffered) plus the free-slip V batch; every `ncheck` iterations the
@@ -33,7 +33,7 @@ ITER_RESIDUAL, ITER_EXACT, ITER_PAIRS = 1, 2, 4
 
 
 def pt_params(grid, re_m=2.3 * math.pi, r=0.5):
-    """PT parameters (stokes_3d_inc_ve_T.jl:163-170)."""
+    """PT parameters (stokes_3d_inc_ve_T.jl:105-112)."""
     lt = min(grid.extent) / re_m
     vdt = min(spacing(grid)) / math.sqrt(grid.nd * 1.1)
     th = lt * (r + 4 / 3) / vdt
@@ -41,7 +41,7 @@ def pt_params(grid, re_m=2.3 * math.pi, r=0.5):
 
 
 def free_slip_batch(grid, V, normal_values):
-    """V.x => (x=Dirichlet(vl, vr), y=Neumann(), z=Neumann()), ... (stokes_3d_inc_ve_T.jl:188-190)
+    """V.x => (x=Dirichlet(vl, vr), y=Neumann(), z=Neumann()), ... (stokes_3d_inc_ve_T.jl:130-132)
     with per-side normal velocities normal_values[D] = (left, right)."""
     names = grid.axes_names()
     pairs = []
@@ -215,7 +215,7 @@ class PTStokes:
 
     def residual(self, tau_sc=1.0, lz=1.0, psc=1.0):
         """err = (max|∇V|·τsc, max|r_V.x|·lz/psc, ...) after zeroing boundary
-        residual vertices (stokes_3d_inc_ve_T.jl:229-233).  Requires the last
+        residual vertices (stokes_3d_inc_ve_T.jl:171-175).  Requires the last
         iteration to have written the residual fields."""
         names = self.grid.axes_names()
         key = id(self.r_V[0])
@@ -250,7 +250,7 @@ class PTStokes:
         return self.residual() + [err_tau]
 
     def solve(self, eps=1e-8, max_iter=100000, ncheck=None, callback=None):
-        """Iterate until all scaled residuals < eps (stokes_3d_inc_ve_T.jl:221-238).
+        """Iterate until all scaled residuals < eps (stokes_3d_inc_ve_T.jl:163-181).
         Raises on non-finite residuals."""
         ncheck = ncheck or 2 * self.grid.dims[0]
         err = None
@@ -316,7 +316,7 @@ class MixedPTStokes:
     starts from 0 and runs `inner` fused f32 iterations; then x += d
     (chmy_stokes_accumulate) and the real V batch is applied to x.  The
     convergence test uses the f64 residual of x: max|∇·V| and max|R_V| with
-    boundary residual vertices zeroed (stokes_3d_inc_ve_T.jl:229-233).
+    boundary residual vertices zeroed (stokes_3d_inc_ve_T.jl:171-175).
     """
 
     def __init__(self, arch, grid, *, eta_field=None, eta=None, G=1.0, dt=1.0, normal_values=None,

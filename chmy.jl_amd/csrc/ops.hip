@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) thermal_flux_kernel(Box3 b, DFL<T> Tf, DF
         const DFL<T>& Vd = d == 0 ? V0 : (d == 1 ? V1 : V2);
         const DFL<T>& qd = d == 0 ? q0 : (d == 1 ? q1 : q2);
         // qT_D = ((-λ ∂_D T) + max(V_D,0) left_D T) + min(V_D,0) right_D T
-        // (stokes_3d_inc_ve_T.jl:120-128); T Center: left = T[I-e], right = T[I]
+        // (stokes_3d_inc_ve_T.jl:59-71); T Center: left = T[I-e], right = T[I]
         const T tl = g_at(Tf, sh(I, d, -1));
         const T v = g_at(Vd, I);
         g_at(qd, I) = (nlam * ((tc - tl) * isp[d]) + jmax0(v) * tl) + jmin0(v) * tc;

@@ -1,5 +1,5 @@
 // Residual-norm reductions on the GPU: the reference's host-side
-// maximum(abs.(interior(X))) (examples/stokes_3d_inc_ve_T.jl:230-233) and a
+// maximum(abs.(interior(X))) (examples/stokes_3d_inc_ve_T.jl:172-175) and a
 // deterministic f64 sum of squares.  max|x| is exact and order-independent:
 // blocks reduce in registers (wave64 shuffles) and LDS, then combine with one
 // 64-bit atomic max per block on the bit pattern (non-negative doubles order

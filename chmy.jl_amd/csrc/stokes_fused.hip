@@ -1,6 +1,6 @@
 // One pseudo-transient Stokes iteration in a single pass over HBM:
-// update_stress! followed by update_velocity! (examples/stokes_3d_inc_ve_T.jl:
-// 222-223), fused and double-buffered.
+// update_stress! followed by update_velocity! (stokes_3d_inc_ve_T.jl:164-165),
+// fused and double-buffered.
 //
 // Work layout (3D): a block is 64 (x, one wave) x BY (y) threads; thread
 // (tx, ty) owns the logical column (i, j) = (i0-1+tx, j0-1+ty) and marches z
@@ -102,7 +102,7 @@ __device__ __forceinline__ void sto(T* p, uint32_t boff, T v) {
 #endif
 }
 
-// EXACT: the reference's expressions (stokes_3d_inc_ve_T.jl:91-114), bitwise.
+// EXACT: the reference's expressions (stokes_3d_inc_ve_T.jl:23-57), bitwise.
 // FAST: the same update with eta_ve*(1/(G dt) + 1/eta) = 1 used to cancel
 // tau/eta: tau_new = tau(1 - dtau_r) + eta_ve dtau_r (tau_old/(G dt) + 2 eps'),
 // eta_ve = eta/(1 + eta/(G dt)), Pr_new = Pr - divV eta_ve dtau_Pr and
@@ -910,7 +910,7 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
 }
 
 // ------------------------------------------------------------------ 3D, two PT iterations per pass
-// The V batch of a 3D free-slip solve (stokes_3d_inc_ve_T.jl:188-190): each
+// The V batch of a 3D free-slip solve (stokes_3d_inc_ve_T.jl:130-132): each
 // component Dirichlet on its own dim (vertex nodes 1 / n+1), Neumann on the
 // other two; v[component][dim][side] the scalar value, h the spacings.
 struct FreeSlip3 {
@@ -1777,7 +1777,7 @@ int chmy_stokes_iter(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fie
 
 // Several PT iterations in one call: each is chmy_stokes_iter from buffer
 // set p into set 1-p followed by the V boundary batch on set 1-p (the
-// Launcher's `bc=` of update_velocity!, examples/stokes_3d_inc_ve_T.jl:223),
+// Launcher's `bc=` of update_velocity!, examples/stokes_3d_inc_ve_T.jl:165),
 // so the host pays one call per `niter` iterations instead of ~4 launches and
 // descriptor packing per iteration.  The V batch is given once: bc_counts[2D+s]
 // entries per (dim, side), each naming a V component (bc_comp) and its

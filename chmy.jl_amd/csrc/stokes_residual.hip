@@ -3,7 +3,7 @@
 // by corrections d computed with f32 PT sweeps:
 //   1. R(x) in f64: R_tau = -(tau - tau_old)/(G dt) - tau/eta + 2 dev(eps(V)),
 //      R_P = -divg(V), R_V = -grad(Pr) + divg(tau) - rhog, each in the
-//      reference's expression order (examples/stokes_3d_inc_ve_T.jl:84-111),
+//      reference's expression order (examples/stokes_3d_inc_ve_T.jl:23-57),
 //      stored as the sweep's forcing in the sweep precision:
 //      tau_old_c = G dt R_tau, f_P = R_P, f_V = R_V;
 //   2. PT sweeps on d from 0 (chmy_stokes_iter with f_V/f_P, homogeneous BCs);
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) residual_kernel(Box3 b, ResArgs<T, TO> a)
         put(a.toc[c].at(O(a.toc[c])), a.Gdt * r);
     }
     put(a.fP.at(O(a.fP)), -dv);
-    // R_V (update_velocity!'s r_V, stokes_3d_inc_ve_T.jl:109-111 / 2D :39-40)
+    // R_V (update_velocity!'s r_V, stokes_3d_inc_ve_T.jl:51-53 / 2D stokes_2d_inc_ve_T.jl:39-40)
     const T P = a.Pr.at(oc);
     T r[3];
     if constexpr (ND == 2) {

@@ -198,8 +198,8 @@ int chmy_thermal_flux(const chmy_grid_t* grid, const chmy_box_t* box, const chmy
 /* update_thermal! (examples/stokes_3d_inc_ve_T.jl:73-77). */
 int chmy_thermal_update(const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* T,
                         const chmy_field_t* T_old, const chmy_field_t* qT, double dt, void* stream);
-/* update_thermal_flux! then update_thermal! (examples/stokes_3d_inc_ve_T.jl:
- * 166-167) in one launch over the same box, writing T_out (a second buffer,
+/* update_thermal_flux! then update_thermal! (stokes_3d_inc_ve_T.jl:167-168)
+ * in one launch over the same box, writing T_out (a second buffer,
  * distinct from T) instead of T; qT is stored as update_thermal_flux! stores
  * it.  T_out and qT equal the two launches' T and qT bitwise (build
  * extension). */
@@ -208,7 +208,7 @@ int chmy_thermal_step(const chmy_grid_t* grid, const chmy_box_t* box, const chmy
                       const chmy_field_t* V, double lambda_rhoCp, double dt, void* stream);
 
 /* One fused PT iteration = update_stress! then update_velocity! over the
- * Launcher box (examples/stokes_3d_inc_ve_T.jl:222-223), double-buffered:
+ * Launcher box (examples/stokes_3d_inc_ve_T.jl:164-165), double-buffered:
  * reads (V, tau, Pr) and writes (V_out, tau_out, Pr_out), so no block reads a
  * value another block has already advanced.  tau/Pr/V outside the box keep
  * their stored values, so V_out/tau_out/Pr_out must equal the inputs there
@@ -232,7 +232,7 @@ int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_
                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                      double nudtau, void* stream);
 
-/* niter PT iterations in one call (the inner loop of stokes_3d_inc_ve_T.jl:221-227):
+/* niter PT iterations in one call (the inner loop of stokes_3d_inc_ve_T.jl:163-169):
  * chmy_stokes_iter from buffer set p = *parity_inout into set 1-p, then the V
  * batch on set 1-p, alternating.  The V batch: bc_counts[2*dim+side-1]
  * entries per (dim, side) in bc! order, entry q applies bcs[q] to V component
@@ -256,7 +256,7 @@ int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const c
 
 /* Defect correction (build extension; the reference has no mixed-precision
  * path).  Residuals of an f64 state in the reference's expression order
- * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:84-111):
+ * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:23-57):
  * R_tau = -(tau - tau_old)/(G dt) - tau/eta + 2 dev(eps(V)), R_P = -divg(V),
  * R_V = -grad(Pr) + divg(tau) - rhog (+ f_ext, an optional f64 vector field:
  * the forcing of an adjoint solve), over the Launcher box.  Writes the
@@ -336,22 +336,8 @@ int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckp
                      int npow, double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot,
                      void* stream);
 
-/* ---------------------------------------------------------------- measurement */
-/* Measurement aid, no counterpart in the reference (its callers: bench.py's
- * access-pattern ceilings).  Reads nr FP64 arrays in[] and writes nw arrays
- * out[] (18 and 10, the fused iteration's counts, or 1 and 1), each the
- * Center storage of an n^3 grid ((n+4)^3
- * elements, n <= 1000), over the Launcher box [0, n+1]^3 with one sum per
- * point, in pattern 0 (linear, x fastest), 1 (the fused 3D iteration's tiles:
- * 64 x by lanes, 62 x (by-1) outputs, z-march of kchunk planes plus one, XCD
- * remap when xcd != 0), 2 (the same tiles, one plane per block) or 3 (as 2,
- * each point also reading every input at plane k-1 and the first three at
- * k+1: the z neighbours a kernel without a z-march re-reads). */
-int chmy_probe_stream(int pattern, const double* const* in, int nr, double* const* out, int nw, int64_t n, int by,
-                      int kchunk, int xcd, void* stream);
-
 /* ---------------------------------------------------------------- reductions */
-/* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:230-233),
+/* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:172-175),
  * exact (order-independent); result in f64 written to *out_dev (device).
  * NaN in the box yields NaN. */
 int chmy_maxabs_dev(const chmy_field_t* f, const chmy_box_t* box, double* out_dev, void* stream);

@@ -433,7 +433,7 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
             r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[2], g, 0, I));
             if (hasrg) r[1] = r[1] - rg(I);
         } else {
-            // examples/stokes_3d_inc_ve_T.jl:109-111
+            // examples/stokes_3d_inc_ve_T.jl:51-53
             r[0] = ((-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[3], g, 1, I)) + dd(t[4], g, 2, I);
             r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[3], g, 0, I)) + dd(t[5], g, 2, I);
             r[2] = ((-dd(Pr, g, 2, I) + dd(t[2], g, 2, I)) + dd(t[4], g, 0, I)) + dd(t[5], g, 1, I);

@@ -101,27 +101,6 @@ def test_runtime_helpers_validate_on_host():
     assert rc == -1 and b"parity" in L.chmy_last_error()
 
 
-def test_probe_validates_on_host():
-    """The access-pattern probe takes the fused iteration's counts (18 reads,
-    10 writes) or a copy (1, 1), a pattern in 0..2, tile rows 4/8/12/16 and a
-    grid of at most 1000³, and rejects anything else before a launch."""
-    L = _lib.lib()
-    ins = (C.c_void_p * 18)(*([0x1000] * 18))
-    outs = (C.c_void_p * 10)(*([0x2000] * 10))
-    cases = [((4, ins, 18, outs, 10, 255, 12, 16, 1), b"pattern"),
-             ((1, ins, 17, outs, 10, 255, 12, 16, 1), b"18 inputs"),
-             ((1, ins, 18, outs, 9, 255, 12, 16, 1), b"18 inputs"),
-             ((1, ins, 18, outs, 10, 1001, 12, 16, 1), b"not in 1..1000"),
-             ((1, ins, 18, outs, 10, 255, 10, 16, 1), b"rows"),
-             ((1, ins, 18, outs, 10, 255, 12, 0, 1), b"kchunk")]
-    for args, msg in cases:
-        assert L.chmy_probe_stream(*args, None) < 0
-        assert msg in L.chmy_last_error(), (args, L.chmy_last_error())
-    nulls = (C.c_void_p * 18)(*([0x1000] * 17 + [0]))
-    assert L.chmy_probe_stream(1, nulls, 18, outs, 10, 255, 12, 16, 1, None) < 0
-    assert b"NULL" in L.chmy_last_error()
-
-
 def test_adjoint_entry_points_validate_on_host():
     """SIA run/reverse and the Stokes gradient pass reject misuse before any
     launch: missing arrays, aliasing, checkpoint counts, wrong locations."""

@@ -21,7 +21,7 @@ def arch():
 
 
 def test_center_dirichlet_reduced_dims_matches_reference_example(arch):
-    """The docstring example of boundary_function.jl:64-75: on a Center field,
+    """The docstring example of boundary_function.jl:60-70: on a Center field,
     x = Dirichlet(BoundaryFunction((y, ly) -> y (ly - y); parameters=(2,))),
     y = Neumann(): the wall value (f[0] + f[1]) / 2 equals the function at the
     ghost's tangential Center coordinate, on both sides, over the tangential
@@ -105,7 +105,7 @@ def test_neumann_discrete_on_vertex_field(arch):
 
 def test_plain_function_and_1d(arch):
     """A plain function is a continuous, reduced BoundaryFunction
-    (boundary_function.jl:97-101); in 1D it reduces to a number per side."""
+    (boundary_function.jl:78-81); in 1D it reduces to a number per side."""
     g = chmy.UniformGrid(arch, origin=(0.0,), extent=(1.0,), dims=(16,))
     og = ogrid(g)
     rng = np.random.default_rng(6)

@@ -62,8 +62,8 @@ def test_sia_flux_update_bitwise(arch, with_A):
 
 
 def test_sia_fused_step_equals_reference_sequence(arch):
-    """chmy_sia_step + Neumann == update_q!, q Dirichlet (all sides, sia.jl:73),
-    update_H!, H Neumann (sia.jl:81), for 20 steps, bitwise."""
+    """chmy_sia_step + Neumann == update_q!, q Dirichlet (all sides, sia.jl:72-73),
+    update_H!, H Neumann (sia.jl:80-81), for 20 steps, bitwise."""
     g, og, H, hH, B, hB, A, hA = _setup(arch, seed=3)
     S = SIA(arch, g, hB.arr, logA=np.log(hA.arr), As=0.5, npow=3, mb=(0.5, 0.3, 0.25))
     S.reset(hH.arr)

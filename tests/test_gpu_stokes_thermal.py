@@ -24,7 +24,7 @@ def arch():
 
 
 def _incl(g, locs, inside, outside, r):
-    """init_incl on the interior at the field's coordinates (stokes_3d_inc_ve_T.jl:123-125)."""
+    """init_incl on the interior at the field's coordinates (stokes_3d_inc_ve_T.jl:125-127)."""
     nd = g.nd
     n = [g.dims[d] + (1 if locs[d] == "V" else 0) for d in range(nd)]
     xs = [g.coords(Vv() if locs[d] == "V" else Cc(), d, first=1, count=n[d]) for d in range(nd)]

@@ -170,7 +170,7 @@ def _stokes_fields(g, T=np.float64):
 
 def _bc_V(g, F, vals):
     """Free slip with Dirichlet normal velocity (vals[D] = (left, right)),
-    tangential Neumann (examples/stokes_3d_inc_ve_T.jl:188-190)."""
+    tangential Neumann (examples/stokes_3d_inc_ve_T.jl:130-132)."""
     nd = g.nd
     pairs = []
     for D in range(nd):
@@ -198,7 +198,7 @@ def _pure_shear(g, F):
 def test_stokes_pure_shear_is_a_fixed_point(nd):
     """Homogeneous η + pure-shear Dirichlet: linear V, Pr = 0 and
     τ = 2η_ve·ε̇ (τ_old = 0) make r_τ = 0 and r_V = 0, so one PT iteration
-    leaves the state unchanged (examples/stokes_3d_inc_ve_T.jl:81-115)."""
+    leaves the state unchanged (examples/stokes_3d_inc_ve_T.jl:23-57)."""
     n = 8
     g = O.Grid((-0.5,) * nd, (1.0,) * nd, (n,) * nd)
     F = _stokes_fields(g)
@@ -347,7 +347,7 @@ def test_stokes_residual_matches_update_kernels(nd, visc):
     """The defect-correction residual restatement agrees with the update
     kernels it is read from: R_tau = (tau_new - tau)/(eta_ve dtau_r) of
     update_stress!, R_P = -divg(V) (the stored ∇V), R_V = update_velocity!'s r_V
-    (examples/stokes_3d_inc_ve_T.jl:92-111)."""
+    (examples/stokes_3d_inc_ve_T.jl:105-112)."""
     n = (9, 8, 7)[:nd]
     g = O.Grid((-0.5,) * nd, (1.0,) * nd, n)
     rng = np.random.default_rng(11 + nd)
