@@ -98,8 +98,8 @@ def _materialise(grid, f, d, side, cond):
     """A BoundaryFunction condition of field f on dim d, side (0 or 1) as a
     condition with a value Field (or a number in 1D).  The point and location
     are those of first_order_boundary_condition.jl: a Vertex-located Dirichlet
-    is evaluated at the boundary node (1 or m) with f's location (:46-50), a
-    Center-located Dirichlet at the ghost (0 or m+1) with Center (:56-68), a
+    is evaluated at the boundary node (1 or m) with f's location (:50-54), a
+    Center-located Dirichlet at the ghost (0 or m+1) with Center (:59-71), a
     Neumann at the ghost with the flipped location (:76-84); tangential indices
     run over 0..n+2."""
     bf = cond.value

@@ -5,8 +5,8 @@ stokes_2d_inc_ve_T.jl) for one physical time step, driven from the host:
 each PT iteration is one fused stress+velocity kernel (chmy_stokes_iter,
 double-buffered) plus the free-slip V batch; every `ncheck` iterations the
 kernel also writes ∇V and r_V, the boundary residual vertices are zeroed
-(Dirichlet on r_V, :229) and the scaled L∞ norms are reduced on the device
-(:230-237).
+(Dirichlet on r_V, :171) and the scaled L∞ norms are reduced on the device
+(:172-175).
 
 Variable viscosity (a build extension; the reference uses a scalar η): η is a
 Center field given on its full padded storage; η_ml (its max over the 3^N

@@ -1,4 +1,4 @@
-"""The physical time loop of examples/stokes_3d_inc_ve_T.jl (:150-240, 2D
+"""The physical time loop of examples/stokes_3d_inc_ve_T.jl (:152-184, 2D
 stokes_2d_inc_ve_T.jl) on the fused PT iteration.
 
 Per time step it: update_old! (T, τ → T_old, τ_old); dt = min(dt_diff,

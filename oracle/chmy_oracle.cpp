@@ -66,6 +66,38 @@ namespace {
 int g_err = 0;
 char g_msg[256];
 
+// Planted mistakes (test infrastructure for the pins themselves): a test may
+// switch on one plausible error at a time (orc_set_plant) to show that the
+// check pinning that oracle function fails on it.  0, the default, is the
+// restatement; with 0 every site computes exactly what it computed before
+// the switch existed (one well-predicted branch on a global).
+enum Plant {
+    PL_NONE = 0,
+    PL_VERTEX_DIFF = 1,      // ∂ of a Vertex field with knots (I-e, I)
+    PL_LERP_KNOTS = 2,       // Center->Vertex lerp knots (I, I+e)
+    PL_CENTER_DIRICHLET = 3, // Center Dirichlet ghost with t = 1
+    PL_NEUMANN_SIGN = 4,     // Neumann flux not negated on side 1
+    PL_BC_ORDER = 5,         // batch dims 1..N instead of N..1 (oracle.py)
+    PL_DIFF_FLUX = 6,        // compute_q! without the minus sign
+    PL_STRESS_DEV = 7,       // diagonal r_tau without the -divV/3 term
+    PL_EDGE_SWAP = 8,        // eta of tau_xz and tau_yz lerped to each other's edge
+    PL_RV_SWAP = 9,          // r_Vx with dz(tau_yz) for dz(tau_xz); 2D dx(tau_xy) for dy(tau_xy)
+    PL_VEL_ML_SIDE = 10,     // velocity coefficient from max(eta_ml[I], eta_ml[I+e])
+    PL_MAXLOC_2N = 11,       // maxloc over the 2^N cells I-e..I instead of 3^N
+    PL_THERMAL_UPWIND = 12,  // upwind knots swapped (left for V<0, right for V>0)
+    PL_SIA_EXP = 13,         // D with A H^(n+1) instead of A H^(n+2)
+    PL_SIA_GRAD = 14,        // D without |∇S|^(n-1)
+    PL_SIA_M = 15,           // M = max(mb_a (S - mb_b), mb_c)
+    PL_RESID_RP = 16,        // R_P = +divg(V)
+    PL_PR_UPDATE = 17,       // Pr += divV eta_ve dtau_Pr
+    PL_GRAD_EDGE = 18,       // gradient edge terms weighted 1/2 instead of 1/4
+    PL_PT_PARAMS = 19,       // theta with (r + 1/3) (oracle.py pt_params)
+    PL_VMAG = 20,            // vmag without the square root
+    PL_DIVG = 21,            // divg = dx Vx - dy Vy
+    PL_THERMAL_SIGN = 22,    // update_thermal! with T_old + dt divg(qT)
+};
+int g_plant = 0;
+
 void fail(const char* m) {
     if (!g_err) { g_err = 1; std::snprintf(g_msg, sizeof g_msg, "%s", m); }
 }
@@ -137,6 +169,7 @@ template <class T> inline T rightv(const FV<T>& f, int D, const Ix& I) {
 }
 // ∂ = δ * iΔ (src/GridOperators/partial_derivatives.jl:2-5)
 template <class T> inline T dd(const FV<T>& f, const G<T>& g, int D, const Ix& I) {
+    if (g_plant == PL_VERTEX_DIFF && f.loc[D] == 1) return (f(I) - f(shift(I, D, -1))) * g.isp[D];
     return (rightv(f, D, I) - leftv(f, D, I)) * g.isp[D];
 }
 
@@ -148,7 +181,7 @@ T itp_rec(const FV<T>& f, const int* dims, int level, const int* to, const Ix& I
     if (level == 0) return f(I);
     int D = dims[level - 1];
     Ix Il = I, Ir = I;
-    if (f.loc[D] == 0 && to[D] == 1) { Il = shift(I, D, -1); }
+    if (f.loc[D] == 0 && to[D] == 1 && g_plant != PL_LERP_KNOTS) { Il = shift(I, D, -1); }
     else { Ir = shift(I, D, +1); }
     T a = itp_rec(f, dims, level - 1, to, Il);
     T b = itp_rec(f, dims, level - 1, to, Ir);
@@ -183,7 +216,7 @@ template <class T> T hlerp(const FV<T>& f, const int* to, const Ix& I) {
 // divg: left-associated Σ ∂_D V_D (src/GridOperators/field_operators.jl:50-59)
 template <class T> T divg(const FV<T>* V, const G<T>& g, const Ix& I) {
     T s = dd(V[0], g, 0, I);
-    for (int d = 1; d < g.nd; ++d) s = s + dd(V[d], g, d, I);
+    for (int d = 1; d < g.nd; ++d) s = g_plant == PL_DIVG && d == 1 ? s - dd(V[d], g, d, I) : s + dd(V[d], g, d, I);
     return s;
 }
 // vmag = sqrt(Σ lerp(V_D, Center)^2) (field_operators.jl:116-125)
@@ -192,7 +225,7 @@ template <class T> T vmag(const FV<T>* V, const G<T>& g, const Ix& I) {
     T a = lerp(V[0], c, I);
     T s = a * a;
     for (int d = 1; d < g.nd; ++d) { T b = lerp(V[d], c, I); s = s + b * b; }
-    return std::sqrt(s);
+    return g_plant == PL_VMAG ? s : std::sqrt(s);
 }
 // ∂² (partial_derivatives.jl:7-12): from = loc = location(f).
 template <class T> T d2(const FV<T>& f, const G<T>& g, int D, const Ix& I) {
@@ -265,10 +298,10 @@ void bc_one(const G<T>& g, int D, int side, const FV<T>& f, const orc_bc& bc, Ix
     if (bc.kind == 0) {                           // Center Dirichlet, t = 2 on uniform grids
         T a = value(If);
         T b = f(If2);
-        f(If) = std::fma(T(2), a - b, b);
+        f(If) = std::fma(g_plant == PL_CENTER_DIRICHLET ? T(1) : T(2), a - b, b);
     } else {                                      // Neumann, any location
         T q = value(If);
-        T qs = side == 1 ? -q : q;
+        T qs = side == 1 && g_plant != PL_NEUMANN_SIGN ? -q : q;
         T h = g.sp[D];
         f(If) = std::fma(h, qs, f(If2));
     }
@@ -311,7 +344,7 @@ template <class T> void compute_q_t(const orc_grid* gg, const orc_box* b, const 
     alignas(FV<T>) unsigned char buf[3 * sizeof(FV<T>)];
     FV<T>* Q = reinterpret_cast<FV<T>*>(buf);
     views(q, g.nd, Q);
-    T nchi = -T(chi);
+    T nchi = g_plant == PL_DIFF_FLUX ? T(chi) : -T(chi);
     for_box(b, g.nd, [&](const Ix& I) {
         for (int d = 0; d < g.nd; ++d) Q[d](I) = nchi * dd(c, g, d, I);
     });
@@ -366,6 +399,10 @@ template <class T> void stress_t(const orc_grid* gg, const orc_box* b, const orc
     // tau component locations: 2D (xx,yy,xy), 3D (xx,yy,zz,xy,xz,yz)
     static const int L2[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 1, 0}};
     static const int L3[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    static const int L3swap[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+    const auto& L3u = g_plant == PL_EDGE_SWAP ? L3swap : L3;
+    const T dev3 = g_plant == PL_STRESS_DEV ? T(0) : T(1);
+    const T prs = g_plant == PL_PR_UPDATE ? T(-1) : T(1);
     for_box(b, nd, [&](const Ix& I) {
         T e[6];
         if (nd == 2) {
@@ -383,17 +420,18 @@ template <class T> void stress_t(const orc_grid* gg, const orc_box* b, const orc
         dV(I) = divg(V, g, I);
         T eta_c = eta_s, eve_c = eve_s;
         if (fieldvisc) { eta_c = eta(I); eve_c = visc_ve(eta_c, Gs, dts); }
-        Pr(I) = Pr(I) - (dV(I) * eve_c) * dPr;
+        if (prs > T(0)) Pr(I) = Pr(I) - (dV(I) * eve_c) * dPr;
+        else Pr(I) = Pr(I) + (dV(I) * eve_c) * dPr;
         const int ndiag = nd;
         for (int c = 0; c < nt; ++c) {
             T et = eta_c, ev = eve_c;
             if (fieldvisc && c >= ndiag) {
-                et = lerp(eta, nd == 2 ? L2[c] : L3[c], I);
+                et = lerp(eta, nd == 2 ? L2[c] : L3u[c], I);
                 ev = visc_ve(et, Gs, dts);
             }
             T t = tau_v[c](I), to = tauo_v[c](I);
             T r;
-            if (c < ndiag) r = ((-(t - to)) / Gdt - t / et) + T(2) * (e[c] - dV(I) / T(3));
+            if (c < ndiag) r = ((-(t - to)) / Gdt - t / et) + T(2) * (e[c] - dev3 * (dV(I) / T(3)));
             else r = ((-(t - to)) / Gdt - t / et) + T(2) * e[c];
             tau_v[c](I) = t + (r * ev) * dr;
         }
@@ -425,16 +463,18 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
     const bool fieldvisc = vs.eta_ml != nullptr;
     FV<T> eml(fieldvisc ? vs.eta_ml : Pr_);
     const T nu = T(nudtau), eve_s = T(vs.eta_ve_s), Gs = T(vs.G), dts = T(vs.dt);
+    const bool rvswap = g_plant == PL_RV_SWAP;
+    const long mlside = g_plant == PL_VEL_ML_SIDE ? 1 : 0;
     for_box(b, nd, [&](const Ix& I) {
         T r[3];
         if (nd == 2) {
             // examples/stokes_2d_inc_ve_T.jl:39-40
-            r[0] = (-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[2], g, 1, I);
+            r[0] = (-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[2], g, rvswap ? 0 : 1, I);
             r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[2], g, 0, I));
             if (hasrg) r[1] = r[1] - rg(I);
         } else {
             // examples/stokes_3d_inc_ve_T.jl:51-53
-            r[0] = ((-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[3], g, 1, I)) + dd(t[4], g, 2, I);
+            r[0] = ((-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[3], g, 1, I)) + dd(t[rvswap ? 5 : 4], g, 2, I);
             r[1] = ((-dd(Pr, g, 1, I) + dd(t[1], g, 1, I)) + dd(t[3], g, 0, I)) + dd(t[5], g, 2, I);
             r[2] = ((-dd(Pr, g, 2, I) + dd(t[2], g, 2, I)) + dd(t[4], g, 0, I)) + dd(t[5], g, 1, I);
             if (hasrg) r[2] = r[2] - rg(I);
@@ -444,14 +484,15 @@ template <class T> void velocity_t(const orc_grid* gg, const orc_box* b, const o
         for (int d = 0; d < nd; ++d) {
             rV[d](I) = r[d];
             T ev = eve_s;
-            if (fieldvisc) ev = visc_ve(jmax(eml(shift(I, d, -1)), eml(I)), Gs, dts);
+            if (fieldvisc) ev = visc_ve(jmax(eml(shift(I, d, mlside - 1)), eml(shift(I, d, mlside))), Gs, dts);
             V[d](I) = V[d](I) + (rV[d](I) * nu) / ev;
         }
     });
 }
 
-// Residuals of a PT state (defect correction): the bracket of update_stress!
-// (r_tau, :92-97), -divg(V) and update_velocity!'s r_V (:109-111), f64.
+// Residuals of a PT state (defect correction), f64: r_tau, the bracket of
+// update_stress! (stokes_3d_inc_ve_T.jl:34-39); -divg(V); and r_V of
+// update_velocity! (stokes_3d_inc_ve_T.jl:51-53).
 template <class T> void residual_t(const orc_grid* gg, const orc_box* b, const orc_field* V_, const orc_field* tau,
                                    const orc_field* Pr_, const orc_field* tauo, const orc_field* eta_, const orc_field* rg_,
                                    double eta_s, double G_, double dt_, const orc_field* Rt, const orc_field* RV,
@@ -495,7 +536,7 @@ template <class T> void residual_t(const orc_grid* gg, const orc_box* b, const o
             rt[c](I) = c < nd ? ((-(tt - tto)) / Gdt - tt / et) + T(2) * (e[c] - dv / T(3))
                               : ((-(tt - tto)) / Gdt - tt / et) + T(2) * e[c];
         }
-        rp(I) = -dv;
+        rp(I) = g_plant == PL_RESID_RP ? dv : -dv;
         T r[3];
         if (nd == 2) {
             r[0] = (-dd(Pr, g, 0, I) + dd(t[0], g, 0, I)) + dd(t[2], g, 1, I);
@@ -566,7 +607,7 @@ template <class T> void stokes_grad_t(const orc_grid* gg, const orc_field* tau, 
                     ge = ge + (ta[c](E) * t[c](E)) / (et * et);
                 }
         }
-        T r = gd + T(0.25) * ge;
+        T r = gd + (g_plant == PL_GRAD_EDGE ? T(0.5) : T(0.25)) * ge;
         if (logscale) r = r * ec;
         gr(I) = r;
     });
@@ -600,10 +641,12 @@ template <class T> void thermal_flux_t(const orc_grid* gg, const orc_box* b, con
     views(V_, g.nd, V);
     FV<T> TT(Tf);
     const T nl = -T(lam);
+    const bool swp = g_plant == PL_THERMAL_UPWIND;
     for_box(b, g.nd, [&](const Ix& I) {
         for (int d = 0; d < g.nd; ++d) {
             T v = V[d](I);
-            q[d](I) = (nl * dd(TT, g, d, I) + jmax0(v) * leftv(TT, d, I)) + jmin0(v) * rightv(TT, d, I);
+            const T lt = swp ? rightv(TT, d, I) : leftv(TT, d, I), rt = swp ? leftv(TT, d, I) : rightv(TT, d, I);
+            q[d](I) = (nl * dd(TT, g, d, I) + jmax0(v) * lt) + jmin0(v) * rt;
         }
     });
 }
@@ -615,20 +658,21 @@ template <class T> void thermal_t(const orc_grid* gg, const orc_box* b, const or
     FV<T>* q = reinterpret_cast<FV<T>*>(b1);
     views(qT, g.nd, q);
     FV<T> TT(Tf), TO(Told);
-    const T Dt = T(dt);
+    const T Dt = g_plant == PL_THERMAL_SIGN ? -T(dt) : T(dt);
     for_box(b, g.nd, [&](const Ix& I) { TT(I) = TO(I) - Dt * divg(q, g, I); });
 }
 
 template <class T> void maxloc_t(const orc_box* b, const orc_field* dst, const orc_field* src) {
     FV<T> d(dst), s(src);
     const int nd = d.nd;
+    const long up = g_plant == PL_MAXLOC_2N ? 0 : 1;
     for_box(b, nd, [&](const Ix& I) {
         T m = s(I);
         long r[3];
         for (int q = 0; q < 3; ++q) r[q] = q < nd ? 1 : 0;
-        for (long c = -r[2]; c <= r[2]; ++c)
-            for (long bb = -r[1]; bb <= r[1]; ++bb)
-                for (long a = -r[0]; a <= r[0]; ++a) {
+        for (long c = -r[2]; c <= r[2] * up; ++c)
+            for (long bb = -r[1]; bb <= r[1] * up; ++bb)
+                for (long a = -r[0]; a <= r[0] * up; ++a) {
                     Ix J{{I.v[0] + a, I.v[1] + bb, I.v[2] + c}};
                     m = jmax(m, s(J));
                 }
@@ -730,11 +774,14 @@ template <class T> struct Sia {
     T D(Ix I) const {
         T h = H(I);
         T a = hasA ? A(I) : T(1);
-        return (a * ipow(h, p.npow + 2) + T(p.As) * ipow(h, p.npow)) * ipow(vmagS(I), p.npow - 1);
+        const int e2 = g_plant == PL_SIA_EXP ? p.npow + 1 : p.npow + 2;
+        const T grad = g_plant == PL_SIA_GRAD ? T(1) : ipow(vmagS(I), p.npow - 1);
+        return (a * ipow(h, e2) + T(p.As) * ipow(h, p.npow)) * grad;
     }
     T M(Ix I) const {
         T m = T(p.mb_a) * (S(I) - T(p.mb_b));
         T c = T(p.mb_c);
+        if (g_plant == PL_SIA_M) return m > c ? m : c;
         return m < c ? m : c;
     }
 };
@@ -788,6 +835,10 @@ int orc_last_error(char* buf, int len) {
     g_err = 0;
     return e;
 }
+
+// Planted mistakes (see enum Plant): returns the previous setting.
+int orc_set_plant(int k) { int old = g_plant; g_plant = k; return old; }
+int orc_get_plant(void) { return g_plant; }
 
 // UniformAxis coordinates (src/Grids/uniform_axis.jl:18-19).
 double orc_vertex(double origin, double spacing, long i) { return std::fma(double(i - 1), spacing, origin); }

@@ -61,6 +61,8 @@ P = C.POINTER
 
 def _declare(L):
     L.orc_last_error.argtypes = [C.c_char_p, C.c_int]
+    L.orc_set_plant.argtypes = [C.c_int]
+    L.orc_get_plant.argtypes = []
     L.orc_vertex.restype = C.c_double
     L.orc_vertex.argtypes = [C.c_double, C.c_double, C.c_long]
     L.orc_center.restype = C.c_double
@@ -104,6 +106,34 @@ def _check(rc):
 
 
 CENTER, VERTEX = 0, 1
+
+# Planted mistakes (chmy_oracle.cpp, enum Plant): used only by the tests that
+# show each pin fails on a plausible error.
+PLANTS = {
+    "vertex_diff": 1, "lerp_knots": 2, "center_dirichlet": 3, "neumann_sign": 4, "bc_order": 5,
+    "diff_flux": 6, "stress_dev": 7, "edge_swap": 8, "rv_swap": 9, "vel_ml_side": 10, "maxloc_2n": 11,
+    "thermal_upwind": 12, "sia_exp": 13, "sia_grad": 14, "sia_m": 15, "resid_rp": 16, "pr_update": 17,
+    "grad_edge": 18, "pt_params": 19, "vmag": 20, "divg": 21, "thermal_sign": 22,
+}
+
+
+class planted:
+    """Context manager: run the oracle with one planted mistake."""
+
+    def __init__(self, name):
+        self.k = PLANTS[name]
+
+    def __enter__(self):
+        self.old = lib().orc_set_plant(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(self.old)
+        return False
+
+
+def _plant():
+    return lib().orc_get_plant()
 
 
 class Grid:
@@ -276,7 +306,8 @@ def bc_apply(grid, pairs):
     dims/sides), a dict {dim: BC or (BC, BC)} (missing dims: nothing), per
     batch.jl:72-157.  Applies dims N..1, side 1 then 2, fields in order."""
     nd = grid.nd
-    for D in reversed(range(nd)):
+    order = range(nd) if _plant() == PLANTS["bc_order"] else reversed(range(nd))
+    for D in order:
         for s in (1, 2):
             fs, bs = [], []
             for f, spec in pairs:
@@ -410,9 +441,9 @@ def sia_Dmax(grid, H, B, A=None, As=0.5, npow=3, mb=(0.5, 0.3, 0.25)):
 
 # ------------------------------------------------------------ PT parameters
 def pt_params(grid, extent=None, re_m=2.3 * np.pi, r=0.5):
-    """examples/stokes_3d_inc_ve_T.jl:163-170 (2D: stokes_2d_inc_ve_T.jl:87-94)."""
+    """examples/stokes_3d_inc_ve_T.jl:105-112 (2D: stokes_2d_inc_ve_T.jl:87-94)."""
     L = extent or grid.extent
     lt = min(L) / re_m
     vdt = min(grid.spacing) / np.sqrt(grid.nd * 1.1)
-    th = lt * (r + 4 / 3) / vdt
+    th = lt * (r + (1 / 3 if _plant() == PLANTS["pt_params"] else 4 / 3)) / vdt
     return dict(dtau_r=1.0 / (th + 1.0), nudtau=vdt * lt, dtau_Pr=r / th)

@@ -252,8 +252,8 @@ def test_boundary_function_points_follow_the_reference():
     """batch() evaluates a BoundaryFunction at the points and locations
     first_order_boundary_condition.jl uses (host only, no device): a
     Vertex-located Dirichlet at the boundary nodes 1 and m with the field's
-    location (:46-50), a Center-located Dirichlet at the ghosts 0 and m+1 with
-    Center (:56-68), a Neumann at the ghosts with the flipped location
+    location (:50-54), a Center-located Dirichlet at the ghosts 0 and m+1 with
+    Center (:59-71), a Neumann at the ghosts with the flipped location
     (:76-84), over the tangential range 0..n+2."""
     from chmy import Center as Cc, Vertex as Vv
     g = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 2.0), dims=(4, 5))
