@@ -919,7 +919,7 @@ struct FreeSlip3 {
 };
 
 #ifndef CHMY_FUSED_TB3_BY64
-#define CHMY_FUSED_TB3_BY64 8
+#define CHMY_FUSED_TB3_BY64 12
 #endif
 #ifndef CHMY_FUSED_TB3_BY32
 #define CHMY_FUSED_TB3_BY32 12
@@ -930,40 +930,71 @@ struct FreeSlip3 {
 template <class T>
 constexpr int kTB3Y = std::is_same<T, double>::value ? CHMY_FUSED_TB3_BY64 : CHMY_FUSED_TB3_BY32;
 constexpr int kTB3Z = CHMY_FUSED_TB3_ZCHUNK;
+static_assert(kTB3Z >= 2, "B's first plane of a chunk above the bottom one must be >= 1");
 
-// Two PT iterations in one pass over a 64 x BY tile marching z: the first
-// iteration (A) is fused3d_kernel's step on planes kb-2 .. ke+1 with nothing
-// stored; its velocities get the free-slip batch in-kernel (per point, the
-// batch's own order: z, then y through an LDS exchange of the neighbouring
-// rows, then x through lane shuffles; the reference's fma steps); the second
-// iteration (B) trails A by two planes with its own LDS planes and stores
-// planes kb .. ke-1.  A is exact on lanes 1..62 and rows 1..BY-1 (rows for
-// its velocities; stresses one more), B's outputs are lanes 2..61 and rows
-// 2..BY-2 (Center values, V) and 1..BY-3 (edges at j+1).  The eta and eta_ml
-// planes are 4-plane rings that B reads two planes behind A; B re-reads
-// tau_old and the forcing of its plane.  Rows, columns and planes are
-// clamped into the stored range for loads; clamped values reach no output.
-// Results are bitwise those of two fused3d_kernel launches with the batch
-// between them.
+// x neighbours inside a row's wave by DPP whole-wave shifts: xm1 gives lane l
+// the value of lane l-1, xp1 that of lane l+1 (lanes 0 / 63 receive 0, and
+// reach no output).
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ double xm1(double v) {
+    return __hiloint2double(dpp_shr1(__double2hiint(v)), dpp_shr1(__double2loint(v)));
+}
+__device__ __forceinline__ double xp1(double v) {
+    return __hiloint2double(dpp_shl1(__double2hiint(v)), dpp_shl1(__double2loint(v)));
+}
+__device__ __forceinline__ float xm1(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+__device__ __forceinline__ float xp1(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+
+// Two PT iterations in one pass over a 64 x BY tile marching z.  Thread
+// (tx, ty) owns column (i, j) = (i0 - 2 + tx, j0 - 2 + ty); the row index is
+// wave-uniform.  Step s:
+//   A (first iteration): fused3d_kernel's step at plane s (Center stresses
+//     and tau_xy at s, tau_xz / tau_yz at s+1, V at s), nothing stored; its
+//     results and the eta-derived coefficients it formed stay in registers
+//     for one step.
+//   batch: A's V at plane s gets the free-slip batch per point (the batch's
+//     order: z from A's planes s-1 / s, y through an LDS exchange of the
+//     neighbouring rows, x through lane shifts; the reference's fma steps);
+//     blocks and planes no rule reaches skip it.
+//   B (second iteration): the same step at plane r = s-1 from A's results of
+//     step s-1 and the batched V at planes r and r+1, with A's coefficients
+//     (the same points, so B forms no eta_ve or eta_ml term itself); it
+//     stores planes kb .. ke-1.
+// The bottom chunk's B starts at plane 0 one step later, since the batch's
+// Neumann ghost at plane 0 needs A's V at plane 1: at that step B also forms
+// its tau_xz / tau_yz at plane 0 from A's values kept from step -1 and the
+// input V at plane -1, which no rule and no iteration changes.
+// A is exact on lanes 1..62 and rows 1..BY-1; B's outputs are lanes 2..61
+// and rows 2..BY-2 (Center values, V) and 1..BY-3 (edges at j+1).  Rows,
+// columns and planes are clamped into the stored range for loads; clamped
+// values reach no output.  The LDS planes of V and of the stresses are
+// shared by A and B (their uses are separated by barriers).  Results are
+// bitwise those of two fused3d_kernel launches with the batch between them.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
-__global__ void __launch_bounds__(kFX * kTB3Y<T>, (kTB3Y<T> + 3) / 4) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
+__global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
     static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
     constexpr int BY = kTB3Y<T>, NR = BY + 1, TW = kFX;
     constexpr bool HF = OPT == 1;
     using AR = Arith<T, EXACT>;
+    using Loc = typename AR::Loc;
     const Kc<T>& K = a.k;
-    __shared__ T s_vx[NR][TW], s_vy[2][NR][TW], s_vz1[NR][TW];
-    __shared__ T s_eta[FVISC ? 4 : 1][FVISC ? NR : 1][TW], s_eml[FVISC ? 2 : 1][FVISC ? BY : 1][TW];
-    __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW], s_tyz[2][BY][TW];
-    __shared__ T b_vx[NR][TW], b_vy[2][NR][TW], b_vz1[NR][TW];
-    __shared__ T b_P[BY][TW], b_tyy[BY][TW], b_txy[BY][TW], b_tyz[2][BY][TW];
+    __shared__ T s_vx[NR][TW], s_vz1[NR][TW];         // Vx at the stressed plane, Vz at the next (A, then B)
+    __shared__ T s_vy[2][NR][TW], b_vy[2][NR][TW];    // Vy at the plane and the next: A's, B's
+    __shared__ T s_eta[FVISC ? 2 : 1][NR][TW];       // eta at s, s+1 (FVISC; else the batch exchange)
+    __shared__ T s_eml[FVISC ? BY : 1][TW];
+    __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW];   // the plane's stresses (A, then B)
+    __shared__ T s_tyz[2][BY][TW], b_tyz[2][BY][TW];         // tau_yz at the plane and the next
+    __shared__ T q_st[7][BY][TW];   // A's stresses of the previous step, one slot per thread (B's inputs)
 
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int nb = a.ntx * a.nty * a.nchunks;
     const int lt = xcd_remap(blockIdx.x, nb);
     const int bx = lt % a.ntx, by = (lt / a.ntx) % a.nty, bz = lt / (a.ntx * a.nty);
-    const int i = bx * (TW - 4) - 2 + tx;
-    const int j = by * (BY - 3) - 2 + ty;
+    const int i0 = bx * (TW - 4) - 2, j0 = by * (BY - 3) - 2;
+    const int i = i0 + tx;
+    const int j = j0 + ty;
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int nb0 = n0 + 1, nb1 = n1 + 1, nb2 = n2 + 1;
     const int kb = bz * a.kchunk;
@@ -979,196 +1010,206 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>, (kTB3Y<T> + 3) / 4) fused3d_tb
     const bool row8 = last && jc + 1 <= n1 + 2;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     constexpr uint32_t E = sizeof(T);
-    uint32_t b0[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
+    // batch rules reach this block: y rows {0, 1, n1+1}, x columns {0, 1, n0+1}
+    const bool yb = j0 <= 1 || (j0 <= nb1 && nb1 <= j0 + BY - 1);
+    const bool xb = i0 <= 1 || (i0 <= nb0 && nb0 <= i0 + TW - 1);
+
+    // byte offsets at plane kb - 2 (clamped to -1); loads of plane s+1 past
+    // the storage (s = n2+2) take plane s instead
+    const int s0 = max(kb - 2, -1);
+    uint32_t o[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
 #pragma unroll
     for (int c = 0; c < NLOC; ++c) {
         const LocStride sl = a.L[c];
-        b0[c] = (uint32_t)((int64_t)sl.o0 + ic + (int64_t)jc * sl.sy - sl.sz) * E;   // plane -1
+        o[c] = (uint32_t)((int64_t)sl.o0 + ic + (int64_t)jc * sl.sy + (int64_t)s0 * sl.sz) * E;
         dz[c] = sl.sz * E;
         dy[c] = sl.sy * E;
         d8[c] = row8 ? dy[c] : 0u;
     }
-    auto zo = [&](int k, int c) -> uint32_t { return b0[c] + (uint32_t)(min(max(k, -1), n2 + 2) + 1) * dz[c]; };
     auto ld = [](const T* p, uint32_t off) -> T { return ldo(p, off); };
     const int tx1 = tx < TW - 1 ? tx + 1 : tx, txm = tx > 0 ? tx - 1 : tx;
     const int tym = ty > 0 ? ty - 1 : ty, typ = ty + 1;   // typ <= BY: row BY exists in every NR array
 
-    // the batch: after z (own plane ring), y (exchange), x (shuffles)
-    const bool ij_in = i >= 0 && i <= n0 + 2 && j >= 0 && j <= n1 + 2;   // z rule's tangential range
+    // the batch of one plane q from A's raw V at q-1 (rm), q (r0), q+1 (rp);
+    // the y rule's exchange goes through two LDS planes free at that point
     const T hx = T(fs.h[0]), hy = T(fs.h[1]), hz = T(fs.h[2]);
-    auto zrule = [&](int c, int q, T rm, T r0, T rp) -> T {
-        if (!ij_in) return r0;
-        if (c == 2) {   // Vz: Dirichlet on its z nodes
-            if (q == 1) return T(fs.v[2][2][0]);
-            if (q == n2 + 1) return T(fs.v[2][2][1]);
-            return r0;
-        }
-        if (q == 0) return fma(hz, -T(fs.v[c][2][0]), rp);
-        if (q == n2 + 1) return fma(hz, T(fs.v[c][2][1]), rm);
-        return r0;
-    };
-    // V after the batch at plane q from the raw first-iteration values at q-1, q, q+1
-    auto batched = [&](int q, const T* rm, const T* r0, const T* rp, T* out) {
-        T az[3];
+    const bool ij_in = i >= 0 && i <= n0 + 2 && j >= 0 && j <= n1 + 2;   // z rule's tangential range
+    auto batched = [&](int q, const T* rm, const T* r0, const T* rp, T (*xa)[TW], T (*xc)[TW], T* out) {
+        T az[3] = {r0[0], r0[1], r0[2]};
+        if (q == 0 || q == 1 || q == n2 + 1) {   // z rule
+            if (ij_in) {
+                if (q == 1) az[2] = T(fs.v[2][2][0]);
+                if (q == n2 + 1) az[2] = T(fs.v[2][2][1]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) az[c] = zrule(c, q, rm[c], r0[c], rp[c]);
-        // the exchange goes through B's Pr / tau_yy planes, free until B's stresses
-        b_P[ty][tx] = az[0];
-        b_tyy[ty][tx] = az[2];
-        __syncthreads();
-        const int tyx = ty + 1 < BY ? ty + 1 : ty;   // the last row is past every output
-        const bool iq_in = i >= 0 && i <= n0 + 2 && q >= 0 && q <= n2 + 2;   // y rule's range
-        const bool jq_in = j >= 0 && j <= n1 + 2 && q >= 0 && q <= n2 + 2;   // x rule's range
-        T ay[3];
-#pragma unroll
-        for (int c = 0; c < 3; c += 2) {   // Vx, Vz: Neumann in y
-            const T(*xc)[TW] = c == 0 ? b_P : b_tyy;
-            T v = az[c];
-            if (iq_in && j == 0) v = fma(hy, -T(fs.v[c][1][0]), xc[tyx][tx]);
-            else if (iq_in && j == n1 + 1) v = fma(hy, T(fs.v[c][1][1]), xc[tym][tx]);
-            ay[c] = v;
+                for (int c = 0; c < 2; ++c) {
+                    if (q == 0) az[c] = fma(hz, -T(fs.v[c][2][0]), rp[c]);
+                    if (q == n2 + 1) az[c] = fma(hz, T(fs.v[c][2][1]), rm[c]);
+                }
+            }
         }
-        ay[1] = az[1];   // Vy: Dirichlet on its y nodes
-        if (iq_in && j == 1) ay[1] = T(fs.v[1][1][0]);
-        else if (iq_in && j == n1 + 1) ay[1] = T(fs.v[1][1][1]);
-        // x: Vx Dirichlet on its x nodes, Vy and Vz Neumann from the neighbouring lane
+        T ay[3] = {az[0], az[1], az[2]};
+        if (yb) {   // y rule (block-uniform branch around the barrier)
+            xa[ty][tx] = az[0];
+            xc[ty][tx] = az[2];
+            __syncthreads();
+            const int tyx = ty + 1 < BY ? ty + 1 : ty;   // the last row is past every output
+            if (i >= 0 && i <= n0 + 2 && q >= 0 && q <= n2 + 2) {
+                if (j == 0) {
+                    ay[0] = fma(hy, -T(fs.v[0][1][0]), xa[tyx][tx]);
+                    ay[2] = fma(hy, -T(fs.v[2][1][0]), xc[tyx][tx]);
+                } else if (j == n1 + 1) {
+                    ay[0] = fma(hy, T(fs.v[0][1][1]), xa[tym][tx]);
+                    ay[2] = fma(hy, T(fs.v[2][1][1]), xc[tym][tx]);
+                }
+                if (j == 1) ay[1] = T(fs.v[1][1][0]);
+                else if (j == n1 + 1) ay[1] = T(fs.v[1][1][1]);
+            }
+        }
         out[0] = ay[0];
-        if (jq_in && i == 1) out[0] = T(fs.v[0][0][0]);
-        else if (jq_in && i == n0 + 1) out[0] = T(fs.v[0][0][1]);
+        out[1] = ay[1];
+        out[2] = ay[2];
+        if (xb) {   // x rule: Vx Dirichlet on its x nodes, Vy and Vz Neumann from the neighbouring lane
+            const bool jq_in = j >= 0 && j <= n1 + 2 && q >= 0 && q <= n2 + 2;
+            if (jq_in && i == 1) out[0] = T(fs.v[0][0][0]);
+            else if (jq_in && i == n0 + 1) out[0] = T(fs.v[0][0][1]);
 #pragma unroll
-        for (int c = 1; c < 3; ++c) {
-            const T dn = shdn(ay[c]), up = shup(ay[c]);
-            T v = ay[c];
-            if (jq_in && i == 0) v = fma(hx, -T(fs.v[c][0][0]), dn);
-            else if (jq_in && i == n0 + 1) v = fma(hx, T(fs.v[c][0][1]), up);
-            out[c] = v;
+            for (int c = 1; c < 3; ++c) {
+                const T dn = xp1(ay[c]), up = xm1(ay[c]);
+                if (jq_in && i == 0) out[c] = fma(hx, -T(fs.v[c][0][0]), dn);
+                else if (jq_in && i == n0 + 1) out[c] = fma(hx, T(fs.v[c][0][1]), up);
+            }
         }
     };
 
-    // ---- A: carried state at plane kb-2
+    // ---- A: carried state at plane s0
     int pa = 0, pb = 0;
-    const int k0 = kb - 2;
-    T Avx = ld(a.Vx, zo(k0, LVX)), Avy = ld(a.Vy, zo(k0, LVY)), Avz = ld(a.Vz, zo(k0, LVZ));
+    T Avx = ld(a.Vx, o[LVX]), Avy = ld(a.Vy, o[LVY]), Avz = ld(a.Vz, o[LVZ]);
     T Aeta = T(0), Aeml_m = T(0), AP_m = T(0), Atzz_m = T(0), Atxz = T(0), Atyz = T(0);
     s_vy[0][ty][tx] = Avy;
     if constexpr (FVISC) {
-        Aeta = ld(a.eta, zo(k0, LC));
-        s_eta[k0 & 3][ty][tx] = Aeta;
+        Aeta = ld(a.eta, o[LC]);
+        s_eta[0][ty][tx] = Aeta;
     }
     if (last) {
-        s_vy[0][BY][tx] = ld(a.Vy, zo(k0, LVY) + d8[LVY]);
-        if constexpr (FVISC) s_eta[k0 & 3][BY][tx] = ld(a.eta, zo(k0, LC) + d8[LC]);
+        s_vy[0][BY][tx] = ld(a.Vy, o[LVY] + d8[LVY]);
+        if constexpr (FVISC) s_eta[0][BY][tx] = ld(a.eta, o[LC] + d8[LC]);
     }
-    // ring of A's results (index 2: plane a-2, 1: a-1, 0: a)
-    T rP[3], rtxx[3], rtyy[3], rtzz[3], rtxy[3], rtxz[3], rtyz[3], rv[3][3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        rP[q] = rtxx[q] = rtyy[q] = rtzz[q] = rtxy[q] = rtxz[q] = rtyz[q] = T(0);
-        rv[q][0] = rv[q][1] = rv[q][2] = T(0);
-    }
-    // B's carried state
+    // A's results of the previous step (plane s-1), for B (its stresses are in q_st)
+    T qv[3] = {T(0), T(0), T(0)};   // A's raw V at s-1
+    Loc qLc{}, qLe{}, qL4{}, qL5{};
+    T qex = T(0), qey = T(0), qez = T(0);
+    // A's tau_xz, tau_yz and their coefficients at plane 0 (bottom chunk)
+    T ztxz = T(0), ztyz = T(0);
+    Loc zL4{}, zL5{};
+    // B's carried state: batched V at r, Center values at r-1, tau_xz / tau_yz at r
     T Bv[3] = {T(0), T(0), T(0)};
-    T BP_m = T(0), Btzz_m = T(0), Btxz = T(0), Btyz = T(0), Beml_m = T(0);
+    T BP_m = T(0), Btzz_m = T(0), Btxz = T(0), Btyz = T(0);
+    const int sb = kb == 0 ? 1 : kb;   // B's first step (plane sb - 1)
 
-    const int S = ke - kb + 4;
-    for (int t = 0; t < S; ++t) {
-        const int ka = kb - 2 + t;
-        // ================= A: plane ka (fused3d_kernel's step, nothing stored)
+    for (int s = s0; s <= ke; ++s) {
+        const bool top = s + 1 > n2 + 2;   // plane s+1 is past the storage
+        uint32_t z1[NLOC];
+#pragma unroll
+        for (int c = 0; c < NLOC; ++c) z1[c] = top ? o[c] : o[c] + dz[c];
+        // ================= A: plane s
+        T va[3], vx1, vy1, vz1, P, txx, tyy, tzz, txy1, txz1, tyz1;
+        Loc Lc, Le, L4, L5;
+        T ex, ey, ez;
         {
-            const uint32_t oC = zo(ka, LC), oC1 = zo(ka + 1, LC);
-            const T vx_k1 = ld(a.Vx, zo(ka + 1, LVX));
-            const T vy_k1 = ld(a.Vy, zo(ka + 1, LVY));
-            const T vz_k1 = ld(a.Vz, zo(ka + 1, LVZ));
-            T eta_k1 = T(0), eml_k = T(0);
+            vx1 = ld(a.Vx, z1[LVX]);
+            vy1 = ld(a.Vy, z1[LVY]);
+            vz1 = ld(a.Vz, z1[LVZ]);
+            T eta1 = T(0), eml = T(0);
             if constexpr (FVISC) {
-                eta_k1 = ld(a.eta, oC1);
-                eml_k = ld(a.eml, oC);
+                eta1 = ld(a.eta, z1[LC]);
+                eml = ld(a.eml, o[LC]);
             }
-            T P = ld(a.Pr, oC), txx = ld(a.txx, oC), tyy = ld(a.tyy, oC), tzz = ld(a.tzz, oC);
-            T txy1 = ld(a.txy, zo(ka, LXY) + dy[LXY]);
-            T txz1 = ld(a.txz, zo(ka + 1, LXZ)), tyz1 = ld(a.tyz, zo(ka + 1, LYZ) + dy[LYZ]);
-            const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
-            const T oxy = ldf<T, BF>(a.Oxy, zo(ka, LXY) + dy[LXY]);
-            const T oxz = ldf<T, BF>(a.Oxz, zo(ka + 1, LXZ)), oyz = ldf<T, BF>(a.Oyz, zo(ka + 1, LYZ) + dy[LYZ]);
+            P = ld(a.Pr, o[LC]);
+            txx = ld(a.txx, o[LC]);
+            tyy = ld(a.tyy, o[LC]);
+            tzz = ld(a.tzz, o[LC]);
+            txy1 = ld(a.txy, o[LXY] + dy[LXY]);
+            txz1 = ld(a.txz, z1[LXZ]);
+            tyz1 = ld(a.tyz, z1[LYZ] + dy[LYZ]);
+            const T oxx = ldf<T, BF>(a.Oxx, o[LC]), oyy = ldf<T, BF>(a.Oyy, o[LC]), ozz = ldf<T, BF>(a.Ozz, o[LC]);
+            const T oxy = ldf<T, BF>(a.Oxy, o[LXY] + dy[LXY]);
+            const T oxz = ldf<T, BF>(a.Oxz, z1[LXZ]), oyz = ldf<T, BF>(a.Oyz, z1[LYZ] + dy[LYZ]);
             T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
             if constexpr (HF) {
-                fP = ldf<T, BF>(a.fP, oC);
-                fVx = ldf<T, BF>(a.fVx, zo(ka, LVX));
-                fVy = ldf<T, BF>(a.fVy, zo(ka, LVY));
-                fVz = ldf<T, BF>(a.fVz, zo(ka, LVZ));
+                fP = ldf<T, BF>(a.fP, o[LC]);
+                fVx = ldf<T, BF>(a.fVx, o[LVX]);
+                fVy = ldf<T, BF>(a.fVy, o[LVY]);
+                fVz = ldf<T, BF>(a.fVz, o[LVZ]);
             }
-            T vx8 = T(0), vy8_1 = T(0), vz8_1 = T(0), eta8_1 = T(0);
+            T vx8 = T(0), vy8 = T(0), vz8 = T(0), eta8 = T(0);
             if (last) {
-                vx8 = ld(a.Vx, zo(ka, LVX) + d8[LVX]);
-                vy8_1 = ld(a.Vy, zo(ka + 1, LVY) + d8[LVY]);
-                vz8_1 = ld(a.Vz, zo(ka + 1, LVZ) + d8[LVZ]);
-                if constexpr (FVISC) eta8_1 = ld(a.eta, oC1 + d8[LC]);
+                vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
+                vy8 = ld(a.Vy, z1[LVY] + d8[LVY]);
+                vz8 = ld(a.Vz, z1[LVZ] + d8[LVZ]);
+                if constexpr (FVISC) eta8 = ld(a.eta, z1[LC] + d8[LC]);
             }
-            const bool in_k = ka >= 0 && ka <= nb2;
-            const bool in_k1 = ka + 1 >= 0 && ka + 1 <= nb2;
-            const int e0 = ka & 3, e1 = (ka + 1) & 3;
+            const bool in_s = s >= 0 && s <= nb2;
+            const bool in_s1 = s + 1 >= 0 && s + 1 <= nb2;
             s_vx[ty][tx] = Avx;
-            s_vy[pa ^ 1][ty][tx] = vy_k1;
-            s_vz1[ty][tx] = vz_k1;
+            s_vy[pa ^ 1][ty][tx] = vy1;
+            s_vz1[ty][tx] = vz1;
             if constexpr (FVISC) {
-                s_eta[e1][ty][tx] = eta_k1;
-                s_eml[pa][ty][tx] = eml_k;
+                s_eta[pa ^ 1][ty][tx] = eta1;
+                s_eml[ty][tx] = eml;
             }
             if (last) {
                 s_vx[BY][tx] = vx8;
-                s_vy[pa ^ 1][BY][tx] = vy8_1;
-                s_vz1[BY][tx] = vz8_1;
-                if constexpr (FVISC) s_eta[e1][BY][tx] = eta8_1;
+                s_vy[pa ^ 1][BY][tx] = vy8;
+                s_vz1[BY][tx] = vz8;
+                if constexpr (FVISC) s_eta[pa ^ 1][BY][tx] = eta8;
             }
             __syncthreads();
-            const T exx = (s_vx[ty][tx1] - Avx) * ix;
+            const T exx = (xp1(Avx) - Avx) * ix;
             const T eyy = (s_vy[pa][typ][tx] - Avy) * iy;
-            const T ezz = (vz_k1 - Avz) * iz;
+            const T ezz = (vz1 - Avz) * iz;
             const T dv = (exx + eyy) + ezz;
+            if constexpr (FVISC) Lc = AR::field(Aeta, K);
+            else Lc = AR::scalar(K);
             {
-                typename AR::Loc Lc;
-                if constexpr (FVISC) Lc = AR::field(Aeta, K);
-                else Lc = AR::scalar(K);
                 const T Pn = AR::pr(P, HF ? dv - fP : dv, Lc, K);
                 const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
                 const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
                 const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
-                const bool u = inc && in_k;
+                const bool u = inc && in_s;
                 P = u ? Pn : P;
                 txx = u ? txxn : txx;
                 tyy = u ? tyyn : tyy;
                 tzz = u ? tzzn : tzz;
             }
-            {
+            {   // tau_xy at (i, j+1, s)
                 const T exy = T(0.5) * ((s_vx[typ][tx] - Avx) * iy + (s_vy[pa][typ][tx] - s_vy[pa][typ][txm]) * ix);
-                typename AR::Loc Le;
                 if constexpr (FVISC)
-                    Le = AR::field(lerp4(s_eta[e0][ty][txm], Aeta, s_eta[e0][typ][txm], s_eta[e0][typ][tx]), K);
+                    Le = AR::field(lerp4(s_eta[pa][ty][txm], Aeta, s_eta[pa][typ][txm], s_eta[pa][typ][tx]), K);
                 else Le = AR::scalar(K);
-                const T n = AR::off(txy1, oxy, exy, Le, K);
-                txy1 = ine && in_k ? n : txy1;
+                const T nxy = AR::off(txy1, oxy, exy, Le, K);
+                txy1 = ine && in_s ? nxy : txy1;
             }
-            {
-                const T exz = T(0.5) * ((vx_k1 - Avx) * iz + (vz_k1 - s_vz1[ty][txm]) * ix);
-                const T eyz = T(0.5) * ((s_vy[pa ^ 1][typ][tx] - s_vy[pa][typ][tx]) * iz + (s_vz1[typ][tx] - vz_k1) * iy);
-                typename AR::Loc L4, L5;
+            {   // tau_xz at (i, j, s+1) and tau_yz at (i, j+1, s+1)
+                const T exz = T(0.5) * ((vx1 - Avx) * iz + (vz1 - xm1(vz1)) * ix);
+                const T eyz = T(0.5) * ((s_vy[pa ^ 1][typ][tx] - s_vy[pa][typ][tx]) * iz + (s_vz1[typ][tx] - vz1) * iy);
                 if constexpr (FVISC) {
-                    L4 = AR::field(lerp4(s_eta[e0][ty][txm], Aeta, s_eta[e1][ty][txm], eta_k1), K);
-                    L5 = AR::field(lerp4(Aeta, s_eta[e0][typ][tx], eta_k1, s_eta[e1][typ][tx]), K);
+                    L4 = AR::field(lerp4(s_eta[pa][ty][txm], Aeta, s_eta[pa ^ 1][ty][txm], eta1), K);
+                    L5 = AR::field(lerp4(Aeta, s_eta[pa][typ][tx], eta1, s_eta[pa ^ 1][typ][tx]), K);
                 } else {
                     L4 = L5 = AR::scalar(K);
                 }
                 const T nxz = AR::off(txz1, oxz, exz, L4, K);
                 const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
-                txz1 = inc && in_k1 ? nxz : txz1;
-                tyz1 = ine && in_k1 ? nyz : tyz1;
+                txz1 = inc && in_s1 ? nxz : txz1;
+                tyz1 = ine && in_s1 ? nyz : tyz1;
             }
-            const T txxl = shup(txx), txzr = shdn(Atxz);   // x neighbours within the row's wave
             s_P[ty][tx] = P;
             s_tyy[ty][tx] = tyy;
             s_txy[ty][tx] = txy1;
             s_tyz[pa][ty][tx] = tyz1;
+            const T txxl = xm1(txx), txzr = xp1(Atxz);
             __syncthreads();
+            // A's V at plane s
             const T txy_j = s_txy[tym][tx];
             T rx = ((-((P - s_P[ty][txm]) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy) + (txz1 - Atxz) * iz;
             T ry = ((-((P - s_P[tym][tx]) * iy) + (tyy - s_tyy[tym][tx]) * iy) + (s_txy[tym][tx1] - txy_j) * ix) +
@@ -1180,186 +1221,188 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>, (kTB3Y<T> + 3) / 4) fused3d_tb
                 ry = ry + fVy;
                 rz = rz + fVz;
             }
-            T ex, ey, ez;
             if constexpr (FVISC) {
-                ex = AR::vcoef_field(jmax(s_eml[pa][ty][txm], eml_k), K);
-                ey = AR::vcoef_field(jmax(s_eml[pa][tym][tx], eml_k), K);
-                ez = AR::vcoef_field(jmax(Aeml_m, eml_k), K);
+                ex = AR::vcoef_field(jmax(xm1(eml), eml), K);
+                ey = AR::vcoef_field(jmax(s_eml[tym][tx], eml), K);
+                ez = AR::vcoef_field(jmax(Aeml_m, eml), K);
             } else {
                 ex = ey = ez = AR::vcoef_scalar(K);
             }
-            const bool keep = inc && in_k && t > 0;   // the first plane has no velocity
-            // shift the ring, then plane ka's results go in slot 0
-#pragma unroll
-            for (int q = 2; q > 0; --q) {
-                rP[q] = rP[q - 1];
-                rtxx[q] = rtxx[q - 1];
-                rtyy[q] = rtyy[q - 1];
-                rtzz[q] = rtzz[q - 1];
-                rtxy[q] = rtxy[q - 1];
-                rtxz[q] = rtxz[q - 1];
-                rtyz[q] = rtyz[q - 1];
-                rv[q][0] = rv[q - 1][0];
-                rv[q][1] = rv[q - 1][1];
-                rv[q][2] = rv[q - 1][2];
-            }
-            rP[0] = P;
-            rtxx[0] = txx;
-            rtyy[0] = tyy;
-            rtzz[0] = tzz;
-            rtxy[0] = txy1;
-            rtxz[0] = txz1;
-            rtyz[0] = tyz1;
-            rv[0][0] = keep ? AR::vel(Avx, rx, ex, K) : Avx;
-            rv[0][1] = keep ? AR::vel(Avy, ry, ey, K) : Avy;
-            rv[0][2] = keep ? AR::vel(Avz, rz, ez, K) : Avz;
+            const bool keep = inc && in_s && s > s0;   // the first plane has no velocity
+            const T vxa = keep ? AR::vel(Avx, rx, ex, K) : Avx;
+            const T vya = keep ? AR::vel(Avy, ry, ey, K) : Avy;
+            const T vza = keep ? AR::vel(Avz, rz, ez, K) : Avz;
+            // A's carried state for step s+1
             AP_m = P;
             Atzz_m = tzz;
             Atxz = txz1;
             Atyz = tyz1;
-            Avx = vx_k1;
-            Avy = vy_k1;
-            Avz = vz_k1;
-            Aeta = eta_k1;
-            Aeml_m = eml_k;
-            pa ^= 1;
-        }
-        if (t < 3) continue;
-        // ================= B: plane r = ka - 2
-        const int r = ka - 2;
-        T V1[3];
-        if (t == 3) {   // B's first plane: V after the batch at plane r as well
-            batched(r, rv[2], rv[2], rv[1], Bv);   // plane r-1 is never a source here (r < n2+1)
-            __syncthreads();
-        }
-        batched(r + 1, rv[2], rv[1], rv[0], V1);
-        const int f0 = r & 3, f1 = (r + 1) & 3;
-        b_vx[ty][tx] = Bv[0];
-        if (t == 3) b_vy[pb][ty][tx] = Bv[1];
-        b_vy[pb ^ 1][ty][tx] = V1[1];
-        b_vz1[ty][tx] = V1[2];
-        __syncthreads();
-        {
-            const uint32_t oC = zo(r, LC);
-            const T eta_k = FVISC ? s_eta[f0][ty][tx] : T(0), eta_k1 = FVISC ? s_eta[f1][ty][tx] : T(0);
-            T eml_k = T(0), eml_d = T(0);   // eta_ml at plane r: own, row j-1
+            Avx = vx1;
+            Avy = vy1;
+            Avz = vz1;
             if constexpr (FVISC) {
-                eml_k = ld(a.eml, oC);
-                eml_d = ld(a.eml, jc > -1 ? oC - dy[LC] : oC);
+                Aeta = eta1;
+                Aeml_m = eml;
             }
-            T P = rP[2], txx = rtxx[2], tyy = rtyy[2], tzz = rtzz[2];
-            T txy1 = rtxy[2], txz1 = rtxz[2], tyz1 = rtyz[2];
+            va[0] = vxa;
+            va[1] = vya;
+            va[2] = vza;
+            if (kb == 0 && s == -1) {   // plane 0's tau_xz / tau_yz for the bottom chunk's B
+                ztxz = txz1;
+                ztyz = tyz1;
+                zL4 = L4;
+                zL5 = L5;
+            }
+        }
+        // ================= B: plane r = s - 1
+        const int r = s - 1;
+        if (s >= sb) {
+            T V1[3];
+            if (s == sb) {   // B's first step: V after the batch at plane r as well
+                if (kb == 0) {
+                    // plane 0 from A's V at planes 0 (previous step) and 1 (this step)
+                    batched(0, qv, qv, va, s_eta[FVISC ? pa : 0], s_vy[pa], Bv);
+                    __syncthreads();
+                    b_vy[pb][ty][tx] = Bv[1];
+                    s_vz1[ty][tx] = Bv[2];   // Vz(0) for its row j+1 neighbour
+                    __syncthreads();
+                    // B's tau_xz at (i, j, 0) and tau_yz at (i, j+1, 0); V at plane -1 is the
+                    // input, since no iteration and no rule changes that plane
+                    const T vxm = ld(a.Vx, o[LVX] - 2 * dz[LVX]);
+                    const T vym = ld(a.Vy, o[LVY] - 2 * dz[LVY] + dy[LVY]);
+                    const T oxz = ldf<T, BF>(a.Oxz, o[LXZ] - dz[LXZ]);
+                    const T oyz = ldf<T, BF>(a.Oyz, o[LYZ] - dz[LYZ] + dy[LYZ]);
+                    BP_m = ld(a.Pr, o[LC] - 2 * dz[LC]);
+                    Btzz_m = ld(a.tzz, o[LC] - 2 * dz[LC]);
+                    const T exz = T(0.5) * ((Bv[0] - vxm) * iz + (Bv[2] - xm1(Bv[2])) * ix);
+                    const T eyz = T(0.5) * ((b_vy[pb][typ][tx] - vym) * iz + (s_vz1[typ][tx] - Bv[2]) * iy);
+                    Btxz = inc ? AR::off(ztxz, oxz, exz, zL4, K) : ztxz;
+                    Btyz = ine ? AR::off(ztyz, oyz, eyz, zL5, K) : ztyz;
+                    __syncthreads();
+                    b_tyz[pb ^ 1][ty][tx] = Btyz;
+                } else {
+                    batched(r, qv, qv, qv, s_eta[FVISC ? pa : 0], s_vy[pa], Bv);
+                    b_vy[pb][ty][tx] = Bv[1];
+                    __syncthreads();   // the exchange planes are used again below
+                }
+            }
+            batched(r + 1, qv, va, va, s_eta[FVISC ? pa : 0], s_vy[pa], V1);
+            s_vx[ty][tx] = Bv[0];
+            b_vy[pb ^ 1][ty][tx] = V1[1];
+            s_vz1[ty][tx] = V1[2];
+            // tau_old (and forcing) of plane r: o[] sits at plane s
+            const uint32_t oC = o[LC] - dz[LC];
             const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
-            const T oxy = ldf<T, BF>(a.Oxy, zo(r, LXY) + dy[LXY]);
-            const T oxz = ldf<T, BF>(a.Oxz, zo(r + 1, LXZ)), oyz = ldf<T, BF>(a.Oyz, zo(r + 1, LYZ) + dy[LYZ]);
+            const T oxy = ldf<T, BF>(a.Oxy, o[LXY] - dz[LXY] + dy[LXY]);
+            const T oxz = ldf<T, BF>(a.Oxz, o[LXZ]), oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
             T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
             if constexpr (HF) {
                 fP = ldf<T, BF>(a.fP, oC);
-                fVx = ldf<T, BF>(a.fVx, zo(r, LVX));
-                fVy = ldf<T, BF>(a.fVy, zo(r, LVY));
-                fVz = ldf<T, BF>(a.fVz, zo(r, LVZ));
+                fVx = ldf<T, BF>(a.fVx, o[LVX] - dz[LVX]);
+                fVy = ldf<T, BF>(a.fVy, o[LVY] - dz[LVY]);
+                fVz = ldf<T, BF>(a.fVz, o[LVZ] - dz[LVZ]);
             }
+            __syncthreads();
             const bool in_k = r >= 0 && r <= nb2;
             const bool in_k1 = r + 1 >= 0 && r + 1 <= nb2;
-            const T vx_k = Bv[0], vy_k = Bv[1], vz_k = Bv[2];
-            const T vx_k1 = V1[0], vz_k1 = V1[2];
-            const T exx = (b_vx[ty][tx1] - vx_k) * ix;
-            const T eyy = (b_vy[pb][typ][tx] - vy_k) * iy;
-            const T ezz = (vz_k1 - vz_k) * iz;
+            T bP = q_st[0][ty][tx], btxx = q_st[1][ty][tx], btyy = q_st[2][ty][tx], btzz = q_st[3][ty][tx];
+            T btxy = q_st[4][ty][tx], btxz = q_st[5][ty][tx], btyz = q_st[6][ty][tx];
+            const T exx = (xp1(Bv[0]) - Bv[0]) * ix;
+            const T eyy = (b_vy[pb][typ][tx] - Bv[1]) * iy;
+            const T ezz = (V1[2] - Bv[2]) * iz;
             const T dv = (exx + eyy) + ezz;
             {
-                typename AR::Loc Lc;
-                if constexpr (FVISC) Lc = AR::field(eta_k, K);
-                else Lc = AR::scalar(K);
-                const T Pn = AR::pr(P, HF ? dv - fP : dv, Lc, K);
-                const T txxn = AR::diag(txx, oxx, exx, dv, Lc, K);
-                const T tyyn = AR::diag(tyy, oyy, eyy, dv, Lc, K);
-                const T tzzn = AR::diag(tzz, ozz, ezz, dv, Lc, K);
+                const T Pn = AR::pr(bP, HF ? dv - fP : dv, qLc, K);
+                const T txxn = AR::diag(btxx, oxx, exx, dv, qLc, K);
+                const T tyyn = AR::diag(btyy, oyy, eyy, dv, qLc, K);
+                const T tzzn = AR::diag(btzz, ozz, ezz, dv, qLc, K);
                 const bool u = inc && in_k;
-                P = u ? Pn : P;
-                txx = u ? txxn : txx;
-                tyy = u ? tyyn : tyy;
-                tzz = u ? tzzn : tzz;
+                bP = u ? Pn : bP;
+                btxx = u ? txxn : btxx;
+                btyy = u ? tyyn : btyy;
+                btzz = u ? tzzn : btzz;
             }
             {
-                const T exy = T(0.5) * ((b_vx[typ][tx] - vx_k) * iy + (b_vy[pb][typ][tx] - b_vy[pb][typ][txm]) * ix);
-                typename AR::Loc Le;
-                if constexpr (FVISC)
-                    Le = AR::field(lerp4(s_eta[f0][ty][txm], eta_k, s_eta[f0][typ][txm], s_eta[f0][typ][tx]), K);
-                else Le = AR::scalar(K);
-                const T n = AR::off(txy1, oxy, exy, Le, K);
-                txy1 = ine && in_k ? n : txy1;
+                const T exy = T(0.5) * ((s_vx[typ][tx] - Bv[0]) * iy + (b_vy[pb][typ][tx] - b_vy[pb][typ][txm]) * ix);
+                const T nxy = AR::off(btxy, oxy, exy, qLe, K);
+                btxy = ine && in_k ? nxy : btxy;
             }
             {
-                const T exz = T(0.5) * ((vx_k1 - vx_k) * iz + (vz_k1 - b_vz1[ty][txm]) * ix);
-                const T eyz = T(0.5) * ((b_vy[pb ^ 1][typ][tx] - b_vy[pb][typ][tx]) * iz + (b_vz1[typ][tx] - vz_k1) * iy);
-                typename AR::Loc L4, L5;
-                if constexpr (FVISC) {
-                    L4 = AR::field(lerp4(s_eta[f0][ty][txm], eta_k, s_eta[f1][ty][txm], eta_k1), K);
-                    L5 = AR::field(lerp4(eta_k, s_eta[f0][typ][tx], eta_k1, s_eta[f1][typ][tx]), K);
-                } else {
-                    L4 = L5 = AR::scalar(K);
-                }
-                const T nxz = AR::off(txz1, oxz, exz, L4, K);
-                const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
-                txz1 = inc && in_k1 ? nxz : txz1;
-                tyz1 = ine && in_k1 ? nyz : tyz1;
+                const T exz = T(0.5) * ((V1[0] - Bv[0]) * iz + (V1[2] - xm1(V1[2])) * ix);
+                const T eyz = T(0.5) * ((b_vy[pb ^ 1][typ][tx] - b_vy[pb][typ][tx]) * iz + (s_vz1[typ][tx] - V1[2]) * iy);
+                const T nxz = AR::off(btxz, oxz, exz, qL4, K);
+                const T nyz = AR::off(btyz, oyz, eyz, qL5, K);
+                btxz = inc && in_k1 ? nxz : btxz;
+                btyz = ine && in_k1 ? nyz : btyz;
             }
             const bool st = r >= kb && r < ke;
             if (st) {
                 if (ownc) {
-                    sto(a.oPr, oC, P);
-                    sto(a.otxx, oC, txx);
-                    sto(a.otyy, oC, tyy);
-                    sto(a.otzz, oC, tzz);
-                    sto(a.otxz, zo(r, LXZ), Btxz);
+                    sto(a.oPr, oC, bP);
+                    sto(a.otxx, oC, btxx);
+                    sto(a.otyy, oC, btyy);
+                    sto(a.otzz, oC, btzz);
+                    sto(a.otxz, o[LXZ] - dz[LXZ], Btxz);
                 }
                 if (owne) {
-                    sto(a.otxy, zo(r, LXY) + dy[LXY], txy1);
-                    sto(a.otyz, zo(r, LYZ) + dy[LYZ], Btyz);
+                    sto(a.otxy, o[LXY] - dz[LXY] + dy[LXY], btxy);
+                    sto(a.otyz, o[LYZ] - dz[LYZ] + dy[LYZ], Btyz);
                 }
             }
-            const T txxl = shup(txx), txzr = shdn(Btxz);
-            b_P[ty][tx] = P;
-            b_tyy[ty][tx] = tyy;
-            b_txy[ty][tx] = txy1;
-            b_tyz[pb][ty][tx] = tyz1;
+            const T txxl = xm1(btxx), txzr = xp1(Btxz);
+            s_P[ty][tx] = bP;
+            s_tyy[ty][tx] = btyy;
+            s_txy[ty][tx] = btxy;
+            b_tyz[pb][ty][tx] = btyz;
             __syncthreads();
-            const T txy_j = b_txy[tym][tx];
-            T rx = ((-((P - b_P[ty][txm]) * ix) + (txx - txxl) * ix) + (txy1 - txy_j) * iy) + (txz1 - Btxz) * iz;
-            T ry = ((-((P - b_P[tym][tx]) * iy) + (tyy - b_tyy[tym][tx]) * iy) + (b_txy[tym][tx1] - txy_j) * ix) +
+            const T txy_j = s_txy[tym][tx];
+            T rx = ((-((bP - s_P[ty][txm]) * ix) + (btxx - txxl) * ix) + (btxy - txy_j) * iy) + (btxz - Btxz) * iz;
+            T ry = ((-((bP - s_P[tym][tx]) * iy) + (btyy - s_tyy[tym][tx]) * iy) + (s_txy[tym][tx1] - txy_j) * ix) +
                    (b_tyz[pb][tym][tx] - b_tyz[pb ^ 1][tym][tx]) * iz;
-            T rz = ((-((P - BP_m) * iz) + (tzz - Btzz_m) * iz) + (txzr - Btxz) * ix) +
+            T rz = ((-((bP - BP_m) * iz) + (btzz - Btzz_m) * iz) + (txzr - Btxz) * ix) +
                    (Btyz - b_tyz[pb ^ 1][tym][tx]) * iy;
             if constexpr (HF) {
                 rx = rx + fVx;
                 ry = ry + fVy;
                 rz = rz + fVz;
             }
-            const T emll = shup(eml_k);   // every lane takes part in the shuffle
             if (st && ownc) {
-                T ex, ey, ez;
-                if constexpr (FVISC) {
-                    ex = AR::vcoef_field(jmax(emll, eml_k), K);
-                    ey = AR::vcoef_field(jmax(eml_d, eml_k), K);
-                    ez = AR::vcoef_field(jmax(Beml_m, eml_k), K);
-                } else {
-                    ex = ey = ez = AR::vcoef_scalar(K);
-                }
-                sto(a.oVx, zo(r, LVX), AR::vel(vx_k, rx, ex, K));
-                sto(a.oVy, zo(r, LVY), AR::vel(vy_k, ry, ey, K));
-                sto(a.oVz, zo(r, LVZ), AR::vel(vz_k, rz, ez, K));
+                sto(a.oVx, o[LVX] - dz[LVX], AR::vel(Bv[0], rx, qex, K));
+                sto(a.oVy, o[LVY] - dz[LVY], AR::vel(Bv[1], ry, qey, K));
+                sto(a.oVz, o[LVZ] - dz[LVZ], AR::vel(Bv[2], rz, qez, K));
             }
-            BP_m = P;
-            Btzz_m = tzz;
-            Btxz = txz1;
-            Btyz = tyz1;
+            BP_m = bP;
+            Btzz_m = btzz;
+            Btxz = btxz;
+            Btyz = btyz;
             Bv[0] = V1[0];
             Bv[1] = V1[1];
             Bv[2] = V1[2];
-            Beml_m = eml_k;
             pb ^= 1;
+        } else {
+            __syncthreads();   // A's next step writes the eta_ml plane its velocity just read
         }
+        // A's results of plane s for B's next step (B of this step has read the slots)
+        q_st[0][ty][tx] = P;
+        q_st[1][ty][tx] = txx;
+        q_st[2][ty][tx] = tyy;
+        q_st[3][ty][tx] = tzz;
+        q_st[4][ty][tx] = txy1;
+        q_st[5][ty][tx] = txz1;
+        q_st[6][ty][tx] = tyz1;
+        qv[0] = va[0];
+        qv[1] = va[1];
+        qv[2] = va[2];
+        qLc = Lc;
+        qLe = Le;
+        qL4 = L4;
+        qL5 = L5;
+        qex = ex;
+        qey = ey;
+        qez = ez;
+        pa ^= 1;
+#pragma unroll
+        for (int c = 0; c < NLOC; ++c) o[c] += dz[c];
     }
 }
 
