@@ -54,14 +54,15 @@ def reference_bytes(n, itemsize=8):
     return 37 * itemsize * n ** 3
 
 
-def setup(n, dtype):
+def setup(n, dtype, exact=False):
     import torch
     import chmy
     from chmy import configs
     from chmy.stokes import PTStokes, pure_shear_initial, pure_shear_values
     arch = chmy.Arch(chmy.MI355XBackend())
     g, eta, dt = configs.stokes3d_inclusion(arch, n)   # dt_diff (stokes_3d_inc_ve_T.jl:157), λ_ρCp = 1e-4
-    S = PTStokes(arch, g, dtype=dtype, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(3))
+    S = PTStokes(arch, g, dtype=dtype, eta_field=eta, G=1.0, dt=dt, normal_values=pure_shear_values(3),
+                 exact=exact)
     pure_shear_initial(g, S.V)
     S.apply_bc()
     S.sync_buffers()
@@ -514,7 +515,7 @@ def pairs_rate(S, args, isz):
     A = algorithmic_bytes(n, isz)
     print(json.dumps({"metric": "T_eff GB/s, two PT iterations per pass", "value": round(2 * A / (ms_pass * 1e-3) / 1e9, 2),
                       "unit": "GB/s", "passes": K, "ms_per_pass": round(ms_pass, 4),
-                      "ms_per_iteration": round(ms_pass / 2, 4), "dtype": args.dtype,
+                      "ms_per_iteration": round(ms_pass / 2, 4), "dtype": args.dtype, "exact": bool(args.exact),
                       "config": {"workload": f"3D PT viscoelastic Stokes, {n}^3 cells, fused3d_tb_kernel + V batch",
                                  "algorithmic_bytes_per_iteration": A}}))
 
@@ -528,6 +529,8 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--exact", action="store_true",
+                    help="the reference's expression order (bitwise equal to update_stress! + update_velocity!)")
     ap.add_argument("--mode", default="fused", choices=["fused", "unfused", "pairs"],
                     help="fused: one-pass double-buffered iteration; unfused: the reference's "
                          "update_stress! + update_velocity! kernels; pairs: two iterations per pass "
@@ -587,7 +590,7 @@ def main():
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     isz = 8 if args.dtype == "f64" else 4
     n = args.n
-    S = setup(n, dtype)
+    S = setup(n, dtype, exact=args.exact)
     print(f"# setup done: {n}^3 {args.dtype}", file=sys.stderr, flush=True)
     if args.mode == "pairs":
         pairs_rate(S, args, isz)

@@ -924,8 +924,9 @@ struct FreeSlip3 {
 #ifndef CHMY_FUSED_TB3_BY32
 #define CHMY_FUSED_TB3_BY32 12
 #endif
+// the shortest z chunk of the two-iteration pass (the host picks the length)
 #ifndef CHMY_FUSED_TB3_ZCHUNK
-#define CHMY_FUSED_TB3_ZCHUNK 16
+#define CHMY_FUSED_TB3_ZCHUNK 2
 #endif
 template <class T>
 constexpr int kTB3Y = std::is_same<T, double>::value ? CHMY_FUSED_TB3_BY64 : CHMY_FUSED_TB3_BY32;
@@ -1098,9 +1099,6 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
     T qv[3] = {T(0), T(0), T(0)};   // A's raw V at s-1
     Loc qLc{}, qLe{}, qL4{}, qL5{};
     T qex = T(0), qey = T(0), qez = T(0);
-    // A's tau_xz, tau_yz and their coefficients at plane 0 (bottom chunk)
-    T ztxz = T(0), ztyz = T(0);
-    Loc zL4{}, zL5{};
     // B's carried state: batched V at r, Center values at r-1, tau_xz / tau_yz at r
     T Bv[3] = {T(0), T(0), T(0)};
     T BP_m = T(0), Btzz_m = T(0), Btxz = T(0), Btyz = T(0);
@@ -1116,6 +1114,15 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
         Loc Lc, Le, L4, L5;
         T ex, ey, ez;
         {
+            // the staging wave's row j+1 first: loads complete in order, and its LDS
+            // writes (before the first barrier) then need not wait for the rest
+            T vx8 = T(0), vy8 = T(0), vz8 = T(0), eta8 = T(0);
+            if (last) {
+                vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
+                vy8 = ld(a.Vy, z1[LVY] + d8[LVY]);
+                vz8 = ld(a.Vz, z1[LVZ] + d8[LVZ]);
+                if constexpr (FVISC) eta8 = ld(a.eta, z1[LC] + d8[LC]);
+            }
             vx1 = ld(a.Vx, z1[LVX]);
             vy1 = ld(a.Vy, z1[LVY]);
             vz1 = ld(a.Vz, z1[LVZ]);
@@ -1140,13 +1147,6 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 fVx = ldf<T, BF>(a.fVx, o[LVX]);
                 fVy = ldf<T, BF>(a.fVy, o[LVY]);
                 fVz = ldf<T, BF>(a.fVz, o[LVZ]);
-            }
-            T vx8 = T(0), vy8 = T(0), vz8 = T(0), eta8 = T(0);
-            if (last) {
-                vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
-                vy8 = ld(a.Vy, z1[LVY] + d8[LVY]);
-                vz8 = ld(a.Vz, z1[LVZ] + d8[LVZ]);
-                if constexpr (FVISC) eta8 = ld(a.eta, z1[LC] + d8[LC]);
             }
             const bool in_s = s >= 0 && s <= nb2;
             const bool in_s1 = s + 1 >= 0 && s + 1 <= nb2;
@@ -1192,9 +1192,12 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             {   // tau_xz at (i, j, s+1) and tau_yz at (i, j+1, s+1)
                 const T exz = T(0.5) * ((vx1 - Avx) * iz + (vz1 - xm1(vz1)) * ix);
                 const T eyz = T(0.5) * ((s_vy[pa ^ 1][typ][tx] - s_vy[pa][typ][tx]) * iz + (s_vz1[typ][tx] - vz1) * iy);
+                T e4 = T(0), e5 = T(0);
                 if constexpr (FVISC) {
-                    L4 = AR::field(lerp4(s_eta[pa][ty][txm], Aeta, s_eta[pa ^ 1][ty][txm], eta1), K);
-                    L5 = AR::field(lerp4(Aeta, s_eta[pa][typ][tx], eta1, s_eta[pa ^ 1][typ][tx]), K);
+                    e4 = lerp4(s_eta[pa][ty][txm], Aeta, s_eta[pa ^ 1][ty][txm], eta1);
+                    e5 = lerp4(Aeta, s_eta[pa][typ][tx], eta1, s_eta[pa ^ 1][typ][tx]);
+                    L4 = AR::field(e4, K);
+                    L5 = AR::field(e5, K);
                 } else {
                     L4 = L5 = AR::scalar(K);
                 }
@@ -1202,6 +1205,14 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 const T nyz = AR::off(tyz1, oyz, eyz, L5, K);
                 txz1 = inc && in_s1 ? nxz : txz1;
                 tyz1 = ine && in_s1 ? nyz : tyz1;
+                if (kb == 0 && s == -1) {
+                    // plane 0's tau_xz / tau_yz and eta at their points, for the bottom
+                    // chunk's B at step 1, in planes B leaves untouched until then
+                    b_tyz[0][ty][tx] = txz1;
+                    b_tyz[1][ty][tx] = tyz1;
+                    b_vy[0][ty][tx] = e4;
+                    b_vy[1][ty][tx] = e5;
+                }
             }
             s_P[ty][tx] = P;
             s_tyy[ty][tx] = tyy;
@@ -1247,19 +1258,31 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             va[0] = vxa;
             va[1] = vya;
             va[2] = vza;
-            if (kb == 0 && s == -1) {   // plane 0's tau_xz / tau_yz for the bottom chunk's B
-                ztxz = txz1;
-                ztyz = tyz1;
-                zL4 = L4;
-                zL5 = L5;
-            }
         }
         // ================= B: plane r = s - 1
         const int r = s - 1;
         if (s >= sb) {
+            // B's inputs are A's stresses of the previous step; this step's take their slots
+            T bP = q_st[0][ty][tx], btxx = q_st[1][ty][tx], btyy = q_st[2][ty][tx], btzz = q_st[3][ty][tx];
+            T btxy = q_st[4][ty][tx], btxz = q_st[5][ty][tx], btyz = q_st[6][ty][tx];
+            q_st[0][ty][tx] = P;
+            q_st[1][ty][tx] = txx;
+            q_st[2][ty][tx] = tyy;
+            q_st[3][ty][tx] = tzz;
+            q_st[4][ty][tx] = txy1;
+            q_st[5][ty][tx] = txz1;
+            q_st[6][ty][tx] = tyz1;
             T V1[3];
             if (s == sb) {   // B's first step: V after the batch at plane r as well
                 if (kb == 0) {
+                    const T ztxz = b_tyz[0][ty][tx], ztyz = b_tyz[1][ty][tx];   // A's, plane 0
+                    Loc zL4, zL5;
+                    if constexpr (FVISC) {
+                        zL4 = AR::field(b_vy[0][ty][tx], K);
+                        zL5 = AR::field(b_vy[1][ty][tx], K);
+                    } else {
+                        zL4 = zL5 = AR::scalar(K);
+                    }
                     // plane 0 from A's V at planes 0 (previous step) and 1 (this step)
                     batched(0, qv, qv, va, s_eta[FVISC ? pa : 0], s_vy[pa], Bv);
                     __syncthreads();
@@ -1305,8 +1328,6 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             __syncthreads();
             const bool in_k = r >= 0 && r <= nb2;
             const bool in_k1 = r + 1 >= 0 && r + 1 <= nb2;
-            T bP = q_st[0][ty][tx], btxx = q_st[1][ty][tx], btyy = q_st[2][ty][tx], btzz = q_st[3][ty][tx];
-            T btxy = q_st[4][ty][tx], btxz = q_st[5][ty][tx], btyz = q_st[6][ty][tx];
             const T exx = (xp1(Bv[0]) - Bv[0]) * ix;
             const T eyy = (b_vy[pb][typ][tx] - Bv[1]) * iy;
             const T ezz = (V1[2] - Bv[2]) * iz;
@@ -1381,15 +1402,14 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             pb ^= 1;
         } else {
             __syncthreads();   // A's next step writes the eta_ml plane its velocity just read
+            q_st[0][ty][tx] = P;   // A's results of plane s for B's next step
+            q_st[1][ty][tx] = txx;
+            q_st[2][ty][tx] = tyy;
+            q_st[3][ty][tx] = tzz;
+            q_st[4][ty][tx] = txy1;
+            q_st[5][ty][tx] = txz1;
+            q_st[6][ty][tx] = tyz1;
         }
-        // A's results of plane s for B's next step (B of this step has read the slots)
-        q_st[0][ty][tx] = P;
-        q_st[1][ty][tx] = txx;
-        q_st[2][ty][tx] = tyy;
-        q_st[3][ty][tx] = tzz;
-        q_st[4][ty][tx] = txy1;
-        q_st[5][ty][tx] = txz1;
-        q_st[6][ty][tx] = tyz1;
         qv[0] = va[0];
         qv[1] = va[1];
         qv[2] = va[2];
@@ -1418,6 +1438,35 @@ template <class T>
 constexpr int kBY3 = std::is_same<T, double>::value ? CHMY_FUSED_BY64 : CHMY_FUSED_BY32;
 template <class T>
 constexpr int kTW3 = std::is_same<T, double>::value ? CHMY_FUSED_TW64 : CHMY_FUSED_TW32;
+
+// Planes per block of the two-iteration pass.  One block runs per CU at a
+// time (LDS), so a launch of B blocks takes ceil(B / CUs) rounds, each as
+// long as one block's march: planes + 3 first-iteration steps.  The chunk
+// count minimising rounds x steps wins (255^3 on 256 CUs: 7 chunks of 37
+// planes, 1015 blocks in 4 rounds; measured ms per iteration at chunk
+// lengths 16 / 22 / 24 / 26 / 37 / 43 / 52: 0.784 / 0.739 / 0.775 / 0.734 /
+// 0.694 / 0.760 / 0.705, the order this cost gives).
+static int32_t tb3_chunk(int64_t tiles, int64_t planes) {
+    static int ncu[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (ncu[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        ncu[dev] = n;
+    }
+    int64_t best = -1, bestk = planes;
+    for (int64_t c = 1; c <= planes; ++c) {
+        const int64_t k = (planes + c - 1) / c;
+        if (k < kTB3Z) break;
+        const int64_t cost = ((tiles * c + ncu[dev] - 1) / ncu[dev]) * (k + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            bestk = k;
+        }
+    }
+    return (int32_t)bestk;
+}
 
 template <class T>
 static Kc<T> make_consts(double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu) {
@@ -1512,10 +1561,10 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
             a.iy = T(g->inv_spacing[1]);
             a.iz = T(g->inv_spacing[2]);
             a.k = make_consts<T>(eta_s, eve_s, G, dt, dPr, dr, nu);
-            a.kchunk = kTB3Z;
             a.ntx = (int32_t)((g->n[0] + 2 + kFX - 5) / (kFX - 4));
             a.nty = (int32_t)((g->n[1] + 2 + kTB3Y<T> - 4) / (kTB3Y<T> - 3));
-            a.nchunks = (int32_t)((g->n[2] + 2 + kTB3Z - 1) / kTB3Z);
+            a.kchunk = tb3_chunk((int64_t)a.ntx * a.nty, g->n[2] + 2);
+            a.nchunks = (int32_t)((g->n[2] + 2 + a.kchunk - 1) / a.kchunk);
             const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
             dim3 blk(kFX, kTB3Y<T>, 1), grd(nb, 1, 1);
             const bool hf = a.has_f;
