@@ -1,14 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: pseudo-transient 3D viscoelastic Stokes at 255³, FP64, one MI355X.
 
-A step is one PT iteration (examples/stokes_3d_inc_ve_T.jl:164-165): the fused
-stress+velocity kernel over the Launcher box plus the free-slip V boundary
-batch, on fields resident in HBM.  Reported value: T_eff = algorithmic bytes
+A step is one PT iteration (examples/stokes_3d_inc_ve_T.jl:164-165) on
+fields resident in HBM.  Iterations run two per pass: the two-iteration kernel
+(update_stress!, update_velocity! and the free-slip V batch of the first
+iteration evaluated in-kernel, then the second iteration) followed by the V
+batch, so K steps are K/2 passes.  Reported value: T_eff = algorithmic bytes
 per iteration / time per iteration (GB/s), with algorithmic bytes = 27 array
 passes over exact interior counts (updated Vx,Vy,Vz,Pr,τ×6 twice; read
 τ_old×6 and η once).  Setup: [-0.5,0.5]³, background η = 1 with a sphere of
 radius 0.1 at η = 1e-3, times exp(-T) with T = 0.1·exp(-|x|²/0.05²); G = 1;
 dt from the reference's diffusion limit; pure-shear free-slip walls ±1.
+
+The same line carries the other two targets: the time to an FP64 residual of
+1e-8 with FP32 sweeps + FP64 defect correction against plain FP64 (2D at
+2047², 3D at 255³), and the SIA adjoint/forward cost with its finite
+difference check (512², 300 steps, 30 checkpoints).
 """
 import argparse
 import json
@@ -23,17 +30,18 @@ sys.path.insert(0, os.path.join(ROOT, "chmy.jl_amd"))
 PEAK_HBM_GBS = 8000.0
 
 
-def counter_traffic(n, dtype):
-    """HBM bytes per fused launch from the committed rocprofv3 counter summary
-    of this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scale factors of a
-    1 GiB device copy), or None when the summary does not cover (n, dtype)."""
+def counter_traffic(n, dtype, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the scale
+    factors of a 1 GiB device copy), or None when the summary does not cover
+    (kernel, n, dtype)."""
     path = os.path.join(ROOT, "profiles", "fused3d_stokes_traffic.txt")
     try:
         for line in open(path):
             line = line.strip()
             if line.startswith("{"):
                 d = json.loads(line)
-                if d.get("n") == n and d.get("dtype") == dtype:
+                if d.get("n") == n and d.get("dtype") == dtype and d.get("kernel", "fused3d_kernel") == kernel:
                     return d["hbm_bytes_per_launch"]
     except OSError:
         pass
@@ -70,10 +78,14 @@ def setup(n, dtype, exact=False):
     return S
 
 
-def cpu_baseline(n, iters):
+def cpu_baseline(n, iters, threads=None, zslab=None):
     """Time the oracle (C++ restatement of the reference's CPU path: unfused
     update_stress!, update_velocity!, bc! per iteration, OpenMP) on a bounded
-    sample of the same workload."""
+    sample of the same workload: one warm-up iteration, then the median of
+    `iters` iterations, on `threads` OpenMP threads (default: all).  With
+    `zslab` the two kernels run over the Launcher box's first zslab planes
+    only (the bc! batch as always); returns (seconds, threads, fraction of the
+    Launcher box's points covered)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -101,20 +113,30 @@ def cpu_baseline(n, iters):
         s[D] = (O.Dirichlet(vals[D][0]), O.Dirichlet(vals[D][1]))
         spec.append((Vf[D], s))
 
+    b = O.box((0, 0, 0), (n + 2, n + 2, zslab)) if zslab else None
+    frac = zslab / (n + 2) if zslab else 1.0
+
     def step():
-        O.update_stress(og, tau, Pr, dV, Vf, told, eta, 0.0, 1.0, dt, p["dtau_Pr"], p["dtau_r"])
-        O.update_velocity(og, Vf, rV, Pr, tau, None, 0.0, p["nudtau"], eta_ml=eml, G=1.0, dt=dt)
+        O.update_stress(og, tau, Pr, dV, Vf, told, eta, 0.0, 1.0, dt, p["dtau_Pr"], p["dtau_r"], b=b)
+        O.update_velocity(og, Vf, rV, Pr, tau, None, 0.0, p["nudtau"], eta_ml=eml, G=1.0, dt=dt, b=b)
         O.bc_apply(og, spec)
 
-    step()  # warm-up
-    ts = []
-    for _ in range(iters):
-        t0 = time.perf_counter()
-        step()
-        ts.append(time.perf_counter() - t0)
-    el = sorted(ts)[len(ts) // 2]   # median
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return el, cores
+    if threads is not None:
+        O.set_threads(threads)
+        cores = threads
+    try:
+        step()  # warm-up
+        ts = []
+        for _ in range(iters):
+            t0 = time.perf_counter()
+            step()
+            ts.append(time.perf_counter() - t0)
+    finally:
+        if threads is not None:
+            O.set_threads(0)
+    el = sorted(ts)[len(ts) // 2]   # median
+    return el, cores, frac
 
 
 def cpu_model():
@@ -184,7 +206,7 @@ def time_to_converge(args):
            "config": {"workload": f"{nd}D VE Stokes PT, {args.n}^{nd} cells, inclusions, eps {args.eps}",
                       "inner": args.inner, "ncheck": args.ncheck},
            "runs": res, "field_errors_vs_f64": errs}
-    print(json.dumps(out))
+    return out
 
 
 def algorithmic_bytes_2d(n, itemsize=8):
@@ -411,7 +433,7 @@ def sia_gradient(args):
                       "forward_ms": round(t_f * 1e3, 3), "gradient_ms": round(t_g * 1e3, 3),
                       "repeats": args.sia_reps, "J": J, "dt_limit_at_end": dt_lim},
            "fd_check": {"directions": args.sia_dirs, "h": args.sia_h, "worst_rel_err": worst, "rows": rows}}
-    print(json.dumps(res))
+    return res
 
 
 def stokes_gradient(args):
@@ -527,14 +549,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=255)
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-iters-1t", type=int, default=5)
+    ap.add_argument("--no-targets", action="store_true",
+                    help="the T_eff line only (no time-to-1e-8 and SIA adjoint parts)")
+    ap.add_argument("--no-ttc3d", action="store_true", help="skip the 3D time-to-1e-8 part (about 2 minutes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--exact", action="store_true",
                     help="the reference's expression order (bitwise equal to update_stress! + update_velocity!)")
-    ap.add_argument("--mode", default="fused", choices=["fused", "unfused", "pairs"],
-                    help="fused: one-pass double-buffered iteration; unfused: the reference's "
-                         "update_stress! + update_velocity! kernels; pairs: two iterations per pass "
-                         "(the free-slip batch in-kernel between them), timed per pass")
+    ap.add_argument("--mode", default="pass2", choices=["pass2", "fused", "unfused", "pairs"],
+                    help="pass2: two iterations per pass (the free-slip batch in-kernel between them); "
+                         "fused: one iteration per launch; unfused: the reference's update_stress! + "
+                         "update_velocity! kernels; pairs: chmy_stokes_iterate two at a time, timed per pass")
     ap.add_argument("--ttc", choices=["2d", "3d"], default=None,
                     help="time-to-convergence comparison instead of the T_eff line")
     ap.add_argument("--eps", type=float, default=1e-8)
@@ -567,7 +593,7 @@ def main():
     if args.ttc:
         if args.ttc == "2d" and args.n == 255:
             args.n = 2047   # the 2D workload's grid
-        time_to_converge(args)
+        print(json.dumps(time_to_converge(args)))
         return
     if args.stokes_adjoint:
         stokes_gradient(args)
@@ -583,7 +609,7 @@ def main():
     if args.sia:
         if args.n == 255:
             args.n = 512
-        sia_gradient(args)
+        print(json.dumps(sia_gradient(args)))
         return
 
     import torch
@@ -595,51 +621,65 @@ def main():
     if args.mode == "pairs":
         pairs_rate(S, args, isz)
         return
-    fused = args.mode == "fused"
-    for _ in range(args.warmup):
-        S.iteration() if fused else S.iteration_unfused()
-    torch.cuda.synchronize()
-    K = args.steps
+    A = algorithmic_bytes(n, isz)
+    if args.mode == "pass2":
+        line = pass2_line(S, args, n, isz, A)
+    else:
+        line = single_line(S, args, n, isz, A)
+    if not args.no_cpu and args.mode != "unfused":
+        line["cpu_baseline"] = cpu_line(n, A, args)
+    if args.mode == "pass2" and not args.no_targets:
+        del S
+        torch.cuda.empty_cache()
+        line["targets"] = other_targets(args)
+    print(json.dumps(line))
+
+
+def timed_steps(S, K, call, use_events):
+    """K calls of `call` (each followed by the V batch and the buffer swap),
+    bracketed by a synchronize on both sides; HIP events on the launch stream
+    around each kernel call.  Returns (total ms, mean kernel ms)."""
+    import torch
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_start.record()
     for q in range(K):
-        if fused:
-            ev[q][0].record()
-            S._iter_call(False)
-            ev[q][1].record()
-            S.apply_bc(S.V2)
-            S._swap()
-        else:
-            # kernel time here = both reference kernels (stress + velocity); the BC batch follows
-            ev[q][0].record()
-            S.iteration_unfused_kernels()
-            ev[q][1].record()
-            S.apply_bc(S.V)
+        ev[q][0].record()
+        call()
+        ev[q][1].record()
+        S.apply_bc(S.V2)
+        S._swap()
     t_end.record()
     torch.cuda.synchronize()
-    total_ms = t_start.elapsed_time(t_end)
+    return t_start.elapsed_time(t_end), sum(a.elapsed_time(b) for a, b in ev) / K
+
+
+def pass2_line(S, args, n, isz, A):
+    """The default line: K PT iterations as K/2 two-iteration passes."""
+    import torch
+    for _ in range(max(1, args.warmup // 2)):
+        S.pass2()
+    K = max(2, args.steps - args.steps % 2)
+    total_ms, k_ms = timed_steps(S, K // 2, S._pass2_call, True)
     ms_step = total_ms / K
-    k_ms = sum(a.elapsed_time(b) for a, b in ev) / K
-    A = algorithmic_bytes(n, isz)
     teff = A / (ms_step * 1e-3) / 1e9
-    achieved = A / (k_ms * 1e-3) / 1e9
-    # residual sanity (not timed)
-    if fused:
-        S.iteration(write_res=True)
-    else:
-        S.iteration_unfused()
+    achieved = 2 * A / (k_ms * 1e-3) / 1e9   # two iterations' algorithmic bytes per launch
+    # the same pass in the reference's expression order (bitwise equal to
+    # update_stress! + update_velocity!), and one iteration per launch
+    S.exact = True
+    for _ in range(2):
+        S.pass2()
+    ex_total, ex_k = timed_steps(S, 20, S._pass2_call, True)
+    S.exact = False
+    for _ in range(2):
+        S.iteration()
+    one_total, one_k = timed_steps(S, 40, lambda: S._iter_call(False), True)
+    # residual sanity (not timed): one iteration writing the residual fields
+    S.iteration(write_res=True)
     err = S.residual()
-    cpu = None
-    if not args.no_cpu:
-        c_s, cores = cpu_baseline(n, args.cpu_iters)
-        cpu = {"value": round(A / c_s / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-               "sample": f"median of {args.cpu_iters} PT iterations at {n}^3 FP64 (oracle: unfused "
-                         f"update_stress!, update_velocity!, bc!, OpenMP; {c_s:.3f} s/iteration; "
-                         f"CPU {cpu_model()})"}
-    out = {
+    return {
         "metric": "T_eff GB/s (% HBM peak); time-to-1e-8 FP32+fix vs FP64; adjoint/fwd cost",
         "value": round(teff, 2),
         "unit": "GB/s",
@@ -653,19 +693,122 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic (seedless analytic inclusion setup)",
         "config": {"workload": f"3D PT viscoelastic Stokes, {n}^3 cells, one PT iteration per step "
-                               "(fused stress+velocity kernel + free-slip V batch)",
+                               "(two-iteration pass: fused stress+velocity, free-slip V batch in-kernel, "
+                               "stress+velocity; then the V batch)",
                    "grid": [n, n, n], "teff_pct_of_peak": round(100 * teff / PEAK_HBM_GBS, 2),
                    "algorithmic_bytes_per_step": A,
                    "teff_reference_convention_GBs": round(reference_bytes(n, isz) / (ms_step * 1e-3) / 1e9, 2),
+                   "exact_form": {"kernel_ms_per_iteration": round(ex_k / 2, 4),
+                                  "ms_per_step": round(ex_total / 40, 4),
+                                  "teff_GBs": round(A / (ex_total / 40 * 1e-3) / 1e9, 2)},
+                   "one_iteration_per_launch": {"kernel": "fused3d_kernel", "kernel_ms": round(one_k, 4),
+                                                "ms_per_step": round(one_total / 40, 4),
+                                                "teff_GBs": round(A / (one_total / 40 * 1e-3) / 1e9, 2)},
                    "residual_after_timing": err},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": counter_traffic(n, args.dtype) if fused else None,
-                     "kernel": "fused3d_kernel" if fused else "stress_kernel + velocity_kernel",
-                     "kernel_ms": round(k_ms, 4)},
-        "cpu_baseline": cpu,
+                     "traffic": counter_traffic(n, args.dtype, "fused3d_tb_kernel"),
+                     "kernel": "fused3d_tb_kernel", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": 2 * A},
     }
-    print(json.dumps(out))
+
+
+def single_line(S, args, n, isz, A):
+    """--mode fused / unfused: one iteration per launch (the fused kernel, or
+    the reference's update_stress! + update_velocity! kernels)."""
+    fused = args.mode == "fused"
+    for _ in range(args.warmup):
+        S.iteration() if fused else S.iteration_unfused()
+    K = args.steps
+    if fused:
+        total_ms, k_ms = timed_steps(S, K, lambda: S._iter_call(False), True)
+    else:
+        import torch
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record()
+        for q in range(K):
+            ev[q][0].record()
+            S.iteration_unfused_kernels()
+            ev[q][1].record()
+            S.apply_bc(S.V)
+        t1.record()
+        torch.cuda.synchronize()
+        total_ms, k_ms = t0.elapsed_time(t1), sum(a.elapsed_time(b) for a, b in ev) / K
+    ms_step = total_ms / K
+    teff = A / (ms_step * 1e-3) / 1e9
+    achieved = A / (k_ms * 1e-3) / 1e9
+    if fused:
+        S.iteration(write_res=True)
+    else:
+        S.iteration_unfused()
+    err = S.residual()
+    kernel = "fused3d_kernel" if fused else "stress_kernel + velocity_kernel"
+    return {
+        "metric": "T_eff GB/s (% HBM peak); time-to-1e-8 FP32+fix vs FP64; adjoint/fwd cost",
+        "value": round(teff, 2), "unit": "GB/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype, "data": "synthetic (seedless analytic inclusion setup)",
+        "config": {"workload": f"3D PT viscoelastic Stokes, {n}^3 cells, one PT iteration per step "
+                               f"({'fused stress+velocity kernel' if fused else 'reference kernels'} + free-slip V batch)",
+                   "grid": [n, n, n], "teff_pct_of_peak": round(100 * teff / PEAK_HBM_GBS, 2),
+                   "algorithmic_bytes_per_step": A, "residual_after_timing": err},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": counter_traffic(n, args.dtype, "fused3d_kernel") if fused else None,
+                     "kernel": kernel, "kernel_ms": round(k_ms, 4)},
+    }
+
+
+def cpu_line(n, A, args):
+    """The oracle on the host's cores: all threads (median of --cpu-iters
+    iterations) and one thread (median of --cpu-iters-1t)."""
+    c_s, cores, _ = cpu_baseline(n, args.cpu_iters)
+    zs = 32
+    c1_s, _, frac = cpu_baseline(n, args.cpu_iters_1t, threads=1, zslab=zs)
+    return {"value": round(A / c_s / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"median of {args.cpu_iters} PT iterations at {n}^3 FP64 after one warm-up (oracle: unfused "
+                      f"update_stress!, update_velocity!, bc!, OpenMP over {cores} threads; {c_s:.3f} s/iteration; "
+                      f"CPU {cpu_model()})",
+            "one_thread": {"value": round(A * frac / c1_s / 1e9, 3), "unit": "GB/s", "cores": 1,
+                           "sample": f"median of {args.cpu_iters_1t} iterations after one warm-up, the kernels over "
+                                     f"the first {zs} of {n + 2} planes of the Launcher box (bytes scaled by "
+                                     f"{frac:.4f}), {c1_s:.3f} s/iteration"}}
+
+
+def other_targets(args):
+    """The metric's other two parts on their own workloads: the time to an
+    FP64 residual of 1e-8 (FP32 sweeps + FP64 defect correction vs plain FP64)
+    in 2D at 2047² and in 3D at 255³, and the SIA adjoint/forward cost at 512²
+    with its finite-difference check."""
+    import copy
+    import torch
+    out = {}
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    a2 = copy.copy(args)
+    a2.ttc, a2.n = "2d", 2047
+    r = time_to_converge(a2)
+    out["time_to_1e-8_2d"] = {"value": r["value"], "unit": r["unit"], "config": r["config"], "runs": r["runs"],
+                             "field_errors_vs_f64": r["field_errors_vs_f64"]}
+    log("2D time to 1e-8:", r["value"])
+    torch.cuda.empty_cache()
+    a3 = copy.copy(args)
+    a3.n = 512
+    r = sia_gradient(a3)
+    out["sia_adjoint"] = {"value": r["value"], "unit": r["unit"], "config": r["config"],
+                          "fd_worst_rel_err": r["fd_check"]["worst_rel_err"], "fd_directions": r["fd_check"]["directions"],
+                          "fd_h": r["fd_check"]["h"]}
+    log("SIA gradient cost:", r["value"])
+    torch.cuda.empty_cache()
+    if not args.no_ttc3d:
+        a4 = copy.copy(args)
+        a4.ttc, a4.n = "3d", 255
+        r = time_to_converge(a4)
+        out["time_to_1e-8_3d"] = {"value": r["value"], "unit": r["unit"], "config": r["config"], "runs": r["runs"],
+                                 "field_errors_vs_f64": r["field_errors_vs_f64"]}
+        log("3D time to 1e-8:", r["value"])
+    return out
 
 
 if __name__ == "__main__":

@@ -78,6 +78,8 @@ SIGNATURES = {
     "chmy_stokes_iterate": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32),
                             P(C.c_int32), P(ChmyBC), C.c_int, C.c_int, C.c_int, D, D, D, D, D, D, D, P(C.c_int),
                             VP],
+    "chmy_stokes_iter2": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32), P(C.c_int32),
+                          P(ChmyBC), C.c_int, D, D, D, D, D, D, D, VP],
     "chmy_stokes_residual": [PG, PF, PF, PF, PF, PF, PF, PF, D, D, D, PF, PF, PF, PF, PF, VP],
     "chmy_stokes_accumulate": [PG, PF, PF, PF, PF, PF, PF, VP],
     "chmy_stokes_grad_eta": [PG, PF, PF, PF, PF, C.c_int, VP],

@@ -152,6 +152,29 @@ def maxabs(f, box=None):
     return out.value
 
 
+_MAX_BUF = {}
+
+
+def maxabs_many(fields):
+    """maximum(abs.(interior(f))) of each field: the reductions run on the
+    device into one buffer and the host reads them once (one synchronisation
+    for a whole convergence check)."""
+    import torch
+    n = len(fields)
+    if n == 0:
+        return []
+    dev = fields[0].data.device
+    buf = _MAX_BUF.get(dev)
+    if buf is None or buf.numel() < n:
+        buf = torch.empty(max(n, 16), dtype=torch.float64, device=dev)
+        _MAX_BUF[dev] = buf
+    for q, f in enumerate(fields):
+        b = _box((1,) * f.nd, f.dims)
+        _lib.call("chmy_maxabs_dev", C.byref(f.desc()), C.byref(b),
+                  C.cast(C.c_void_p(buf.data_ptr() + 8 * q), C.POINTER(C.c_double)), current_stream())
+    return buf[:n].cpu().tolist()
+
+
 def sumsq(f, box=None):
     out = C.c_double()
     b = _box((1,) * f.nd, f.dims) if box is None else box

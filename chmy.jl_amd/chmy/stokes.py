@@ -29,7 +29,7 @@ from .kernels import _arr, _box, maxloc, HipKernel
 from .kernel_launch import Launcher
 
 
-ITER_RESIDUAL, ITER_EXACT, ITER_PAIRS = 1, 2, 4
+ITER_RESIDUAL, ITER_EXACT, ITER_PAIRS, ITER_SINGLE = 1, 2, 4, 8
 
 
 def pt_params(grid, re_m=2.3 * math.pi, r=0.5):
@@ -166,16 +166,16 @@ class PTStokes:
             self._vb = (counts, (C.c_int32 * n)(*comp), (_lib.ChmyBC * n)(*conds), keep)
         return self._vb
 
-    def iterate(self, niter, residual_last=False, pairs=False):
+    def iterate(self, niter, residual_last=False, pairs=True):
         """niter PT iterations in one native call (fused kernel + V batch,
-        buffers alternating); (V, tau, Pr) hold the result afterwards.  2D
-        runs them in pairs (one pass per two iterations); `pairs` asks for
-        the same in 3D (bitwise the same results)."""
+        buffers alternating); (V, tau, Pr) hold the result afterwards.  With
+        the free-slip batch they run in pairs (one pass per two iterations,
+        bitwise the same results); pairs=False keeps single iterations."""
         if niter <= 0:
             return
         g = self.grid
         counts, comp, conds, _keep = self._vbatch()
-        base = (ITER_EXACT if self.exact else 0) | (ITER_PAIRS if pairs else 0)
+        base = (ITER_EXACT if self.exact else 0) | (0 if pairs else ITER_SINGLE)
         par = C.c_int(0)
         _lib.call("chmy_stokes_iterate", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
                   _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(self.tau_old),
@@ -190,6 +190,29 @@ class PTStokes:
         if par.value == 1:
             self._swap()
         self.iters += niter
+
+    def _pass2_call(self):
+        """chmy_stokes_iter2: two PT iterations in one pass from (V, tau, Pr)
+        into (V2, tau2, Pr2), the free-slip batch in-kernel between them."""
+        g = self.grid
+        counts, comp, conds, _keep = self._vbatch()
+        _lib.call("chmy_stokes_iter2", C.byref(g.desc()), _arr(self.V), _arr(self.tau), C.byref(self.Pr.desc()),
+                  _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(self.tau_old),
+                  C.byref(self.eta.desc()) if self.eta is not None else None,
+                  C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  _arr(self.f_V) if self.f_V is not None else None,
+                  C.byref(self.f_P.desc()) if self.f_P is not None else None,
+                  counts, comp, conds, ITER_EXACT if self.exact else 0, self.eta_s, self.eta_ve_s, self.G,
+                  self.dt, self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], current_stream())
+
+    def pass2(self):
+        """Two PT iterations: one two-iteration pass, then the V batch on its
+        output, then swap (the same state as two iteration() calls)."""
+        self._pass2_call()
+        self.apply_bc(self.V2)
+        self._swap()
+        self.iters += 2
 
     def iteration_unfused(self):
         """The reference's sequence: update_stress!, update_velocity! (per-point
@@ -224,9 +247,9 @@ class PTStokes:
             rb = batch(self.grid, *[(self.r_V[D], {names[D]: Dirichlet()}) for D in range(self.nd)])
             self._rbc[key] = rb
         rb.apply()
-        from .kernels import maxabs
-        errs = [maxabs(self.divV) * tau_sc] + [maxabs(self.r_V[D]) * lz / psc for D in range(self.nd)]
-        return errs
+        from .kernels import maxabs_many
+        m = maxabs_many([self.divV] + [self.r_V[D] for D in range(self.nd)])
+        return [m[0] * tau_sc] + [m[1 + D] * lz / psc for D in range(self.nd)]
 
     def true_residual(self):
         """Residual norms of the current f64 state evaluated from scratch:
@@ -348,9 +371,17 @@ class MixedPTStokes:
                   _arr(x.f_V) if x.f_V is not None else None, x.eta_s, x.G, x.dt,
                   _arr(d.tau_old), _arr(d.f_V), C.byref(d.f_P.desc()), C.byref(x.divV.desc()), _arr(x.r_V),
                   current_stream())
-        from .kernels import maxabs
-        err_tau = max(maxabs(t) for t in d.tau_old) / (x.G * x.dt)
-        return x.residual() + [err_tau]
+        from .kernels import maxabs_many
+        names = x.grid.axes_names()
+        key = id(x.r_V[0])
+        rb = x._rbc.get(key)
+        if rb is None:
+            rb = batch(x.grid, *[(x.r_V[D], {names[D]: Dirichlet()}) for D in range(x.nd)])
+            x._rbc[key] = rb
+        rb.apply()
+        m = maxabs_many([x.divV] + [x.r_V[D] for D in range(x.nd)] + list(d.tau_old))
+        err_tau = max(m[1 + x.nd:]) / (x.G * x.dt)
+        return m[:1 + x.nd] + [err_tau]
 
     def set_eta(self, eta_field):
         self.x.set_eta(eta_field)
@@ -363,8 +394,12 @@ class MixedPTStokes:
         self.sweeps = 0
 
     def reset_correction(self):
+        """d = 0: the set the next sweep reads.  The other set needs no reset:
+        the sweep overwrites every point of it inside the Launcher box, and
+        outside the box both sets hold the zeros they were allocated with,
+        since no kernel and no homogeneous batch rule writes a non-zero there."""
         d = self.d
-        for f in list(d.V) + list(d.tau) + [d.Pr] + list(d.V2) + list(d.tau2) + [d.Pr2]:
+        for f in list(d.V) + list(d.tau) + [d.Pr]:
             f.data.zero_()
 
     def accumulate(self):

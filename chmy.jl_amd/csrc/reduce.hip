@@ -3,7 +3,8 @@
 // deterministic f64 sum of squares.  max|x| is exact and order-independent:
 // blocks reduce in registers (wave64 shuffles) and LDS, then combine with one
 // 64-bit atomic max per block on the bit pattern (non-negative doubles order
-// like their unsigned bits; NaN's bits exceed +inf's, so NaN propagates).
+// like their unsigned bits; NaN's bits exceed +inf's, so NaN propagates:
+// fmax drops NaN, so a NaN seen anywhere is carried as a flag instead).
 #include <mutex>
 
 #include "common.hpp"
@@ -26,31 +27,49 @@ __global__ void zero_u64(unsigned long long* p) {
     if (threadIdx.x == 0) p[0] = 0ull;
 }
 
+// One wave per x-row of the box at a time (rows strided over the grid's
+// waves): the row's base offset is formed once, every lane's loads of the
+// row are issued before the max, and no per-element division is needed.
+constexpr int kRowChunk = 8;   // 64-wide slices of a row loaded before the max
+
 template <class T>
 __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned long long* out) {
     __shared__ double part[kRB / 64];
-    __shared__ int nanflag;
-    if (threadIdx.x == 0) nanflag = 0;
-    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t nx = b.len[0], ny = b.len[1];
-    const int64_t tot = nx * ny * b.len[2];
+    const int64_t rows = ny * b.len[2];
+    const int64_t nw = (int64_t)gridDim.x * (kRB / 64);
     double m = 0.0;
-    int isnan_ = 0;
-    for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
-        int64_t i = t % nx, r = t / nx;
-        int64_t j = r % ny, k = r / ny;
-        double a = fabs(as_double(f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k)));
-        if (a != a) isnan_ = 1;
-        else m = fmax(m, a);
+    bool isnan_ = false;
+    for (int64_t r = (int64_t)blockIdx.x * (kRB / 64) + w; r < rows; r += nw) {
+        const int64_t j = r % ny, k = r / ny;
+        const T* row = &f(b.lo[0], b.lo[1] + j, b.lo[2] + k);
+        for (int64_t x0 = 0; x0 < nx; x0 += 64 * kRowChunk) {
+            double a[kRowChunk];
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) {
+                const int64_t x = x0 + 64 * q + lane;
+                a[q] = x < nx ? fabs(as_double(row[x])) : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) {
+                isnan_ |= a[q] != a[q];
+                m = fmax(m, a[q]);
+            }
+        }
     }
     m = wave_max(m);
-    if (isnan_) nanflag = 1;
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    const bool anynan = __any(isnan_);
+    if (lane == 0) part[w] = anynan ? __longlong_as_double(0x7ff8000000000000ll) : m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double v = part[0];
-        for (int w = 1; w < kRB / 64; ++w) v = fmax(v, part[w]);
-        if (nanflag) v = __longlong_as_double(0x7ff8000000000000ll);
+        bool nan = v != v;
+        for (int q = 1; q < kRB / 64; ++q) {
+            nan |= part[q] != part[q];
+            v = fmax(v, part[q]);
+        }
+        if (nan) v = __longlong_as_double(0x7ff8000000000000ll);
         atomicMax(out, (unsigned long long)__double_as_longlong(v));
     }
 }
@@ -138,6 +157,15 @@ static unsigned grid_blocks(const Box3& b) {
     return (unsigned)nb;
 }
 
+// maxabs: one wave per row, about 8 rows per wave at most
+static unsigned row_blocks(const Box3& b) {
+    int64_t rows = b.len[1] * b.len[2];
+    int64_t nb = (rows + 4 * 8 - 1) / (4 * 8);
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
 }  // namespace chmy
 
 using namespace chmy;
@@ -152,11 +180,11 @@ int chmy_maxabs_dev(const chmy_field_t* f, const chmy_box_t* box, double* out_de
     unsigned long long* o = reinterpret_cast<unsigned long long*>(out_dev);
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, o);
     if (f->dtype == CHMY_F64)
-        hipLaunchKernelGGL(maxabs_kernel<double>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<double>(*f), o);
+        hipLaunchKernelGGL(maxabs_kernel<double>, dim3(row_blocks(b)), dim3(kRB), 0, s, b, view<double>(*f), o);
     else if (f->dtype == CHMY_BF16)
-        hipLaunchKernelGGL(maxabs_kernel<bf16s>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<bf16s>(*f), o);
+        hipLaunchKernelGGL(maxabs_kernel<bf16s>, dim3(row_blocks(b)), dim3(kRB), 0, s, b, view<bf16s>(*f), o);
     else
-        hipLaunchKernelGGL(maxabs_kernel<float>, dim3(grid_blocks(b)), dim3(kRB), 0, s, b, view<float>(*f), o);
+        hipLaunchKernelGGL(maxabs_kernel<float>, dim3(row_blocks(b)), dim3(kRB), 0, s, b, view<float>(*f), o);
     CHMY_HIP(hipGetLastError());
     return 0;
 }

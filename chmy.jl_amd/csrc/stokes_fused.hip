@@ -130,11 +130,17 @@ struct Arith<T, true> {
     __device__ static T vel(T v, T r, T c, const Kc<T>& k) { return v + (r * k.nu) / c; }
 };
 
+// Divisions of the fast form: IEEE in FP64; in FP32 (the defect-correction
+// sweeps) a times the hardware reciprocal of b (v_rcp_f32, 1 ulp), for its 7
+// divisions per point
+__device__ __forceinline__ double fast_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <class T>
 struct Arith<T, false> {
     struct Loc { T a, b; };   // a = eta_ve dtau_r, b = eta_ve dtau_Pr
     __device__ static Loc field(T eta, const Kc<T>& k) {
-        const T eve = eta / fma(eta, k.iGdt, T(1));
+        const T eve = fast_div(eta, fma(eta, k.iGdt, T(1)));
         return Loc{eve * k.dr, eve * k.dPr};
     }
     __device__ static Loc scalar(const Kc<T>& k) { return Loc{k.a_s, k.b_s}; }
@@ -145,7 +151,7 @@ struct Arith<T, false> {
         return fma(t, k.omdr, L.a * fma(to, k.iGdt, T(2) * e));
     }
     __device__ static T pr(T P, T dv, Loc L, const Kc<T>& k) { return fma(-dv, L.b, P); }
-    __device__ static T vcoef_field(T em, const Kc<T>& k) { return k.nu * (T(1) / em + k.iGdt); }
+    __device__ static T vcoef_field(T em, const Kc<T>& k) { return k.nu * (fast_div(T(1), em) + k.iGdt); }
     __device__ static T vcoef_scalar(const Kc<T>& k) { return k.cv_s; }
     __device__ static T vel(T v, T r, T c, const Kc<T>& k) { return fma(r, c, v); }
 };
@@ -922,7 +928,7 @@ struct FreeSlip3 {
 #define CHMY_FUSED_TB3_BY64 12
 #endif
 #ifndef CHMY_FUSED_TB3_BY32
-#define CHMY_FUSED_TB3_BY32 12
+#define CHMY_FUSED_TB3_BY32 16
 #endif
 // the shortest z chunk of the two-iteration pass (the host picks the length)
 #ifndef CHMY_FUSED_TB3_ZCHUNK
@@ -974,9 +980,9 @@ __device__ __forceinline__ float xp1(float v) { return __int_as_float(dpp_shl1(_
 // bitwise those of two fused3d_kernel launches with the batch between them.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
 __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
-    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only");
+    static_assert(OPT == 0 || OPT == 1 || OPT == 2, "two-iteration pass: plain, forcing-only or rho g-only");
     constexpr int BY = kTB3Y<T>, NR = BY + 1, TW = kFX;
-    constexpr bool HF = OPT == 1;
+    constexpr bool HF = OPT == 1, RG = OPT == 2;
     using AR = Arith<T, EXACT>;
     using Loc = typename AR::Loc;
     const Kc<T>& K = a.k;
@@ -1148,6 +1154,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 fVy = ldf<T, BF>(a.fVy, o[LVY]);
                 fVz = ldf<T, BF>(a.fVz, o[LVZ]);
             }
+            const T rgz = RG ? ld(a.rg, o[LVZ]) : T(0);
             const bool in_s = s >= 0 && s <= nb2;
             const bool in_s1 = s + 1 >= 0 && s + 1 <= nb2;
             s_vx[ty][tx] = Avx;
@@ -1227,6 +1234,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                    (s_tyz[pa][tym][tx] - s_tyz[pa ^ 1][tym][tx]) * iz;
             T rz = ((-((P - AP_m) * iz) + (tzz - Atzz_m) * iz) + (txzr - Atxz) * ix) +
                    (Atyz - s_tyz[pa ^ 1][tym][tx]) * iy;
+            if constexpr (RG) rz = rz - rgz;
             if constexpr (HF) {
                 rx = rx + fVx;
                 ry = ry + fVy;
@@ -1325,6 +1333,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 fVy = ldf<T, BF>(a.fVy, o[LVY] - dz[LVY]);
                 fVz = ldf<T, BF>(a.fVz, o[LVZ] - dz[LVZ]);
             }
+            const T rgz = RG ? ld(a.rg, o[LVZ] - dz[LVZ]) : T(0);
             __syncthreads();
             const bool in_k = r >= 0 && r <= nb2;
             const bool in_k1 = r + 1 >= 0 && r + 1 <= nb2;
@@ -1382,6 +1391,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                    (b_tyz[pb][tym][tx] - b_tyz[pb ^ 1][tym][tx]) * iz;
             T rz = ((-((bP - BP_m) * iz) + (btzz - Btzz_m) * iz) + (txzr - Btxz) * ix) +
                    (Btyz - b_tyz[pb ^ 1][tym][tx]) * iy;
+            if constexpr (RG) rz = rz - rgz;
             if constexpr (HF) {
                 rx = rx + fVx;
                 ry = ry + fVy;
@@ -1555,7 +1565,7 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
         a.L[LYZ] = loc_stride(tau[5]);
         for (int d = 0; d < 3; ++d) a.n[d] = (int32_t)g->n[d];
         if (tb3) {   // two iterations, the free-slip batch between them
-            a.has_rg = 0;
+            a.has_rg = rg != nullptr;
             a.write_res = 0;
             a.ix = T(g->inv_spacing[0]);
             a.iy = T(g->inv_spacing[1]);
@@ -1567,12 +1577,13 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
             a.nchunks = (int32_t)((g->n[2] + 2 + a.kchunk - 1) / a.kchunk);
             const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
             dim3 blk(kFX, kTB3Y<T>, 1), grd(nb, 1, 1);
-            const bool hf = a.has_f;
+            const bool hf = a.has_f, hr = a.has_rg;
 #define CHMY_TB3(FV, EX)                                                                                    \
     do {                                                                                                    \
         constexpr bool F32 = std::is_same<T, float>::value;                                                 \
         if (hf && bf) hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 1, F32>), grd, blk, 0, s, a, *tb3);   \
         else if (hf) hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 1, false>), grd, blk, 0, s, a, *tb3);  \
+        else if (hr) hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 2, false>), grd, blk, 0, s, a, *tb3);  \
         else hipLaunchKernelGGL((fused3d_tb_kernel<T, FV, EX, 0, false>), grd, blk, 0, s, a, *tb3);          \
     } while (0)
             if (fv && exact) CHMY_TB3(true, true);
@@ -1722,9 +1733,9 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
                             double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
                             double nudtau, void* stream, const FreeSlip2* tb, const FreeSlip3* tb3) {
     CHMY_TRY(check_grid(g));
-    if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || (tb3 && rhog) || ((tb || tb3) && (flags & CHMY_ITER_RESIDUAL)))
+    if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || (tb3 && rhog && f_V) || ((tb || tb3) && (flags & CHMY_ITER_RESIDUAL)))
         return set_error(CHMY_E_ARG, "two-iteration pass: free-slip batch of the grid's dims, no residual fields "
-                                     "(and no rho g in 3D)");
+                                     "(and in 3D not rho g together with forcing)");
     if (g->nd < 2) return set_error(CHMY_E_ARG, "Stokes kernels need nd >= 2");
     if (flags & ~(CHMY_ITER_RESIDUAL | CHMY_ITER_EXACT)) return set_error(CHMY_E_ARG, "unknown flags %d", flags);
     const int write_residual = (flags & CHMY_ITER_RESIDUAL) != 0;
@@ -1904,18 +1915,17 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
         for (int q = 0; q < total; ++q) fset[p][q] = Vs[p][bc_comp[q]];
     const chmy_field_t* T[2] = {tau0, tau1};
     const chmy_field_t* P[2] = {Pr0, Pr1};
-    // 2D free-slip batches run iterations in pairs (one pass, the batch
-    // in-kernel between them); the pair writes the other set, as one iteration
-    // does, so the parity flips once per pair
-    // (3D pairs on request: measured slower than single iterations in FP64
-    // and in the correction sweep at 255^3, faster for plain FP32 sweeps)
-    const bool pairs3 = (flags & CHMY_ITER_PAIRS) != 0;
-    flags &= ~CHMY_ITER_PAIRS;
-    flags_last &= ~CHMY_ITER_PAIRS;
+    // free-slip batches run iterations in pairs (one pass, the batch in-kernel
+    // between them); the pair writes the other set, as one iteration does, so
+    // the parity flips once per pair.  CHMY_ITER_SINGLE keeps single
+    // iterations (CHMY_ITER_PAIRS, once the 3D opt-in, is accepted and ignored)
+    const bool single = ((flags | flags_last) & CHMY_ITER_SINGLE) != 0;
+    flags &= ~(CHMY_ITER_PAIRS | CHMY_ITER_SINGLE);
+    flags_last &= ~(CHMY_ITER_PAIRS | CHMY_ITER_SINGLE);
     FreeSlip2 fs;
     FreeSlip3 fs3;
-    const bool tb2 = CHMY_FUSED_TB && nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
-    const bool tb3 = pairs3 && !rhog && nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
+    const bool tb2 = CHMY_FUSED_TB && !single && nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    const bool tb3 = !single && !(rhog && f_V) && nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
     const bool tb = tb2 || tb3;
     int p = *parity_inout;
     for (int it = 0; it < niter;) {
@@ -1937,6 +1947,39 @@ int chmy_stokes_iterate(const chmy_grid_t* g, const chmy_field_t* V0, const chmy
     }
     *parity_inout = p;
     return 0;
+}
+
+// Two PT iterations in one pass with the free-slip V batch between them,
+// evaluated in-kernel: (V, tau, Pr) -> (V_out, tau_out, Pr_out); the caller
+// applies the batch to V_out afterwards (build extension).
+int chmy_stokes_iter2(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau, const chmy_field_t* Pr,
+                      const chmy_field_t* V_out, const chmy_field_t* tau_out, const chmy_field_t* Pr_out,
+                      const chmy_field_t* tau_old, const chmy_field_t* eta, const chmy_field_t* eta_ml,
+                      const chmy_field_t* rhog, const chmy_field_t* f_V, const chmy_field_t* f_P,
+                      const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int flags,
+                      double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                      double nudtau, void* stream) {
+    CHMY_TRY(check_grid(g));
+    if (!bc_counts || !bc_comp || !bcs) return set_error(CHMY_E_ARG, "V batch is NULL");
+    if (flags & ~CHMY_ITER_EXACT) return set_error(CHMY_E_ARG, "chmy_stokes_iter2 takes CHMY_ITER_EXACT only");
+    const int nd = g->nd;
+    int total = 0;
+    for (int q = 0; q < 2 * nd; ++q) {
+        if (bc_counts[q] < 0) return set_error(CHMY_E_ARG, "negative batch count");
+        total += bc_counts[q];
+    }
+    for (int q = 0; q < total; ++q)
+        if (bc_comp[q] < 0 || bc_comp[q] >= nd) return set_error(CHMY_E_ARG, "bc_comp out of range");
+    FreeSlip2 fs;
+    FreeSlip3 fs3;
+    const bool tb2 = nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    const bool tb3 = nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
+    if (!tb2 && !tb3)
+        return set_error(CHMY_E_ARG, "chmy_stokes_iter2 needs the free-slip V batch (each component Dirichlet on "
+                                     "its own dim, Neumann on the others, scalar values, >= 2 cells per dim)");
+    return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, nullptr,
+                            nullptr, flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream,
+                            tb2 ? &fs : nullptr, tb3 ? &fs3 : nullptr);
 }
 
 }  // extern "C"

@@ -223,7 +223,7 @@ int chmy_thermal_step(const chmy_grid_t* grid, const chmy_box_t* box, const chmy
  * the iteration solves for a correction δ driven by a residual R of an f64
  * state: r_V += f_V, and Pr uses (divV - f_P); the stress forcing enters as
  * tau_old = G dt R_tau (see chmy_stokes_residual).  NULL f_V/f_P: plain PT. */
-enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2, CHMY_ITER_PAIRS = 4 };
+enum { CHMY_ITER_RESIDUAL = 1, CHMY_ITER_EXACT = 2, CHMY_ITER_PAIRS = 4, CHMY_ITER_SINGLE = 8 };
 int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                      const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                      const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
@@ -241,10 +241,12 @@ int chmy_stokes_iter(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_
  * *parity_inout names the set holding the new state.
  * Build extension: with a free-slip V batch (each component Dirichlet on its
  * own dim, Neumann on the others, scalar values) and no residual fields,
- * 2D iterations run in pairs, one pass each with the batch applied in-kernel
- * between the two; CHMY_ITER_PAIRS (in `flags`) asks for the same in 3D
- * (without rho g).  A pair writes set 1-p like one iteration, so the parity flips once
- * per pair; results are bitwise those of single iterations. */
+ * iterations run in pairs, one pass each with the batch applied in-kernel
+ * between the two (as chmy_stokes_iter2; in 3D not with rho g and forcing
+ * together); CHMY_ITER_SINGLE (in `flags`) keeps single iterations, and
+ * CHMY_ITER_PAIRS is accepted and ignored.  A pair writes set 1-p like one
+ * iteration, so the parity flips once per pair; results are bitwise those of
+ * single iterations. */
 int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const chmy_field_t* tau0,
                         const chmy_field_t* Pr0, const chmy_field_t* V1, const chmy_field_t* tau1,
                         const chmy_field_t* Pr1, const chmy_field_t* tau_old, const chmy_field_t* eta,
@@ -253,6 +255,26 @@ int chmy_stokes_iterate(const chmy_grid_t* grid, const chmy_field_t* V0, const c
                         const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int niter,
                         int flags, int flags_last, double eta_s, double eta_ve_s, double G, double dt,
                         double dtau_Pr, double dtau_r, double nudtau, int* parity_inout, void* stream);
+
+/* Two PT iterations in one pass (build extension; the reference runs
+ * update_stress!, update_velocity! and the V batch per iteration,
+ * examples/stokes_3d_inc_ve_T.jl:164-165): chmy_stokes_iter from (V, tau, Pr),
+ * the V batch, and chmy_stokes_iter again into (V_out, tau_out, Pr_out), with
+ * the first iteration's state never stored: the batch between the two is
+ * evaluated in-kernel.  The V batch (given as in chmy_stokes_iterate) must be
+ * the free-slip batch: each component Dirichlet on its own dim, Neumann on the
+ * others, scalar values, at least 2 cells per dim; else CHMY_E_ARG.  The
+ * caller applies the batch to V_out afterwards, as after chmy_stokes_iter.
+ * flags: CHMY_ITER_EXACT only.  In 3D, rho g and f_V/f_P not together.
+ * Results are bitwise those of the two chmy_stokes_iter calls with
+ * chmy_bc_batchset between them. */
+int chmy_stokes_iter2(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
+                      const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
+                      const chmy_field_t* Pr_out, const chmy_field_t* tau_old, const chmy_field_t* eta,
+                      const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
+                      const chmy_field_t* f_P, const int32_t* bc_counts, const int32_t* bc_comp,
+                      const chmy_bc_t* bcs, int flags, double eta_s, double eta_ve_s, double G, double dt,
+                      double dtau_Pr, double dtau_r, double nudtau, void* stream);
 
 /* Defect correction (build extension; the reference has no mixed-precision
  * path).  Residuals of an f64 state in the reference's expression order

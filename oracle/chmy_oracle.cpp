@@ -27,6 +27,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <algorithm>
+#include <omp.h>
 
 extern "C" {
 
@@ -828,6 +829,14 @@ template <class T> double sia_Dmax_t(const orc_grid* gg, const orc_field* H, con
 
 // ================================================================ C ABI
 extern "C" {
+
+// OpenMP threads of the host loops: n > 0 sets n, 0 restores the count the
+// process started with (the CPU baseline's 1-thread figure)
+int orc_set_threads(int n) {
+    static const int initial = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : initial);
+    return 0;
+}
 
 int orc_last_error(char* buf, int len) {
     int e = g_err;

@@ -62,6 +62,7 @@ P = C.POINTER
 def _declare(L):
     L.orc_last_error.argtypes = [C.c_char_p, C.c_int]
     L.orc_set_plant.argtypes = [C.c_int]
+    L.orc_set_threads.argtypes = [C.c_int]
     L.orc_get_plant.argtypes = []
     L.orc_vertex.restype = C.c_double
     L.orc_vertex.argtypes = [C.c_double, C.c_double, C.c_long]
@@ -96,6 +97,11 @@ def _declare(L):
     L.orc_sia_Dmax.restype = C.c_double
     L.orc_sia_Dmax.argtypes = [P(OGrid), P(OField), P(OField), P(OField), C.c_double, C.c_int,
                                C.c_double, C.c_double, C.c_double]
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (0: the count the process started with)."""
+    _check(lib().orc_set_threads(int(n)))
 
 
 def _check(rc):

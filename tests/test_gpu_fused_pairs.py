@@ -4,12 +4,12 @@ between and after them, bitwise, on seeded random storage.
 
 chmy_stokes_iterate takes iterations in pairs when the V batch is the
 free-slip batch (each component Dirichlet on its own dim, Neumann on the
-others; scalar values) and no residual fields or rho g are requested (in 3D
-when CHMY_ITER_PAIRS asks for it); the
-reference here is the same entry point's single-iteration path, driven one
-iteration at a time (chmy_stokes_iter + chmy_bc_batchset), which
+others; scalar values) and no residual fields are requested (in 3D not with
+rho g and forcing together); the reference here is the single-iteration path,
+driven one iteration at a time (chmy_stokes_iter + chmy_bc_batchset), which
 tests/test_gpu_fused.py pins to the oracle.  Grids cover partial x tiles and
 y chunks, chunks of one row, and the smallest grid the pass takes (2x2).
+chmy_stokes_iter2 (one pass, no batch after it) is checked the same way.
 """
 import ctypes as C
 
@@ -25,7 +25,7 @@ from chmy.stokes import pt_params
 
 pytestmark = pytest.mark.gpu
 
-ITER_EXACT, ITER_PAIRS = 2, 4
+ITER_EXACT, ITER_PAIRS, ITER_SINGLE = 2, 4, 8
 
 
 @pytest.fixture(scope="module")
@@ -67,7 +67,7 @@ class _Set:
         return list(self.V) + list(self.tau) + [self.Pr]
 
 
-def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed, rhog=False):
+def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed, rhog=False, mode=None):
     dt = torch.float64 if T == np.float64 else torch.float32
     rng = np.random.default_rng(seed)
     fill = lambda f, lo=-1.0, hi=1.0: f.data.copy_(
@@ -112,8 +112,21 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed, rhog=False)
                          _arr(list(fV)) if fV is not None else None,
                          C.byref(fP.desc()) if fP is not None else None, C.byref(divV.desc()), _arr(list(rV)))
     scal = (eta_s, eve, G, dtp, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
-    if pairs:   # 2D takes pairs by itself, 3D on request
-        fp = flags | (ITER_PAIRS if g.nd == 3 else 0)
+    if mode == "iter2":   # one pass per two iterations, the batch after each pass
+        assert niter % 2 == 0
+        for _ in range(niter // 2):
+            a, b = sets[par.value], sets[1 - par.value]
+            a_ = args(a, b)[:13]   # the descriptors without divV, r_V
+            _lib.call("chmy_stokes_iter2", *a_, counts, comp, conds, flags, *scal, chmy.current_stream())
+            _lib.call("chmy_bc_batchset", C.byref(g.desc()), counts, _arr([b.V[c] for c in comp]), conds,
+                      chmy.current_stream())
+            par.value = 1 - par.value
+    elif mode == "single-flag":   # chmy_stokes_iterate told to keep single iterations
+        fs = flags | ITER_SINGLE
+        _lib.call("chmy_stokes_iterate", *args(sets[0], sets[1]), counts, comp, conds, niter, fs, fs, *scal,
+                  C.byref(par), chmy.current_stream())
+    elif pairs:   # free-slip batches run in pairs by default (ITER_PAIRS is accepted and ignored)
+        fp = flags | (ITER_PAIRS if niter % 2 else 0)
         _lib.call("chmy_stokes_iterate", *args(sets[0], sets[1]), counts, comp, conds, niter, fp, fp, *scal,
                   C.byref(par), chmy.current_stream())
     else:
@@ -175,10 +188,11 @@ GENERAL3 = [[[0.7, -1.3], [0.25, -0.6], [0.1, 0.35]],
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 @pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("visc", ["field", "scalar"])
-@pytest.mark.parametrize("dims", [(61, 13, 17), (20, 14, 35), (5, 4, 3), (2, 2, 2)])
+@pytest.mark.parametrize("dims", [(61, 13, 17), (20, 14, 35), (130, 30, 41), (5, 4, 3), (2, 2, 2)])
 def test_pair_pass_3d_equals_single_iterations(arch, T, exact, visc, dims):
-    """3D: x spans two 60-column tiles, y several 5-row tiles, z several
-    16-plane chunks, down to the smallest grid the pass takes."""
+    """3D: x spans up to three 60-column tiles, y several 9-row tiles, z
+    several chunks (the host picks chunks of 2 to 6 planes at these sizes,
+    37 at 255^3), down to the smallest grid the pass takes."""
     g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=dims)
     for vals in (SHEAR3, GENERAL3):
         for niter in (2, 3):
@@ -206,3 +220,58 @@ def test_pair_pass_2d_with_rho_g(arch, T, exact, forcing):
     a = _run(arch, g, T, exact, "field", forcing, GENERAL, 5, True, 31, rhog=True)
     b = _run(arch, g, T, exact, "field", forcing, GENERAL, 5, False, 31, rhog=True)
     _compare(a, b, f"rho g, forcing {forcing}")
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("forcing", [None, "same"])
+def test_pair_pass_3d_with_rho_g(arch, T, exact, forcing):
+    """rho g at Vz (examples/stokes_3d_inc_ve_T.jl passes rho g_z to
+    update_velocity!); with forcing as well the call keeps single iterations."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=(61, 13, 17))
+    a = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, True, 32, rhog=True)
+    b = _run(arch, g, T, exact, "field", forcing, GENERAL3, 4, False, 32, rhog=True)
+    _compare(a, b, f"3D rho g, forcing {forcing}")
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("visc", ["field", "scalar"])
+def test_iter2_equals_two_single_iterations(arch, nd, T, visc):
+    """chmy_stokes_iter2 + the batch, twice, against four single iterations."""
+    if nd == 2:
+        g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=(70, 45))
+        vals = GENERAL
+    else:
+        g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=(61, 13, 17))
+        vals = GENERAL3
+    for exact in (True, False):
+        a = _run(arch, g, T, exact, visc, None, vals, 4, True, 41, rhog=True, mode="iter2")
+        b = _run(arch, g, T, exact, visc, None, vals, 4, False, 41, rhog=True)
+        _compare(a, b, f"iter2 {nd}D exact {exact}")
+
+
+def test_single_flag_keeps_results(arch):
+    """CHMY_ITER_SINGLE: chmy_stokes_iterate one iteration per launch, same bits."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=(20, 14, 35))
+    a = _run(arch, g, np.float64, False, "field", None, GENERAL3, 4, True, 43, mode="single-flag")
+    b = _run(arch, g, np.float64, False, "field", None, GENERAL3, 4, False, 43)
+    _compare(a, b, "single flag")
+
+
+def test_iter2_rejects_other_batches(arch):
+    """A V batch that is not the free-slip batch is refused on the host."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=(8, 6))
+    sets = [_Set(arch, g, torch.float64), _Set(arch, g, torch.float64)]
+    told = chmy.TensorField(arch, g, torch.float64)
+    D, N, S = _lib.DIRICHLET, _lib.NEUMANN, _lib.VALUE_SCALAR
+    rows = [(0, N, 0.0), (1, N, 0.0), (0, D, 0.0), (1, N, 0.0), (0, N, 0.0), (1, D, 0.0), (0, N, 0.0), (1, D, 0.0)]
+    counts = (C.c_int32 * 6)(2, 2, 2, 2, 0, 0)
+    comp = (C.c_int32 * 8)(*[c for c, _, _ in rows])
+    conds = (_lib.ChmyBC * 8)(*[_lib.ChmyBC(k, S, v, None) for _, k, v in rows])
+    p = pt_params(g)
+    with pytest.raises(_lib.ChmyError, match="free-slip"):
+        _lib.call("chmy_stokes_iter2", C.byref(g.desc()), _arr(list(sets[0].V)), _arr(list(sets[0].tau)),
+                  C.byref(sets[0].Pr.desc()), _arr(list(sets[1].V)), _arr(list(sets[1].tau)),
+                  C.byref(sets[1].Pr.desc()), _arr(list(told)), None, None, None, None, None, counts, comp, conds,
+                  0, 0.8, 0.5, 1.0, 0.37, p["dtau_Pr"], p["dtau_r"], p["nudtau"], chmy.current_stream())
