@@ -414,7 +414,8 @@ def sia_gradient(args):
     rng = np.random.default_rng(args.sia_seed)
     rows = []
     for d in range(args.sia_dirs):
-        v = rng.standard_normal(la0.shape)
+        v = np.zeros(la0.shape)                      # cells only: the gradient covers no halo
+        v[2:-2, 2:-2] = rng.standard_normal((la0.shape[0] - 4, la0.shape[1] - 4))
         h = args.sia_h
         S.set_logA(la0 + h * v)
         Jp = fwd()
