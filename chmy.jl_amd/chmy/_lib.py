@@ -91,6 +91,8 @@ SIGNATURES = {
     "chmy_sia_run": [PG, PF, C.c_int, PF, PF, D, C.c_int, D, D, D, P(C.c_double), VP],
     "chmy_sia_reverse": [PG, PF, C.c_int, C.c_int, C.c_int, PF, PF, PF, PF, PF, PF, D, C.c_int, D, D, D,
                          P(C.c_double), P(C.c_int), VP],
+    "chmy_sia_reverse_noalloc": [PG, PF, C.c_int, C.c_int, C.c_int, PF, PF, PF, PF, D, C.c_int, D, D, D,
+                                 P(C.c_double), VP],
     "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
     "chmy_maxabs": [PF, PB, P(C.c_double), VP],
     "chmy_sumsq": [PF, PB, P(C.c_double), VP],

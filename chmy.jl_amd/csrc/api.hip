@@ -146,7 +146,7 @@ int chmy_init(int device) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(CHMY_E_NODEV, "no HIP device visible");
     if (device < 0 || device >= n) return set_error(CHMY_E_ARG, "device %d not in [0, %d)", device, n);
     CHMY_HIP(hipSetDevice(device));
-    return 0;
+    return sia_scratch_ptr(nullptr);
 }
 
 int chmy_malloc_zeros(void** ptr, size_t bytes, void* stream) {

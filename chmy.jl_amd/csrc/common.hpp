@@ -117,6 +117,9 @@ int check_center(const chmy_grid_t* g, const chmy_field_t* f, const char* name, 
 int check_vector(const chmy_grid_t* g, const chmy_field_t* V, const char* name, int dtype, bool allow_bf16 = false);
 int check_tensor(const chmy_grid_t* g, const chmy_field_t* tau, const char* name, int dtype, bool allow_bf16 = false);
 void tensor_loc(int nd, int c, int* loc);
+// Address on the current device of the SIA reverse sweep's file-scope device
+// scratch (sia.hip), resolved once per device; p may be NULL.
+int sia_scratch_ptr(void** p);
 
 struct Box3 {
     int64_t lo[3];

@@ -21,6 +21,7 @@
 //              halo cells fold into their neighbour (the Neumann copy H_0 = H_1).
 // Each pass is a gather, so no float atomics and the result is deterministic.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -600,6 +601,110 @@ static int sia_check_states(const chmy_grid_t* g, const chmy_field_t* st, int n,
     return 0;
 }
 
+// Validation shared by both reverse-sweep entry points.
+static int sia_reverse_check(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
+                             const chmy_field_t* lam, const chmy_field_t* grad_logA, const chmy_field_t* B,
+                             const chmy_field_t* A, int npow, const double* dts) {
+    if (nsteps < 1 || stride < 1) return set_error(CHMY_E_ARG, "nsteps=%d stride=%d", nsteps, stride);
+    const int nseg = (nsteps + stride - 1) / stride;
+    if (nckpt != nseg) return set_error(CHMY_E_ARG, "nckpt=%d, need ceil(nsteps/stride)=%d", nckpt, nseg);
+    if (!dts) return set_error(CHMY_E_ARG, "dts is NULL");
+    CHMY_TRY(check_grid(g));
+    CHMY_TRY(sia_check_states(g, ckpt, nckpt, "ckpt"));
+    CHMY_TRY(sia_check(g, &ckpt[0], B, A, npow));
+    CHMY_TRY(check_center(g, lam, "lam_a", CHMY_F64));
+    if (A) CHMY_TRY(check_center(g, grad_logA, "grad_logA", CHMY_F64));
+    return 0;
+}
+
+// The reverse sweep on the stream: seg(i) gives segment state buffer i
+// (0 <= i < 2 (stride - 1)) by value.  Returns the slot that holds dJ/dH_0
+// (0 = lam_a, 1 = lam_b).
+template <class SegF>
+static int sia_reverse_sweep(const chmy_grid_t* g, const chmy_field_t* ckpt, int stride, int nsteps, SegF seg,
+                             const chmy_field_t* lam_a, const chmy_field_t* lam_b, const chmy_field_t* grad_logA,
+                             const chmy_field_t* B, const chmy_field_t* A, const SiaPar& p, const double* dts,
+                             hipStream_t st) {
+    const int nseg = (nsteps + stride - 1) / stride;
+    const int64_t nx = g->n[0], ny = g->n[1];
+    // state q of segment s: q = 0 its checkpoint, q >= 1 seg((s % 2)(stride - 1) + q - 1)
+    auto state = [&](int sg, int q) -> chmy_field_t {
+        return q == 0 ? ckpt[sg] : seg((sg % 2) * (stride - 1) + q - 1);
+    };
+    auto len = [&](int sg) { return std::min(stride, nsteps - sg * stride); };
+    // recompute the last segment on its own
+    for (int q = 1; q < len(nseg - 1); ++q) {
+        const chmy_field_t s0 = state(nseg - 1, q - 1), s1 = state(nseg - 1, q);
+        SiaStep<double> f = sia_stepper(g, &s0, &s1, B, A, p, dts[(nseg - 1) * stride + q - 1]);
+        launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
+    }
+    int slot = 0;   // lam_in of the next adjoint step: 0 = lam_a, 1 = lam_b
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int L = len(sg), Lp = sg > 0 ? len(sg - 1) : 0;
+        for (int r = 0; r < std::max(L, Lp - 1); ++r) {
+            // the recompute of segment sg - 1 (state r -> r + 1) rides along with
+            // adjoint step q = L - 1 - r of segment sg
+            const bool fw = sg > 0 && r + 1 < Lp;
+            SiaStep<double> f{};
+            if (fw) {
+                const chmy_field_t s0 = state(sg - 1, r), s1 = state(sg - 1, r + 1);
+                f = sia_stepper(g, &s0, &s1, B, A, p, dts[(sg - 1) * stride + r]);
+            }
+            if (r >= L) {
+                launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
+                continue;
+            }
+            const int q = L - 1 - r;
+            const chmy_field_t* li = slot ? lam_b : lam_a;
+            const chmy_field_t* lo = slot ? lam_a : lam_b;
+            const chmy_field_t sq = state(sg, q);
+            SiaAdj<double> a = sia_adjointer(g, &sq, li, lo, A ? grad_logA : nullptr, B, A, p,
+                                             dts[sg * stride + q]);
+            launch_adj_tile(sia_tiles(nx, ny, fw ? 2 : 1), st, a, f, nx, ny);
+            slot ^= 1;
+        }
+    }
+    return slot;
+}
+
+// Device scratch of chmy_sia_reverse_noalloc: the segment states and the
+// second adjoint buffer in one file-scope array, which each GPU holds until
+// the process ends.  48 MiB covers 512 x 512 cells (516 x 516 Center storage,
+// 2.13 MB a field) at 10 steps per checkpoint: 2 x 9 segment states + 1.
+// In host code `scratch` is not a device address: its address on a device
+// comes from hipGetSymbolAddress with that device current.
+constexpr size_t kSiaScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) double scratch[kSiaScratchBytes / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[64] = {nullptr};   // scratch's address on each device
+
+__global__ void __launch_bounds__(256) sia_fill_zero(double* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+}  // namespace chmy
+
+// Resolves scratch's address on the current device (chmy_init calls it; the
+// no-allocation entry point calls it on its first use on a device).
+int chmy::sia_scratch_ptr(void** p) {
+    int dev = 0;
+    CHMY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(CHMY_E_ARG, "device index %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q)
+            return set_error(CHMY_E_NODEV, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = q;
+    }
+    if (p) *p = g_scratch_dev[dev];
+    return 0;
+}
+
+namespace chmy {
+
 static dim3 sia_blk() { return dim3(64, 4, 1); }
 static dim3 sia_grid(int64_t nx, int64_t ny) {
     return dim3((unsigned)((nx + 1 + 63) / 64), (unsigned)((ny + 1 + 3) / 4), 1);
@@ -725,56 +830,51 @@ int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, 
                      const chmy_field_t* seg, const chmy_field_t* lam_a, const chmy_field_t* lam_b,
                      const chmy_field_t* grad_logA, const chmy_field_t* B, const chmy_field_t* A, double As, int npow,
                      double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot, void* stream) {
-    if (nsteps < 1 || stride < 1) return set_error(CHMY_E_ARG, "nsteps=%d stride=%d", nsteps, stride);
-    const int nseg = (nsteps + stride - 1) / stride;
-    if (nckpt != nseg) return set_error(CHMY_E_ARG, "nckpt=%d, need ceil(nsteps/stride)=%d", nckpt, nseg);
-    if (!dts || !lam_slot) return set_error(CHMY_E_ARG, "dts or lam_slot is NULL");
-    CHMY_TRY(check_grid(g));
-    CHMY_TRY(sia_check_states(g, ckpt, nckpt, "ckpt"));
-    CHMY_TRY(sia_check(g, &ckpt[0], B, A, npow));
+    CHMY_TRY(sia_reverse_check(g, ckpt, nckpt, stride, nsteps, lam_a, grad_logA, B, A, npow, dts));
+    if (!lam_slot) return set_error(CHMY_E_ARG, "lam_slot is NULL");
     if (stride > 1) CHMY_TRY(sia_check_states(g, seg, 2 * (stride - 1), "seg"));
-    CHMY_TRY(check_center(g, lam_a, "lam_a", CHMY_F64));
     CHMY_TRY(check_center(g, lam_b, "lam_b", CHMY_F64));
     if (lam_a->data == lam_b->data) return set_error(CHMY_E_ARG, "lam_a and lam_b must differ");
-    if (A) CHMY_TRY(check_center(g, grad_logA, "grad_logA", CHMY_F64));
     const SiaPar p{As, npow, mb_a, mb_b, mb_c, 0.0};
-    hipStream_t st = (hipStream_t)stream;
-    const int64_t nx = g->n[0], ny = g->n[1];
-    // state q of segment s: q = 0 its checkpoint, q >= 1 seg[(s % 2)(stride - 1) + q - 1]
-    auto state = [&](int sg, int q) -> const chmy_field_t* {
-        return q == 0 ? &ckpt[sg] : &seg[(sg % 2) * (stride - 1) + q - 1];
-    };
-    auto len = [&](int sg) { return std::min(stride, nsteps - sg * stride); };
-    // recompute the last segment on its own
-    for (int q = 1; q < len(nseg - 1); ++q) {
-        SiaStep<double> f = sia_stepper(g, state(nseg - 1, q - 1), state(nseg - 1, q), B, A, p,
-                                        dts[(nseg - 1) * stride + q - 1]);
-        launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
-    }
-    int slot = 0;   // lam_in of the next adjoint step: 0 = lam_a, 1 = lam_b
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int L = len(sg), Lp = sg > 0 ? len(sg - 1) : 0;
-        for (int r = 0; r < std::max(L, Lp - 1); ++r) {
-            // the recompute of segment sg - 1 (state r -> r + 1) rides along with
-            // adjoint step q = L - 1 - r of segment sg
-            const bool fw = sg > 0 && r + 1 < Lp;
-            SiaStep<double> f{};
-            if (fw) f = sia_stepper(g, state(sg - 1, r), state(sg - 1, r + 1), B, A, p, dts[(sg - 1) * stride + r]);
-            if (r >= L) {
-                launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
-                continue;
-            }
-            const int q = L - 1 - r;
-            const chmy_field_t* li = slot ? lam_b : lam_a;
-            const chmy_field_t* lo = slot ? lam_a : lam_b;
-            SiaAdj<double> a = sia_adjointer(g, state(sg, q), li, lo, A ? grad_logA : nullptr, B, A, p,
-                                             dts[sg * stride + q]);
-            launch_adj_tile(sia_tiles(nx, ny, fw ? 2 : 1), st, a, f, nx, ny);
-            slot ^= 1;
-        }
-    }
+    *lam_slot = sia_reverse_sweep(g, ckpt, stride, nsteps, [&](int i) { return seg[i]; }, lam_a, lam_b, grad_logA,
+                                  B, A, p, dts, (hipStream_t)stream);
     CHMY_HIP(hipGetLastError());
-    *lam_slot = slot;
+    return 0;
+}
+
+int chmy_sia_reverse_noalloc(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
+                             const chmy_field_t* lam, const chmy_field_t* grad_logA, const chmy_field_t* B,
+                             const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
+                             const double* dts, void* stream) {
+    CHMY_TRY(sia_reverse_check(g, ckpt, nckpt, stride, nsteps, lam, grad_logA, B, A, npow, dts));
+    // 2 (stride - 1) segment states and the second adjoint buffer, one Center
+    // storage each, every one starting on a 256 B boundary
+    const int nbuf = 2 * (stride - 1) + 1;
+    const size_t fbytes = (size_t)(pad_of(*lam, 0) * pad_of(*lam, 1)) * sizeof(double);
+    const size_t fstride = (fbytes + 255) / 256 * 256;
+    if ((size_t)nbuf > kSiaScratchBytes / fstride)
+        return set_error(CHMY_E_SIZE,
+                         "chmy_sia_reverse_noalloc: %d Center fields of %zu bytes (stride %d) exceed its %zu-byte "
+                         "device scratch; use chmy_sia_reverse with caller-allocated seg and lam_b",
+                         nbuf, fstride, stride, kSiaScratchBytes);
+    char* base = nullptr;
+    CHMY_TRY(sia_scratch_ptr(reinterpret_cast<void**>(&base)));
+    hipStream_t st = (hipStream_t)stream;
+    // the caller-allocated workspace of chmy_sia_reverse starts at zero; so
+    // does this one, whatever the previous call on this device left in it
+    const int64_t nz = (int64_t)(nbuf * fstride / sizeof(double));
+    hipLaunchKernelGGL(sia_fill_zero, dim3((unsigned)std::min<int64_t>((nz + 255) / 256, 2048)), dim3(256), 0, st,
+                       reinterpret_cast<double*>(base), nz);
+    auto buf = [&](int i) {
+        chmy_field_t f = *lam;
+        f.data = base + (size_t)i * fstride;
+        return f;
+    };
+    const chmy_field_t lam_b = buf(nbuf - 1);
+    const SiaPar p{As, npow, mb_a, mb_b, mb_c, 0.0};
+    const int slot = sia_reverse_sweep(g, ckpt, stride, nsteps, buf, lam, &lam_b, grad_logA, B, A, p, dts, st);
+    if (slot == 1) CHMY_HIP(hipMemcpyAsync(lam->data, lam_b.data, fbytes, hipMemcpyDeviceToDevice, st));
+    CHMY_HIP(hipGetLastError());
     return 0;
 }
 

@@ -357,6 +357,22 @@ int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckp
                      const chmy_field_t* grad_logA, const chmy_field_t* B, const chmy_field_t* A, double As,
                      int npow, double mb_a, double mb_b, double mb_c, const double* dts, int* lam_slot,
                      void* stream);
+/* chmy_sia_reverse without workspace arguments, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * segment states and the second adjoint buffer live in the library's
+ * file-scope device scratch of 48 MiB (one per GPU, held until the process
+ * ends): 2 (stride - 1) + 1 Center storages, each rounded up to 256 B, must
+ * fit, else CHMY_E_SIZE and nothing runs (512 x 512 cells: stride <= 12).
+ * The call allocates nothing, clears the part of the scratch it uses, and
+ * leaves dJ/dH_0 in lam (which holds dJ/dH_N on entry); grad_logA as in
+ * chmy_sia_reverse.  Results are bitwise those of chmy_sia_reverse with
+ * zero-filled seg and lam_b.  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams or host threads) race. */
+int chmy_sia_reverse_noalloc(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
+                             const chmy_field_t* lam, const chmy_field_t* grad_logA, const chmy_field_t* B,
+                             const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
+                             const double* dts, void* stream);
 
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:172-175),

@@ -138,6 +138,19 @@ def test_adjoint_entry_points_validate_on_host():
     rc = L.chmy_sia_reverse(C.byref(gd), ck, 2, 2, 4, seg, C.byref(la), C.byref(la), None, C.byref(B), None, 0.0,
                             3, 0.0, 0.0, 0.0, dts, C.byref(slot), None)
     assert rc == -1 and b"lam_a and lam_b" in L.chmy_last_error()
+    # the no-allocation variant: same validation, and a workspace above its
+    # 48 MiB device scratch is refused before any HIP call
+    rc = L.chmy_sia_reverse_noalloc(C.byref(gd), ck, 2, 2, 5, C.byref(la), None, C.byref(B), None, 0.0, 3, 0.0,
+                                    0.0, 0.0, dts, None)
+    assert rc == -1 and b"nckpt" in L.chmy_last_error()
+    gbig = chmy.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 1.0), dims=(1024, 1024)).desc()
+    Cb = lambda data: fdesc((0, 0), (1024, 1024), data)
+    ckb = (_lib.ChmyField * 30)(*[Cb(0x10000 * (q + 1)) for q in range(30)])
+    Bb, lb_ = Cb(0x400000), Cb(0x500000)
+    dtb = (C.c_double * 300)(*([1.0] * 300))
+    rc = L.chmy_sia_reverse_noalloc(C.byref(gbig), ckb, 30, 10, 300, C.byref(lb_), None, C.byref(Bb), None, 0.0,
+                                    3, 0.0, 0.0, 0.0, dtb, None)
+    assert rc == -7 and b"scratch" in L.chmy_last_error()
     # Stokes gradient pass: tau_a with a wrong location
     tl = [(0, 0), (0, 0), (1, 1)]
     tn = lambda l: (8 + l[0], 6 + l[1])
