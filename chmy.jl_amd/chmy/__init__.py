@@ -2,7 +2,7 @@
 
 Mirrors the reference's exported names (src/Chmy.jl:91-99) for the hot path:
 Arch / UniformGrid / Field / VectorField / TensorField / Launcher / bc! /
-batch / Dirichlet / Neumann / set! / interior, with Julia's `!` suffix
+batch / Dirichlet / Neumann / set! / FunctionField / interior, with Julia's `!` suffix
 written as a trailing underscore (bc_, set_).  Kernels are the HIP
 counterparts of the reference's example kernels (chmy.kernels); every call
 goes through libchmy_mi355x.so's C ABI.
@@ -15,6 +15,7 @@ from .boundary_conditions import (BatchSet, BoundaryFunction, Dirichlet, EmptyBa
                                   Neumann, batch, bc_)
 from .fields import (Field, TensorField, VectorField, halo, interior, location, set_, set_storage,
                      tensor_locations, vector_location)
+from .function_field import FunctionField, TraceError, ifelse
 from .grids import Center, UniformGrid, Vertex, coord, flip, inv_spacing, spacing
 from .kernel_launch import Launcher, Offset
 
@@ -24,5 +25,5 @@ __all__ = [
     "FieldBatch", "FirstOrderBC", "Neumann", "batch", "bc_", "Field", "TensorField", "VectorField", "halo",
     "interior", "location", "set_", "set_storage", "tensor_locations", "vector_location", "Center",
     "UniformGrid", "Vertex", "coord", "flip", "inv_spacing", "spacing", "Launcher", "Offset", "kernels",
-    "ChmyError",
+    "ChmyError", "FunctionField", "TraceError", "ifelse",
 ]

@@ -33,6 +33,19 @@ class ChmyBox(C.Structure):
     _fields_ = [("lo", C.c_int64 * 3), ("len", C.c_int64 * 3)]
 
 
+class ChmyPop(C.Structure):
+    """One point-program operation (chmy_pop_t)."""
+    _fields_ = [("op", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("c", C.c_int32)]
+
+
+# point-program operation codes (chmy_set_program)
+(P_CONST, P_INDEX, P_COORD, P_SPACING, P_INV_SPACING, P_LOAD) = 0, 1, 2, 3, 4, 5
+(P_NEG, P_ABS, P_SQRT, P_EXP, P_LOG, P_SIN, P_COS, P_TANH, P_FLOOR, P_NOT) = range(8, 18)
+(P_ADD, P_SUB, P_MUL, P_DIV, P_POW, P_MIN, P_MAX, P_LT, P_LE, P_GT, P_GE, P_EQ, P_NE, P_AND,
+ P_OR) = range(24, 39)
+P_POWI, P_SELECT, P_FMA = 40, 48, 49
+
+
 class ChmyBC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("vkind", C.c_int32), ("value", C.c_double),
                 ("vfield", C.POINTER(ChmyField))]
@@ -93,6 +106,7 @@ SIGNATURES = {
                          P(C.c_double), P(C.c_int), VP],
     "chmy_sia_reverse_noalloc": [PG, PF, C.c_int, C.c_int, C.c_int, PF, PF, PF, PF, D, C.c_int, D, D, D,
                                  P(C.c_double), VP],
+    "chmy_set_program": [PG, PF, PB, P(ChmyPop), C.c_int, P(C.c_double), C.c_int, PF, C.c_int, VP],
     "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
     "chmy_maxabs": [PF, PB, P(C.c_double), VP],
     "chmy_sumsq": [PF, PB, P(C.c_double), VP],

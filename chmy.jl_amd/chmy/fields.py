@@ -101,24 +101,43 @@ def halo(f):
 def set_(f, value, grid=None, *, parameters=(), discrete=False):
     """set!(f, val | array | other | (grid, fun)) (field.jl:87-142) on the interior.
 
-    A function is evaluated on the host with the grid's fma coordinates
-    (continuous: fun(x, y, z, *parameters)) or logical indices (discrete:
-    fun(grid, loc, ix, iy, iz, *parameters)), vectorised over numpy arrays,
-    and uploaded.  A VectorField/TensorField sets every component (field.jl:144-145)."""
+    A function (continuous: fun(x, y, z, *parameters) at the grid's fma
+    coordinates; discrete: fun(grid, loc, ix, iy, iz, *parameters) at logical
+    indices) is traced into a point program and evaluated on the GPU, as
+    the reference's _set_continuous!/_set_discrete! kernels do; a
+    FunctionField likewise (set!(f, other::AbstractField)).  On a host field
+    (no GPU) the function is evaluated with numpy.  A VectorField/TensorField
+    sets every component (field.jl:144-145)."""
+    from .function_field import FunctionField, _params, trace
     if isinstance(f, tuple):
         for c in f:
             set_(c, value, grid, parameters=parameters, discrete=discrete)
         return
     it = f.interior()
+    loc = f.loc
+    if isinstance(value, FunctionField):
+        # interior(other) into interior(f), index by index (field.jl:109-114):
+        # the function at its own location
+        if tuple(value.dims) != tuple(f.dims):
+            raise ValueError(f"set_: FunctionField of size {value.dims} into a field of size {f.dims}")
+        if f.data.is_cuda:
+            value.program().run(value.grid, f)
+            return
+        parameters, discrete = value.parameters, value.discrete
+        value, grid, loc = value.func, value.grid, value.loc
     if callable(value):
         if grid is None:
             raise ValueError("set_ with a function needs the grid")
+        if f.data.is_cuda:
+            trace(value, grid, f.loc, discrete=discrete, parameters=parameters).run(grid, f)
+            return
+        params = _params(parameters)
         if discrete:
             idx = np.meshgrid(*[np.arange(1, n + 1) for n in f.dims], indexing="ij")
-            vals = value(grid, f.loc, *idx, *parameters)
+            vals = value(grid, loc, *idx, *params)
         else:
-            xs = np.meshgrid(*[grid.coords(f.loc, d) for d in range(f.nd)], indexing="ij")
-            vals = value(*xs, *parameters)
+            xs = np.meshgrid(*[grid.coords(loc, d) for d in range(f.nd)], indexing="ij")
+            vals = value(*xs, *params)
         vals = np.array(np.broadcast_to(np.asarray(vals, dtype=np.float64), f.dims))
         it.copy_(torch.from_numpy(vals).to(it.dtype))
     elif isinstance(value, Field):

@@ -20,6 +20,7 @@ import ctypes as C
 from . import _lib
 from .architectures import current_stream
 from .fields import Field
+from .function_field import FunctionField
 
 
 def _box(lo, ws):
@@ -36,6 +37,10 @@ def _arr(fields):
 
 
 def _opt(f):
+    """An optional field argument; a FunctionField (the reference passes ρg as
+    one, examples/stokes_2d_inc_ve_T.jl:107,146) is evaluated into a field."""
+    if isinstance(f, FunctionField):
+        f = f.materialize()
     return C.byref(f.desc()) if f is not None else None
 
 

@@ -23,6 +23,7 @@ from . import _lib
 from .architectures import current_stream
 from .boundary_conditions import Neumann, batch
 from .fields import Field, VectorField, set_storage
+from .function_field import FunctionField
 from .grids import Center, Vertex, UniformGrid, spacing
 from .kernels import _arr, maxabs
 
@@ -37,7 +38,12 @@ class SIA:
         self.dt_max = float(dt_max)
         f = lambda loc=Center(): Field(arch, grid, loc, torch.float64)
         self.H, self.H2, self.B, self.D = f(), f(), f(), f()
-        set_storage(self.B, B)
+        if isinstance(B, FunctionField):
+            # the bed as a FunctionField (examples/sia.jl:30-36), evaluated on
+            # the device over the whole storage
+            B.materialize(arch, out=self.B)
+        else:
+            set_storage(self.B, B)
         self.A = None
         if logA is not None:
             self.A = f()

@@ -24,6 +24,7 @@ from . import _lib
 from .architectures import current_stream
 from .boundary_conditions import Dirichlet, Neumann, batch
 from .fields import Field, TensorField, VectorField, set_storage
+from .function_field import as_field
 from .grids import Center, UniformGrid, spacing
 from .kernels import _arr, _box, maxloc, HipKernel
 from .kernel_launch import Launcher
@@ -67,7 +68,9 @@ class PTStokes:
         self.tau_old = TensorField(arch, grid, dtype)
         self.divV = Field(arch, grid, Center(), dtype)
         self.r_V = VectorField(arch, grid, dtype)
-        self.rhog = rhog
+        # a FunctionField ρg (examples/stokes_2d_inc_ve_T.jl:107) is evaluated
+        # on the device into a field of the solver's dtype
+        self.rhog = as_field(rhog, arch, dtype)
         self.f_V = None          # defect-correction forcing (set by MixedPTStokes)
         self.f_P = None
         self.G = float(G)

@@ -374,6 +374,48 @@ int chmy_sia_reverse_noalloc(const chmy_grid_t* grid, const chmy_field_t* ckpt, 
                              const chmy_field_t* A, double As, int npow, double mb_a, double mb_b, double mb_c,
                              const double* dts, void* stream);
 
+/* ---------------------------------------------------------------- set! / FunctionField */
+/* Device evaluation of set!(f, grid, fun; discrete, parameters) and of
+ * FunctionField values (src/Fields/field.jl:119-142,
+ * src/Fields/function_field.jl:51-61).  The function is given as a point
+ * program: nprog operations in SSA form, operation q writing register q from
+ * registers < q; the last register is stored at each point of the box
+ * (logical indices of dst; NULL = its interior 1..n, as set! does; any box
+ * inside its storage, e.g. the whole storage for a FunctionField read in the
+ * halo), converted to dst's dtype.  Registers are f64; each operation is one
+ * IEEE operation (no contraction).  Operands:
+ *   CONST a            consts[a]
+ *   INDEX a, b         logical index along dim a, plus b
+ *   COORD a, b, c      coord(grid, loc b, index + c) along dim a: vertex
+ *                      fma(i-1, Δ, o), centre fma(i-1, Δ, fma(0.5, Δ, o))
+ *                      (uniform_axis.jl:18-19)
+ *   SPACING a, INV_SPACING a   Δ and 1/Δ along dim a
+ *   LOAD a, b          srcs[a] at the point's logical index plus the offsets
+ *                      packed in b (one signed byte per dim, x lowest); every
+ *                      point of the box plus the offsets must lie in the
+ *                      source's storage (else CHMY_E_BOX)
+ *   unary a            NEG ABS SQRT EXP LOG SIN COS TANH FLOOR NOT
+ *   binary a, b        ADD SUB MUL DIV POW MIN MAX (Julia's min/max: NaN wins,
+ *                      -0.0 < 0.0) LT LE GT GE EQ NE AND OR (results 1.0/0.0)
+ *   POWI a, b          r[a]^b by repeated products left to right (Julia's
+ *                      literal powers; b < 0: the same on 1/r[a]), |b| <= 64
+ *   SELECT a, b, c     ifelse(r[a] != 0, r[b], r[c]);  FMA a, b, c  fma(r[a], r[b], r[c])
+ * Limits: 128 operations, 64 constants, 8 source fields. */
+typedef struct { int32_t op, a, b, c; } chmy_pop_t;
+enum {
+  CHMY_P_CONST = 0, CHMY_P_INDEX = 1, CHMY_P_COORD = 2, CHMY_P_SPACING = 3, CHMY_P_INV_SPACING = 4,
+  CHMY_P_LOAD = 5,
+  CHMY_P_NEG = 8, CHMY_P_ABS = 9, CHMY_P_SQRT = 10, CHMY_P_EXP = 11, CHMY_P_LOG = 12, CHMY_P_SIN = 13,
+  CHMY_P_COS = 14, CHMY_P_TANH = 15, CHMY_P_FLOOR = 16, CHMY_P_NOT = 17,
+  CHMY_P_ADD = 24, CHMY_P_SUB = 25, CHMY_P_MUL = 26, CHMY_P_DIV = 27, CHMY_P_POW = 28, CHMY_P_MIN = 29,
+  CHMY_P_MAX = 30, CHMY_P_LT = 31, CHMY_P_LE = 32, CHMY_P_GT = 33, CHMY_P_GE = 34, CHMY_P_EQ = 35,
+  CHMY_P_NE = 36, CHMY_P_AND = 37, CHMY_P_OR = 38,
+  CHMY_P_POWI = 40, CHMY_P_SELECT = 48, CHMY_P_FMA = 49
+};
+int chmy_set_program(const chmy_grid_t* grid, const chmy_field_t* dst, const chmy_box_t* box,
+                     const chmy_pop_t* prog, int nprog, const double* consts, int nconst,
+                     const chmy_field_t* srcs, int nsrc, void* stream);
+
 /* ---------------------------------------------------------------- reductions */
 /* maximum(abs.(view)) over a box of a field (stokes_3d_inc_ve_T.jl:172-175),
  * exact (order-independent); result in f64 written to *out_dev (device).
