@@ -15,6 +15,7 @@ from .boundary_conditions import (BatchSet, BoundaryFunction, Dirichlet, EmptyBa
                                   Neumann, batch, bc_)
 from .fields import (Field, TensorField, VectorField, halo, interior, location, set_, set_storage,
                      tensor_locations, vector_location)
+from .field_mask import FieldMask
 from .function_field import FunctionField, TraceError, ifelse
 from .grids import Center, UniformGrid, Vertex, coord, flip, inv_spacing, spacing
 from .kernel_launch import Launcher, Offset
@@ -25,5 +26,5 @@ __all__ = [
     "FieldBatch", "FirstOrderBC", "Neumann", "batch", "bc_", "Field", "TensorField", "VectorField", "halo",
     "interior", "location", "set_", "set_storage", "tensor_locations", "vector_location", "Center",
     "UniformGrid", "Vertex", "coord", "flip", "inv_spacing", "spacing", "Launcher", "Offset", "kernels",
-    "ChmyError", "FunctionField", "TraceError", "ifelse",
+    "ChmyError", "FunctionField", "TraceError", "ifelse", "FieldMask",
 ]

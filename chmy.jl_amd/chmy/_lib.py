@@ -16,6 +16,7 @@ CENTER, VERTEX = 0, 1
 DIRICHLET, NEUMANN = 0, 1
 VALUE_ZERO, VALUE_SCALAR, VALUE_FIELD = 0, 1, 2
 OP_PARTIAL, OP_LERP, OP_DIVG, OP_VMAG, OP_LEFT, OP_RIGHT = 0, 1, 2, 3, 8, 9
+OP_D2, OP_DKD, OP_LAPL, OP_DIVG_GRAD, OP_HLERP, OP_DELTA = 4, 5, 6, 7, 10, 11
 
 
 class ChmyField(C.Structure):
@@ -106,6 +107,7 @@ SIGNATURES = {
                          P(C.c_double), P(C.c_int), VP],
     "chmy_sia_reverse_noalloc": [PG, PF, C.c_int, C.c_int, C.c_int, PF, PF, PF, PF, D, C.c_int, D, D, D,
                                  P(C.c_double), VP],
+    "chmy_op_masked": [C.c_int, PG, PB, PF, C.c_int, PF, PF, PF, C.c_int, VP],
     "chmy_set_program": [PG, PF, PB, P(ChmyPop), C.c_int, P(C.c_double), C.c_int, PF, C.c_int, VP],
     "chmy_maxabs_dev": [PF, PB, P(C.c_double), VP],
     "chmy_maxabs": [PF, PB, P(C.c_double), VP],

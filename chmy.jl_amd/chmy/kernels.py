@@ -12,6 +12,12 @@ the same argument list the reference kernel takes, and it calls the C ABI.
   thermal_step                     the last two in one launch, double-buffered T (extension)
   partial, lerp, divg, vmag, left,
   right                            GridOperators as used in test/test_grid_operators.jl
+  delta, hlerp, partial2,
+  partial_k_partial, lapl,
+  divg_grad                        δ, hlerp, ∂², ∂k∂, lapl, divg_grad (partial_derivatives.jl,
+                                   field_operators.jl, interpolation.jl); every operator but
+                                   lerp/hlerp/vmag also masked, given a FieldMask
+                                   (masked_operators.jl, masked_field_operators.jl)
 """
 from __future__ import annotations
 
@@ -19,6 +25,7 @@ import ctypes as C
 
 from . import _lib
 from .architectures import current_stream
+from .field_mask import FieldMask
 from .fields import Field
 from .function_field import FunctionField
 
@@ -118,13 +125,25 @@ def _maxloc(grid, box, dst, src):
               current_stream())
 
 
-def _op(code):
+_PLAIN = {_lib.OP_PARTIAL, _lib.OP_LERP, _lib.OP_DIVG, _lib.OP_VMAG, _lib.OP_LEFT, _lib.OP_RIGHT}
+
+
+def _op(code, has_k=False):
     def fn(grid, box, dst, src, *rest):
-        # rest = (dim, grid) for ∂/left/right, (grid,) otherwise
+        # rest = ([k,] [ω,] dim, grid) for the per-dim operators, ([k,] [ω,] grid) otherwise:
+        # the argument order of ∂x(f, ω, g, I), ∂k∂(f, k, ω, g, dim, I), divg(V, ω, g, I)
         srcs = list(src) if isinstance(src, (tuple, list)) else [src]
+        rest = list(rest)
+        k = rest.pop(0) if has_k else None
+        w = rest.pop(0) if rest and isinstance(rest[0], FieldMask) else None
         dim = int(rest[0]) if len(rest) == 2 else 0
-        _lib.call("chmy_op", code, C.byref(grid.desc()), C.byref(box), C.byref(dst.desc()), len(srcs), _arr(srcs),
-                  dim, current_stream())
+        if w is None and k is None and code in _PLAIN:
+            _lib.call("chmy_op", code, C.byref(grid.desc()), C.byref(box), C.byref(dst.desc()), len(srcs),
+                      _arr(srcs), dim, current_stream())
+            return
+        masks = _arr([w.at(s.loc) for s in srcs]) if w is not None else None
+        _lib.call("chmy_op_masked", code, C.byref(grid.desc()), C.byref(box), C.byref(dst.desc()), len(srcs),
+                  _arr(srcs), masks, C.byref(k.desc()) if k is not None else None, dim, current_stream())
     return fn
 
 
@@ -140,13 +159,20 @@ update_thermal_flux = HipKernel("update_thermal_flux!", _update_thermal_flux)
 update_thermal = HipKernel("update_thermal!", _update_thermal)
 thermal_step = HipKernel("update_thermal_flux! + update_thermal!", _thermal_step)
 maxloc = HipKernel("maxloc", _maxloc)
-# generic operators: (dst, src[, dim], grid)
+# generic operators: (dst, src[, k][, ω][, dim], grid); ω a FieldMask gives the
+# masked operator (masked_operators.jl, masked_field_operators.jl)
 partial = HipKernel("∂", _op(_lib.OP_PARTIAL))
+delta = HipKernel("δ", _op(_lib.OP_DELTA))
 lerp = HipKernel("lerp", _op(_lib.OP_LERP))
+hlerp = HipKernel("hlerp", _op(_lib.OP_HLERP))
 divg = HipKernel("divg", _op(_lib.OP_DIVG))
 vmag = HipKernel("vmag", _op(_lib.OP_VMAG))
 left = HipKernel("left", _op(_lib.OP_LEFT))
 right = HipKernel("right", _op(_lib.OP_RIGHT))
+partial2 = HipKernel("∂²", _op(_lib.OP_D2))
+partial_k_partial = HipKernel("∂k∂", _op(_lib.OP_DKD, has_k=True))
+lapl = HipKernel("lapl", _op(_lib.OP_LAPL))
+divg_grad = HipKernel("divg_grad", _op(_lib.OP_DIVG_GRAD, has_k=True))
 
 
 def maxabs(f, box=None):

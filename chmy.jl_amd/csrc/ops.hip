@@ -99,6 +99,121 @@ __global__ void __launch_bounds__(256) op_kernel(Box3 b, DFL<T> dst, DFL<T> s0, 
     g_at(dst, I) = v;
 }
 
+// ------------------------------------------------------------------ masked and second-order operators
+// Masked forms (masked_operators.jl:11-39): left/right multiply the knot
+// value by the mask at the field's own location at that knot,
+// left(f, ω)[I] = f[Il]·ω[Il]; with M = false the plain forms, bitwise those
+// of leftv/rightv/ddv.  ∂² and ∂k∂ (partial_derivatives.jl:7-21) take
+// differences of ∂ at the two knots of the flipped location; ∂k∂ weights
+// them with lerp(k) at those knots (the coefficient is never masked).
+template <class T, bool M>
+__device__ __forceinline__ T knot(const DFL<T>& f, const DFL<T>& w, Ix3 J) {
+    const T v = g_at(f, J);
+    return M ? v * g_at(w, J) : v;
+}
+template <class T, bool M>
+__device__ __forceinline__ T leftk(const DFL<T>& f, const DFL<T>& w, int D, Ix3 I) {
+    return knot<T, M>(f, w, f.loc[D] == 0 ? sh(I, D, -1) : I);
+}
+template <class T, bool M>
+__device__ __forceinline__ T rightk(const DFL<T>& f, const DFL<T>& w, int D, Ix3 I) {
+    return knot<T, M>(f, w, f.loc[D] == 0 ? I : sh(I, D, 1));
+}
+template <class T, bool M>
+__device__ __forceinline__ T ddk(const DFL<T>& f, const DFL<T>& w, T isp, int D, Ix3 I) {
+    return (rightk<T, M>(f, w, D, I) - leftk<T, M>(f, w, D, I)) * isp;
+}
+// knots of the flipped location around I: Center f -> (I, I+e), Vertex f -> (I-e, I)
+__device__ __forceinline__ void flip_knots(int loc, int D, Ix3 I, Ix3& Il, Ix3& Ir) {
+    Il = loc == 0 ? I : sh(I, D, -1);
+    Ir = loc == 0 ? sh(I, D, 1) : I;
+}
+template <class T, bool M>
+__device__ T d2k(const DFL<T>& f, const DFL<T>& w, T isp, int D, Ix3 I) {
+    Ix3 Il, Ir;
+    flip_knots(f.loc[D], D, I, Il, Ir);
+    return (ddk<T, M>(f, w, isp, D, Ir) - ddk<T, M>(f, w, isp, D, Il)) * isp;
+}
+template <class T, bool M>
+__device__ T dkdk(const DFL<T>& f, const DFL<T>& w, const DFL<T>& k, T isp, int D, Ix3 I) {
+    Ix3 Il, Ir;
+    flip_knots(f.loc[D], D, I, Il, Ir);
+    int32_t floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
+    floc[D] = 1 - floc[D];
+    return (lerpv(k, floc, Ir) * ddk<T, M>(f, w, isp, D, Ir) - lerpv(k, floc, Il) * ddk<T, M>(f, w, isp, D, Il)) *
+           isp;
+}
+// hlerp (interpolation.jl:15,101): the lerp knots reduced with
+// inv(muladd(0.5, inv(b) - inv(a), inv(a))), first differing dim innermost
+template <class T>
+__device__ T hlerpv(const DFL<T>& f, const int32_t* to, Ix3 I) {
+    int dims[3], m = 0;
+    for (int d = 0; d < f.nd; ++d)
+        if (f.loc[d] != to[d]) dims[m++] = d;
+    T v[8];
+    for (int q = 0; q < (1 << m); ++q) {
+        Ix3 J = I;
+        for (int b = 0; b < m; ++b) {
+            int D = dims[b];
+            bool right = (q >> b) & 1;
+            int64_t s = f.loc[D] == 0 ? (right ? 0 : -1) : (right ? 1 : 0);
+            J = sh(J, D, s);
+        }
+        v[q] = g_at(f, J);
+    }
+    for (int b = 0; b < m; ++b) {
+        int half = 1 << (m - b - 1);
+        for (int q = 0; q < half; ++q) {
+            T a = v[2 * q], c = v[2 * q + 1];
+            v[q] = T(1) / fma(T(0.5), T(1) / c - T(1) / a, T(1) / a);
+        }
+    }
+    return v[0];
+}
+
+template <class T, bool M>
+__global__ void __launch_bounds__(256) op_ext_kernel(Box3 b, DFL<T> dst, DFL<T> s0, DFL<T> s1, DFL<T> s2, DFL<T> w0,
+                                                     DFL<T> w1, DFL<T> w2, DFL<T> k, OpArgs a, T isp0, T isp1,
+                                                     T isp2) {
+    bool ok;
+    Ix3 I = box_index(b, ok);
+    if (!ok) return;
+    const DFL<T>* s[3] = {&s0, &s1, &s2};
+    const DFL<T>* w[3] = {&w0, &w1, &w2};
+    const T isp[3] = {isp0, isp1, isp2};
+    T v = T(0);
+    switch (a.op) {
+        case CHMY_OP_PARTIAL: v = ddk<T, M>(s0, w0, isp[a.D], a.D, I); break;
+        case CHMY_OP_DELTA: v = rightk<T, M>(s0, w0, a.D, I) - leftk<T, M>(s0, w0, a.D, I); break;
+        case CHMY_OP_LEFT: v = leftk<T, M>(s0, w0, a.D, I); break;
+        case CHMY_OP_RIGHT: v = rightk<T, M>(s0, w0, a.D, I); break;
+        case CHMY_OP_DIVG: {
+            v = ddk<T, M>(*s[0], *w[0], isp[0], 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + ddk<T, M>(*s[d], *w[d], isp[d], d, I);
+        } break;
+        case CHMY_OP_D2: v = d2k<T, M>(s0, w0, isp[a.D], a.D, I); break;
+        case CHMY_OP_DKD: v = dkdk<T, M>(s0, w0, k, isp[a.D], a.D, I); break;
+        case CHMY_OP_LAPL: {
+            v = d2k<T, M>(s0, w0, isp[0], 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + d2k<T, M>(s0, w0, isp[d], d, I);
+        } break;
+        case CHMY_OP_DIVG_GRAD: {
+            v = dkdk<T, M>(s0, w0, k, isp[0], 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + dkdk<T, M>(s0, w0, k, isp[d], d, I);
+        } break;
+        case CHMY_OP_LERP: v = lerpv(s0, dst.loc, I); break;
+        case CHMY_OP_HLERP: v = hlerpv(s0, dst.loc, I); break;
+        case CHMY_OP_VMAG: {
+            const int32_t c[3] = {0, 0, 0};
+            T x = lerpv(*s[0], c, I);
+            T acc = x * x;
+            for (int d = 1; d < a.nd; ++d) { T y = lerpv(*s[d], c, I); acc = acc + y * y; }
+            v = sqrt(acc);
+        } break;
+    }
+    g_at(dst, I) = v;
+}
+
 // ------------------------------------------------------------------ diffusion
 template <class T>
 __global__ void __launch_bounds__(256) diffusion_flux_kernel(Box3 b, DFL<T> C, DFL<T> q0, DFL<T> q1, DFL<T> q2,
@@ -441,6 +556,62 @@ int chmy_op(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_fiel
         hipLaunchKernelGGL(op_kernel<float>, grd, blk2(), 0, s, b, d, s0, s1, s2, a, (float)g->inv_spacing[0],
                            (float)g->inv_spacing[1], (float)g->inv_spacing[2]);
     }
+    CHMY_HIP(hipGetLastError());
+    return 0;
+}
+
+int chmy_op_masked(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, int nsrc,
+                   const chmy_field_t* src, const chmy_field_t* masks, const chmy_field_t* k, int dim, void* stream) {
+    CHMY_TRY(check_grid(g));
+    chmy_box_t bx;
+    CHMY_TRY(check_box(g, box, &bx));
+    CHMY_TRY(check_field(g, dst, "dst", nullptr, -1));
+    const int dt = dst->dtype;
+    const bool vec = op == CHMY_OP_DIVG || op == CHMY_OP_VMAG;
+    const bool per_dim = op == CHMY_OP_PARTIAL || op == CHMY_OP_DELTA || op == CHMY_OP_LEFT || op == CHMY_OP_RIGHT ||
+                         op == CHMY_OP_D2 || op == CHMY_OP_DKD;
+    const bool known = per_dim || vec || op == CHMY_OP_LERP || op == CHMY_OP_HLERP || op == CHMY_OP_LAPL ||
+                       op == CHMY_OP_DIVG_GRAD;
+    if (!known) return set_error(CHMY_E_ARG, "unknown operator %d", op);
+    const bool maskable = op != CHMY_OP_LERP && op != CHMY_OP_HLERP && op != CHMY_OP_VMAG;
+    const bool needs_k = op == CHMY_OP_DKD || op == CHMY_OP_DIVG_GRAD;
+    if (vec && nsrc != g->nd) return set_error(CHMY_E_ARG, "divg/vmag need nd source fields");
+    if (!vec && nsrc != 1) return set_error(CHMY_E_ARG, "operator needs one source field");
+    if (!src) return set_error(CHMY_E_ARG, "src is NULL");
+    if (per_dim && (dim < 0 || dim >= g->nd)) return set_error(CHMY_E_ARG, "dim %d out of range", dim);
+    if (masks && !maskable) return set_error(CHMY_E_ARG, "operator %d has no masked form", op);
+    if (needs_k != (k != nullptr)) return set_error(CHMY_E_ARG, needs_k ? "k is NULL" : "k is given to an operator without a coefficient");
+    for (int q = 0; q < nsrc; ++q) {
+        const int* want = nullptr;
+        int loc[3];
+        if (vec) {
+            for (int d = 0; d < 3; ++d) loc[d] = d == q ? 1 : 0;
+            want = loc;
+        }
+        CHMY_TRY(check_field(g, &src[q], "src", want, dt));
+        // the mask component read with src[q] is the one at src[q]'s location
+        if (masks) CHMY_TRY(check_field(g, &masks[q], "mask", src[q].loc, dt));
+    }
+    if (k) CHMY_TRY(check_field(g, k, "k", nullptr, dt));
+    Box3 b = box3(bx, g->nd);
+    OpArgs a{op, g->nd, dim, nsrc};
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grd = grid_for(b, blk2());
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        DFL<T> d = lview<T>(*dst), s0 = lview<T>(src[0]);
+        DFL<T> s1 = nsrc > 1 ? lview<T>(src[1]) : s0, s2 = nsrc > 2 ? lview<T>(src[2]) : s0;
+        DFL<T> w0 = masks ? lview<T>(masks[0]) : s0;
+        DFL<T> w1 = masks && nsrc > 1 ? lview<T>(masks[1]) : w0, w2 = masks && nsrc > 2 ? lview<T>(masks[2]) : w0;
+        DFL<T> kk = k ? lview<T>(*k) : s0;
+        const T i0 = (T)g->inv_spacing[0], i1 = (T)g->inv_spacing[1], i2 = (T)g->inv_spacing[2];
+        if (masks)
+            hipLaunchKernelGGL((op_ext_kernel<T, true>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a, i0, i1, i2);
+        else
+            hipLaunchKernelGGL((op_ext_kernel<T, false>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a, i0, i1, i2);
+    };
+    if (dt == CHMY_F64) run(double{});
+    else run(float{});
     CHMY_HIP(hipGetLastError());
     return 0;
 }

@@ -137,9 +137,29 @@ int chmy_bc_batchset(const chmy_grid_t* grid, const int32_t* counts,
  *   CHMY_OP_VMAG     dst[I] = vmag(src[0..nd-1])[I]  (field_operators.jl:116-125)
  *   CHMY_OP_LEFT/RIGHT  leftx/rightx... (cartesian_field_operators.jl:17-28) */
 enum { CHMY_OP_PARTIAL = 0, CHMY_OP_LERP = 1, CHMY_OP_DIVG = 2, CHMY_OP_VMAG = 3,
-       CHMY_OP_LEFT = 8, CHMY_OP_RIGHT = 9 };
+       CHMY_OP_D2 = 4, CHMY_OP_DKD = 5, CHMY_OP_LAPL = 6, CHMY_OP_DIVG_GRAD = 7,
+       CHMY_OP_LEFT = 8, CHMY_OP_RIGHT = 9, CHMY_OP_HLERP = 10, CHMY_OP_DELTA = 11 };
 int chmy_op(int op, const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* dst,
             int nsrc, const chmy_field_t* src, int dim, void* stream);
+/* Every operator, plain or masked (src/GridOperators/masked_operators.jl:11-39,
+ * masked_field_operators.jl:1-127, field_mask.jl:1-117), and the second-order
+ * and harmonic ones (partial_derivatives.jl:7-21, field_operators.jl:72-81,95-104,
+ * interpolation.jl:15,101):
+ *   CHMY_OP_DELTA      dst[I] = δ_dim src[I]            (partial_derivatives.jl:2)
+ *   CHMY_OP_D2         dst[I] = ∂²_dim src[I]           (at src's location)
+ *   CHMY_OP_DKD        dst[I] = ∂k∂_dim(src, k)[I]      (k lerped, never masked)
+ *   CHMY_OP_LAPL       dst[I] = lapl(src)[I]            (Σ_D ∂²_D, left-associated)
+ *   CHMY_OP_DIVG_GRAD  dst[I] = divg_grad(src, k)[I]    (Σ_D ∂k∂_D)
+ *   CHMY_OP_HLERP      dst[I] = hlerp(src, loc(dst))[I]
+ * and the chmy_op operators.  masks (NULL = unmasked): one field per source,
+ * the FieldMask component at that source's location, at(ω, loc(src[q]));
+ * a masked left/right reads f[Il]·ω[Il], so masked δ, ∂, ∂², ∂k∂, divg,
+ * lapl, divg_grad are built on f·ω at the knots.  LERP, HLERP, VMAG have no
+ * masked form.  k: the coefficient of DKD/DIVG_GRAD (any location), NULL
+ * otherwise.  Masks and k have the sources' dtype. */
+int chmy_op_masked(int op, const chmy_grid_t* grid, const chmy_box_t* box, const chmy_field_t* dst,
+                   int nsrc, const chmy_field_t* src, const chmy_field_t* masks, const chmy_field_t* k,
+                   int dim, void* stream);
 
 /* ---------------------------------------------------------------- diffusion */
 /* compute_q! (examples/diffusion_2d.jl:8-13): q_D[I] = -χ ∂_D C[I]. */

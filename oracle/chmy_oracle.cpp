@@ -96,6 +96,7 @@ enum Plant {
     PL_VMAG = 20,            // vmag without the square root
     PL_DIVG = 21,            // divg = dx Vx - dy Vy
     PL_THERMAL_SIGN = 22,    // update_thermal! with T_old + dt divg(qT)
+    PL_MASK_AT_OUT = 23,     // masked left/right read the mask at the output index, not the knot
 };
 int g_plant = 0;
 
@@ -719,7 +720,7 @@ template <class T> double sumsq_t(const orc_field* f, const orc_box* b) {
 
 // ---------------------------------------------------------------- generic operators (test kernels)
 // op: 0 ∂_D, 1 lerp to dst location, 2 divg, 3 vmag, 4 ∂²_D, 5 ∂k∂_D, 6 lapl, 7 divg_grad,
-//     8 left_D, 9 right_D, 10 hlerp to dst location
+//     8 left_D, 9 right_D, 10 hlerp to dst location, 11 δ_D
 template <class T> void op_t(int op, const orc_grid* gg, const orc_box* b, const orc_field* dst, int nsrc,
                              const orc_field* src, const orc_field* k, int D) {
     G<T> g(gg);
@@ -743,6 +744,72 @@ template <class T> void op_t(int op, const orc_grid* gg, const orc_box* b, const
             case 8: v = leftv(s[0], D, I); break;
             case 9: v = rightv(s[0], D, I); break;
             case 10: v = hlerp(s[0], d.loc, I); break;
+            case 11: v = rightv(s[0], D, I) - leftv(s[0], D, I); break;   // δ (partial_derivatives.jl:2)
+        }
+        d(I) = v;
+    });
+}
+
+// ---------------------------------------------------------------- masked operators
+// src/GridOperators/masked_operators.jl:11-39: left/right read the field at
+// the knot and multiply by the mask at the field's own location at that knot,
+// left(f, ω)[I] = f[Il]·at(ω, loc(f), Il); δ, ∂, ∂², ∂k∂ are built on them
+// as the unmasked forms are (∂k∂'s coefficient lerp is not masked), and
+// divg/lapl/divg_grad are the left-associated sums of
+// masked_field_operators.jl:56-127.  `w` is at(ω, loc(f)): the FieldMask
+// component at the source's location (field_mask.jl:1-117).
+template <class T> inline T leftm(const FV<T>& f, const FV<T>& w, int D, const Ix& I) {
+    const Ix J = f.loc[D] == 0 ? shift(I, D, -1) : I;
+    return f(J) * (g_plant == PL_MASK_AT_OUT ? w(I) : w(J));
+}
+template <class T> inline T rightm(const FV<T>& f, const FV<T>& w, int D, const Ix& I) {
+    const Ix J = f.loc[D] == 0 ? I : shift(I, D, +1);
+    return f(J) * (g_plant == PL_MASK_AT_OUT ? w(I) : w(J));
+}
+template <class T> inline T deltam(const FV<T>& f, const FV<T>& w, int D, const Ix& I) {
+    return rightm(f, w, D, I) - leftm(f, w, D, I);
+}
+template <class T> inline T ddm(const FV<T>& f, const FV<T>& w, const G<T>& g, int D, const Ix& I) {
+    return deltam(f, w, D, I) * g.isp[D];
+}
+template <class T> T d2m(const FV<T>& f, const FV<T>& w, const G<T>& g, int D, const Ix& I) {
+    Ix Ir = I, Il = I;
+    if (f.loc[D] == 0) { Ir = shift(I, D, +1); } else { Il = shift(I, D, -1); }
+    return (ddm(f, w, g, D, Ir) - ddm(f, w, g, D, Il)) * g.isp[D];
+}
+template <class T> T dkdm(const FV<T>& f, const FV<T>& w, const FV<T>& k, const G<T>& g, int D, const Ix& I) {
+    Ix Ir = I, Il = I;
+    if (f.loc[D] == 0) { Ir = shift(I, D, +1); } else { Il = shift(I, D, -1); }
+    int floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
+    floc[D] = 1 - floc[D];
+    return (lerp(k, floc, Ir) * ddm(f, w, g, D, Ir) - lerp(k, floc, Il) * ddm(f, w, g, D, Il)) * g.isp[D];
+}
+
+// op as op_t (0 ∂, 2 divg, 4 ∂², 5 ∂k∂, 6 lapl, 7 divg_grad, 8 left, 9 right, 11 δ);
+// masks[q] is the mask component at src[q]'s location.
+template <class T> void op_masked_t(int op, const orc_grid* gg, const orc_box* b, const orc_field* dst, int nsrc,
+                                    const orc_field* src, const orc_field* masks, const orc_field* k, int D) {
+    G<T> g(gg);
+    FV<T> d(dst);
+    alignas(FV<T>) unsigned char b1[3 * sizeof(FV<T>)], b2[3 * sizeof(FV<T>)];
+    FV<T>* s = reinterpret_cast<FV<T>*>(b1);
+    FV<T>* w = reinterpret_cast<FV<T>*>(b2);
+    views(src, nsrc, s);
+    views(masks, nsrc, w);
+    FV<T> kk(k ? k : src);
+    for_box(b, g.nd, [&](const Ix& I) {
+        T v = 0;
+        switch (op) {
+            case 0: v = ddm(s[0], w[0], g, D, I); break;
+            case 2: { v = ddm(s[0], w[0], g, 0, I); for (int q = 1; q < g.nd; ++q) v = v + ddm(s[q], w[q], g, q, I); } break;
+            case 4: v = d2m(s[0], w[0], g, D, I); break;
+            case 5: v = dkdm(s[0], w[0], kk, g, D, I); break;
+            case 6: { v = d2m(s[0], w[0], g, 0, I); for (int q = 1; q < g.nd; ++q) v = v + d2m(s[0], w[0], g, q, I); } break;
+            case 7: { v = dkdm(s[0], w[0], kk, g, 0, I); for (int q = 1; q < g.nd; ++q) v = v + dkdm(s[0], w[0], kk, g, q, I); } break;
+            case 8: v = leftm(s[0], w[0], D, I); break;
+            case 9: v = rightm(s[0], w[0], D, I); break;
+            case 11: v = deltam(s[0], w[0], D, I); break;
+            default: fail("oracle: operator without a masked form");
         }
         d(I) = v;
     });
@@ -946,6 +1013,12 @@ double orc_sumsq(const orc_field* f, const orc_box* b) {
 int orc_op(int op, const orc_grid* g, const orc_box* b, const orc_field* dst, int nsrc, const orc_field* src,
            const orc_field* k, int D) {
     if (dst->dtype == 0) op_t<double>(op, g, b, dst, nsrc, src, k, D); else op_t<float>(op, g, b, dst, nsrc, src, k, D);
+    return g_err;
+}
+int orc_op_masked(int op, const orc_grid* g, const orc_box* b, const orc_field* dst, int nsrc,
+                  const orc_field* src, const orc_field* masks, const orc_field* k, int D) {
+    if (dst->dtype == 0) op_masked_t<double>(op, g, b, dst, nsrc, src, masks, k, D);
+    else op_masked_t<float>(op, g, b, dst, nsrc, src, masks, k, D);
     return g_err;
 }
 int orc_sia_flux(const orc_grid* g, const orc_field* q, const orc_field* H, const orc_field* B, const orc_field* A,

@@ -90,6 +90,8 @@ def _declare(L):
     L.orc_sumsq.restype = C.c_double
     L.orc_sumsq.argtypes = [P(OField), P(OBox)]
     L.orc_op.argtypes = [C.c_int, P(OGrid), P(OBox), P(OField), C.c_int, P(OField), P(OField), C.c_int]
+    L.orc_op_masked.argtypes = [C.c_int, P(OGrid), P(OBox), P(OField), C.c_int, P(OField), P(OField), P(OField),
+                                C.c_int]
     L.orc_sia_flux.argtypes = [P(OGrid), P(OField), P(OField), P(OField), P(OField), C.c_double, C.c_int,
                                C.c_double, C.c_double, C.c_double]
     L.orc_sia_update.argtypes = [P(OGrid), P(OField), P(OField), P(OField), P(OField), C.c_double, C.c_int,
@@ -119,7 +121,7 @@ PLANTS = {
     "vertex_diff": 1, "lerp_knots": 2, "center_dirichlet": 3, "neumann_sign": 4, "bc_order": 5,
     "diff_flux": 6, "stress_dev": 7, "edge_swap": 8, "rv_swap": 9, "vel_ml_side": 10, "maxloc_2n": 11,
     "thermal_upwind": 12, "sia_exp": 13, "sia_grad": 14, "sia_m": 15, "resid_rp": 16, "pr_update": 17,
-    "grad_edge": 18, "pt_params": 19, "vmag": 20, "divg": 21, "thermal_sign": 22,
+    "grad_edge": 18, "pt_params": 19, "vmag": 20, "divg": 21, "thermal_sign": 22, "mask_at_out": 23,
 }
 
 
@@ -420,7 +422,7 @@ def sumsq(f, b=None):
 
 
 OPS = {"partial": 0, "lerp": 1, "divg": 2, "vmag": 3, "d2": 4, "dkd": 5, "lapl": 6, "divg_grad": 7,
-       "left": 8, "right": 9, "hlerp": 10}
+       "left": 8, "right": 9, "hlerp": 10, "delta": 11}
 
 
 def op(name, grid, dst, src, k=None, dim=0, b=None):
@@ -428,6 +430,17 @@ def op(name, grid, dst, src, k=None, dim=0, b=None):
     src = src if isinstance(src, (list, tuple)) else [src]
     _check(lib().orc_op(OPS[name], C.byref(grid.desc()), C.byref(b), C.byref(dst.desc()), len(src),
                         _fields(src), _opt(k), dim))
+
+
+def op_masked(name, grid, dst, src, masks, k=None, dim=0, b=None):
+    """Masked operator (masked_operators.jl, masked_field_operators.jl);
+    masks[q] = the FieldMask component at src[q]'s location."""
+    b = b or grid.launcher_box()
+    src = src if isinstance(src, (list, tuple)) else [src]
+    masks = masks if isinstance(masks, (list, tuple)) else [masks]
+    assert len(masks) == len(src)
+    _check(lib().orc_op_masked(OPS[name], C.byref(grid.desc()), C.byref(b), C.byref(dst.desc()), len(src),
+                               _fields(src), _fields(masks), _opt(k), dim))
 
 
 # ------------------------------------------------------------ SIA

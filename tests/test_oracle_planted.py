@@ -8,6 +8,7 @@ import pytest
 import numpy as np
 import oracle as O
 import test_oracle_closed_forms as CF
+import test_oracle_masked as MK
 import test_oracle_pins as PN
 import test_oracle_reference_units as RU
 import test_oracle_stokes_adjoint as SA
@@ -42,6 +43,8 @@ CASES = [
     ("pt_params", PN.test_pt_params_converge_at_the_damped_wave_rate, ()),
     ("vmag", RU.test_vmag_constant, (F64,)),
     ("divg", RU.test_divg_equals_sum_of_partials, (F64,)),
+    ("mask_at_out", MK.test_masked_operators_are_unmasked_operators_of_the_masked_field, (2, (0, 1), 1, "fraction")),
+    ("mask_at_out", MK.test_masked_sums_divg_lapl_divg_grad, (3, "fraction")),
 ]
 
 
