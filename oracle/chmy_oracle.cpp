@@ -97,6 +97,8 @@ enum Plant {
     PL_DIVG = 21,            // divg = dx Vx - dy Vy
     PL_THERMAL_SIGN = 22,    // update_thermal! with T_old + dt divg(qT)
     PL_MASK_AT_OUT = 23,     // masked left/right read the mask at the output index, not the knot
+    PL_HLERP_ARITH = 24,     // hlerp with the arithmetic rule muladd(t, b - a, a)
+    PL_DELTA_SIGN = 25,      // δ = left - right
 };
 int g_plant = 0;
 
@@ -205,6 +207,7 @@ template <class T> T hlerp_rec(const FV<T>& f, const int* dims, int level, const
     T a = hlerp_rec(f, dims, level - 1, to, Il);
     T b = hlerp_rec(f, dims, level - 1, to, Ir);
     // itp_rule(::HarmonicLinear, t, a, b) = inv(muladd(t, inv(b) - inv(a), inv(a)))
+    if (g_plant == PL_HLERP_ARITH) return std::fma(T(0.5), b - a, a);
     return T(1) / std::fma(T(0.5), T(1) / b - T(1) / a, T(1) / a);
 }
 template <class T> T hlerp(const FV<T>& f, const int* to, const Ix& I) {
@@ -744,7 +747,9 @@ template <class T> void op_t(int op, const orc_grid* gg, const orc_box* b, const
             case 8: v = leftv(s[0], D, I); break;
             case 9: v = rightv(s[0], D, I); break;
             case 10: v = hlerp(s[0], d.loc, I); break;
-            case 11: v = rightv(s[0], D, I) - leftv(s[0], D, I); break;   // δ (partial_derivatives.jl:2)
+            case 11:   // δ (partial_derivatives.jl:2)
+                v = g_plant == PL_DELTA_SIGN ? leftv(s[0], D, I) - rightv(s[0], D, I) : rightv(s[0], D, I) - leftv(s[0], D, I);
+                break;
         }
         d(I) = v;
     });

@@ -122,6 +122,7 @@ PLANTS = {
     "diff_flux": 6, "stress_dev": 7, "edge_swap": 8, "rv_swap": 9, "vel_ml_side": 10, "maxloc_2n": 11,
     "thermal_upwind": 12, "sia_exp": 13, "sia_grad": 14, "sia_m": 15, "resid_rp": 16, "pr_update": 17,
     "grad_edge": 18, "pt_params": 19, "vmag": 20, "divg": 21, "thermal_sign": 22, "mask_at_out": 23,
+    "hlerp_arith": 24, "delta_sign": 25,
 }
 
 

@@ -328,3 +328,55 @@ def test_pt_params_converge_at_the_damped_wave_rate():
     n = 32
     it = _pt_pure_shear_iterations(n)
     assert it <= 30 * n, it
+
+
+# ---------------------------------------------------------------- δ and hlerp
+def _rand_field(g, loc, rng, lo, hi):
+    f = O.HField(g, loc)
+    f.arr[...] = rng.uniform(lo, hi, f.arr.shape)
+    return f
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_delta_is_the_partial_before_its_inverse_spacing(nd):
+    """δ (partial_derivatives.jl:2): ∂ = δ·iΔ (:5), so δ times iΔ is the
+    reference-pinned ∂ bitwise, for every source location and dim."""
+    g = O.Grid((-0.3,) * nd, (1.3,) * nd, [(11,), (9, 7), (7, 6, 5)][nd - 1])
+    rng = np.random.default_rng(nd)
+    for m in range(1 << nd):
+        loc = tuple((m >> d) & 1 for d in range(nd))
+        f = _rand_field(g, loc, rng, -1.0, 1.0)
+        for D in range(nd):
+            to = tuple(1 - l if d == D else l for d, l in enumerate(loc))
+            a, b = O.HField(g, to), O.HField(g, to)
+            O.op("delta", g, a, f, dim=D)
+            O.op("partial", g, b, f, dim=D)
+            assert np.array_equal(a.arr * g.inv_spacing[D], b.arr), (loc, D)
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_hlerp_is_the_harmonic_rule(nd):
+    """hlerp (interpolation.jl:15,101): each knot pair a, b reduces to
+    inv(muladd(0.5, inv(b) - inv(a), inv(a))), the harmonic mean 2ab/(a+b).
+    In 1D hlerp(f) is 1/lerp(1/f) bitwise, lerp being reference-pinned; in
+    2D/3D the nested harmonic means equal the harmonic mean of the 2^m knots,
+    1/mean(1/knots), to a few ulp."""
+    g = O.Grid((-0.3,) * nd, (1.3,) * nd, [(11,), (9, 7), (7, 6, 5)][nd - 1])
+    rng = np.random.default_rng(10 + nd)
+    for m in range(1 << nd):
+        loc = tuple((m >> d) & 1 for d in range(nd))
+        f = _rand_field(g, loc, rng, 0.2, 5.0)
+        inv = O.HField(g, loc)
+        inv.arr[...] = 1.0 / f.arr
+        for q in range(1 << nd):
+            to = tuple((q >> d) & 1 for d in range(nd))
+            h, l = O.HField(g, to), O.HField(g, to)
+            box = O.interior_box(h)
+            O.op("hlerp", g, h, f, b=box)
+            O.op("lerp", g, l, inv, b=box)
+            sl = tuple(slice(2, 2 + n) for n in reversed(h.n))
+            want = 1.0 / l.arr[sl]
+            if nd == 1:
+                assert np.array_equal(h.arr[sl], want), (loc, to)
+            else:
+                assert np.max(np.abs(h.arr[sl] / want - 1.0)) <= 8e-16 * (1 << nd), (loc, to)

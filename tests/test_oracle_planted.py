@@ -45,6 +45,9 @@ CASES = [
     ("divg", RU.test_divg_equals_sum_of_partials, (F64,)),
     ("mask_at_out", MK.test_masked_operators_are_unmasked_operators_of_the_masked_field, (2, (0, 1), 1, "fraction")),
     ("mask_at_out", MK.test_masked_sums_divg_lapl_divg_grad, (3, "fraction")),
+    ("hlerp_arith", PN.test_hlerp_is_the_harmonic_rule, (1,)),
+    ("hlerp_arith", PN.test_hlerp_is_the_harmonic_rule, (3,)),
+    ("delta_sign", PN.test_delta_is_the_partial_before_its_inverse_spacing, (2,)),
 ]
 
 
