@@ -17,7 +17,8 @@ from .fields import (Field, TensorField, VectorField, halo, interior, location, 
                      tensor_locations, vector_location)
 from .field_mask import FieldMask
 from .function_field import FunctionField, TraceError, ifelse
-from .grids import Center, UniformGrid, Vertex, coord, flip, inv_spacing, spacing
+from .grids import (Center, FunctionAxis, StructuredGrid, UniformAxis, UniformGrid, Vertex, coord, flip,
+                    inv_spacing, spacing)
 from .kernel_launch import Launcher, Offset
 
 __all__ = [
@@ -26,5 +27,6 @@ __all__ = [
     "FieldBatch", "FirstOrderBC", "Neumann", "batch", "bc_", "Field", "TensorField", "VectorField", "halo",
     "interior", "location", "set_", "set_storage", "tensor_locations", "vector_location", "Center",
     "UniformGrid", "Vertex", "coord", "flip", "inv_spacing", "spacing", "Launcher", "Offset", "kernels",
-    "ChmyError", "FunctionField", "TraceError", "ifelse", "FieldMask",
+    "ChmyError", "FunctionField", "TraceError", "ifelse", "FieldMask", "FunctionAxis", "UniformAxis",
+    "StructuredGrid",
 ]

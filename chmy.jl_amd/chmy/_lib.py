@@ -27,7 +27,13 @@ class ChmyField(C.Structure):
 class ChmyGrid(C.Structure):
     _fields_ = [("nd", C.c_int32), ("reserved", C.c_int32), ("n", C.c_int64 * 3),
                 ("origin", C.c_double * 3), ("extent", C.c_double * 3),
-                ("spacing", C.c_double * 3), ("inv_spacing", C.c_double * 3)]
+                ("spacing", C.c_double * 3), ("inv_spacing", C.c_double * 3),
+                ("axis", C.c_void_p * 3)]
+
+
+# FunctionAxis metric tables (chmy_grid_t.axis)
+AXIS_PAD, AXIS_ROWS = 4, 7
+AX_VERTEX, AX_CENTER, AX_DC, AX_DV, AX_IDC, AX_IDV, AX_WCV = range(7)
 
 
 class ChmyBox(C.Structure):

@@ -24,11 +24,21 @@ int hip_status(hipError_t e, const char* what) {
     return (int)e;
 }
 
-int check_grid(const chmy_grid_t* g) {
+int check_grid_any(const chmy_grid_t* g) {
     if (!g) return set_error(CHMY_E_ARG, "grid is NULL");
     if (g->nd < 1 || g->nd > 3) return set_error(CHMY_E_ARG, "grid.nd = %d not in 1..3", g->nd);
     for (int d = 0; d < g->nd; ++d)
         if (g->n[d] < 1) return set_error(CHMY_E_ARG, "grid.n[%d] = %lld < 1", d, (long long)g->n[d]);
+    for (int d = g->nd; d < 3; ++d)
+        if (g->axis[d]) return set_error(CHMY_E_ARG, "grid.axis[%d] set beyond nd = %d", d, g->nd);
+    return 0;
+}
+
+int check_grid(const chmy_grid_t* g) {
+    CHMY_TRY(check_grid_any(g));
+    if (nonuniform(g))
+        return set_error(CHMY_E_ARG, "a grid with a FunctionAxis (non-uniform spacing) is accepted by the operators, "
+                                     "bc! and set! only; this entry point needs uniform axes");
     return 0;
 }
 

@@ -45,12 +45,23 @@ struct BCLaunch {
     int32_t td[2];    // tangential dims
     int64_t tn[2];    // tangential extents (n_t + 3)
     int32_t K[2];     // fields per side
-    double h;         // spacing along D (Neumann)
+    double h;         // spacing along D (Neumann, uniform axis)
+    AxisT ax;         // FunctionAxis table along D, or p == nullptr
     BCItem it[2][kMaxBatch];
 };
 
+// Center Dirichlet coefficient t and Neumann spacing h of the ghost ig
+// (inner neighbour in) along an axis; uniform: 2 and h_uniform.
+__device__ __forceinline__ double dirichlet_t(const AxisT& ax, int64_t ig, int64_t in) {
+    return ax.p ? (2.0 * ax.at(CHMY_AX_DC, in)) * ax.at(CHMY_AX_IDV, ig) : 2.0;
+}
+__device__ __forceinline__ double neumann_h(const AxisT& ax, int vertex, int64_t ig, double h) {
+    return ax.p ? ax.at(vertex ? CHMY_AX_DC : CHMY_AX_DV, ig) : h;
+}
+
 template <class T>
-__device__ __forceinline__ void apply_one(const BCItem& b, int side, int D, const int64_t* It, double h) {
+__device__ __forceinline__ void apply_one(const BCItem& b, int side, int D, const int64_t* It, double h,
+                                          const AxisT& ax) {
     T* p = static_cast<T*>(b.p);
     int64_t base = 0;
     for (int d = 0; d < 3; ++d)
@@ -74,10 +85,10 @@ __device__ __forceinline__ void apply_one(const BCItem& b, int side, int D, cons
     int64_t in = side == 1 ? 1 : b.m;
     T bn = p[base + in * sD];
     if (b.kind == CHMY_DIRICHLET) {
-        p[base + ig * sD] = fma(T(2), val - bn, bn);
+        p[base + ig * sD] = fma(T(dirichlet_t(ax, ig, in)), val - bn, bn);
     } else {
         T qs = side == 1 ? -val : val;
-        p[base + ig * sD] = fma(T(h), qs, bn);
+        p[base + ig * sD] = fma(T(neumann_h(ax, b.vertex, ig, h)), qs, bn);
     }
 }
 
@@ -90,7 +101,7 @@ __global__ void __launch_bounds__(256) bc_kernel(BCLaunch L) {
     if (L.nt > 0) It[L.td[0]] = t % L.tn[0];
     if (L.nt > 1) It[L.td[1]] = t / L.tn[0];
     for (int s = 0; s < 2; ++s)
-        for (int k = 0; k < L.K[s]; ++k) apply_one<T>(L.it[s][k], s + 1, L.D, It, L.h);
+        for (int k = 0; k < L.K[s]; ++k) apply_one<T>(L.it[s][k], s + 1, L.D, It, L.h, L.ax);
 }
 
 static int fill_item(const chmy_grid_t* g, int D, const chmy_field_t* f, const chmy_bc_t* bc, int dtype,
@@ -151,6 +162,7 @@ static int launch_dim(const chmy_grid_t* g, int D, const int* K, const chmy_fiel
     for (int d = 0; d < g->nd; ++d)
         if (d != D) { L.td[L.nt] = d; L.tn[L.nt] = g->n[d] + 3; ++L.nt; }
     L.h = g->spacing[D];
+    L.ax = axes_of(g).a[D];
     for (int s = 0; s < 2; ++s) {
         L.K[s] = K[s];
         for (int k = 0; k < K[s]; ++k) CHMY_TRY(fill_item(g, D, &f[s][k], &bc[s][k], dtype, &L.it[s][k]));
@@ -187,6 +199,7 @@ struct BCSetLaunch {
     int32_t nd, nf;
     int64_t tn[3];    // n_d + 3
     double h[3];
+    Axes3 ax;
     BCSetField f[kSetFields];
 };
 
@@ -273,9 +286,11 @@ __global__ void __launch_bounds__(256) bc_set_kernel(BCSetLaunch L) {
         side = ns;
     }
     for (int q = nop - 1; q >= 0; --q) {
-        const BCRule& r = F.r[oj[q]][os[q]];
-        if (r.kind == CHMY_DIRICHLET) v = fma(T(2), ov[q] - v, v);
-        else v = fma(T(L.h[oj[q]]), os[q] == 0 ? -ov[q] : ov[q], v);
+        const int jq = oj[q];
+        const BCRule& r = F.r[jq][os[q]];
+        const int64_t ig = os[q] == 0 ? 0 : F.m[jq] + 1, in = os[q] == 0 ? 1 : F.m[jq];
+        if (r.kind == CHMY_DIRICHLET) v = fma(T(dirichlet_t(L.ax.a[jq], ig, in)), ov[q] - v, v);
+        else v = fma(T(neumann_h(L.ax.a[jq], F.vertex[jq], ig, L.h[jq])), os[q] == 0 ? -ov[q] : ov[q], v);
     }
     static_cast<T*>(F.p)[pofs] = v;
 }
@@ -295,6 +310,7 @@ static int pack_set(const chmy_grid_t* g, const int32_t* counts, const chmy_fiel
                     BCSetLaunch* L, int* dtype) {
     std::memset(L, 0, sizeof *L);
     L->nd = g->nd;
+    L->ax = axes_of(g);
     for (int d = 0; d < 3; ++d) {
         L->tn[d] = d < g->nd ? g->n[d] + 3 : 1;
         L->h[d] = d < g->nd ? g->spacing[d] : 0.0;
@@ -367,7 +383,7 @@ extern "C" {
 
 int chmy_bc_side(const chmy_grid_t* g, int dim, int side, int K, const chmy_field_t* fields,
                  const chmy_bc_t* bcs, void* stream) {
-    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_grid_any(g));
     if (dim < 0 || dim >= g->nd || (side != 1 && side != 2) || K < 0 || (K > 0 && (!fields || !bcs)))
         return set_error(CHMY_E_ARG, "chmy_bc_side: bad arguments");
     int Ks[2] = {0, 0};
@@ -379,7 +395,7 @@ int chmy_bc_side(const chmy_grid_t* g, int dim, int side, int K, const chmy_fiel
 
 int chmy_bc_batchset(const chmy_grid_t* g, const int32_t* counts, const chmy_field_t* fields,
                      const chmy_bc_t* bcs, void* stream) {
-    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_grid_any(g));
     if (!counts) return set_error(CHMY_E_ARG, "counts is NULL");
     int offs[7] = {0};
     for (int q = 0; q < 2 * g->nd; ++q) {

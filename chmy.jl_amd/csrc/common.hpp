@@ -106,7 +106,34 @@ DFL<T> lview(const chmy_field_t& f) {
 // ------------------------------------------------------------------ validation
 // All descriptors are checked on the host before any launch, so that every
 // access a kernel makes (box ± 1 in each dim) lies inside the field storage.
+// check_grid accepts uniform grids only; check_grid_any also grids with a
+// FunctionAxis (the operators, bc! and set!).
 int check_grid(const chmy_grid_t* g);
+int check_grid_any(const chmy_grid_t* g);
+
+// A FunctionAxis metric table on the device (chmy_grid_t.axis): entry (row, i)
+// at p[row * rs + i + CHMY_AXIS_PAD]; p == nullptr on a uniform dim.
+struct AxisT {
+    const double* p;
+    int64_t rs;
+    __host__ __device__ __forceinline__ double at(int row, int64_t i) const { return p[row * rs + i + CHMY_AXIS_PAD]; }
+};
+struct Axes3 {
+    AxisT a[3];
+};
+inline Axes3 axes_of(const chmy_grid_t* g) {
+    Axes3 r;
+    for (int d = 0; d < 3; ++d) {
+        r.a[d].p = d < g->nd ? g->axis[d] : nullptr;
+        r.a[d].rs = d < g->nd ? g->n[d] + 2 + 2 * CHMY_AXIS_PAD : 0;
+    }
+    return r;
+}
+inline bool nonuniform(const chmy_grid_t* g) {
+    for (int d = 0; d < g->nd; ++d)
+        if (g->axis[d]) return true;
+    return false;
+}
 int check_box(const chmy_grid_t* g, const chmy_box_t* b, chmy_box_t* out);
 // loc: expected location per dim (-1 = any); dtype: expected (-1 = any).
 // bf16 storage is rejected unless allow_bf16 (only the forcing of FP32

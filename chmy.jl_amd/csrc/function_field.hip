@@ -39,6 +39,7 @@ struct PSrc {
 struct PProg {
     int32_t n, nd;
     double o[3], h[3], ih[3];
+    Axes3 ax;            // FunctionAxis tables (p == nullptr: uniform dim)
     chmy_pop_t op[kPMaxOps];
     double k[kPMaxConst];
     PSrc src[kPMaxSrc];
@@ -78,13 +79,23 @@ __global__ void __launch_bounds__(256) program_kernel(const PProg P, DF<T> dst, 
             case CHMY_P_CONST: v = P.k[o.a]; break;
             case CHMY_P_INDEX: v = (double)(I[o.a] + o.b); break;
             case CHMY_P_COORD: {
+                if (P.ax.a[o.a].p) {
+                    v = P.ax.a[o.a].at(o.b == CHMY_VERTEX ? CHMY_AX_VERTEX : CHMY_AX_CENTER, I[o.a] + o.c);
+                    break;
+                }
                 const double i1 = (double)(I[o.a] + o.c - 1);
                 const double h = P.h[o.a], og = P.o[o.a];
                 v = o.b == CHMY_VERTEX ? fma(i1, h, og) : fma(i1, h, fma(0.5, h, og));
                 break;
             }
-            case CHMY_P_SPACING: v = P.h[o.a]; break;
-            case CHMY_P_INV_SPACING: v = P.ih[o.a]; break;
+            case CHMY_P_SPACING:
+                v = P.ax.a[o.a].p ? P.ax.a[o.a].at(o.b == CHMY_VERTEX ? CHMY_AX_DV : CHMY_AX_DC, I[o.a] + o.c)
+                                  : P.h[o.a];
+                break;
+            case CHMY_P_INV_SPACING:
+                v = P.ax.a[o.a].p ? P.ax.a[o.a].at(o.b == CHMY_VERTEX ? CHMY_AX_IDV : CHMY_AX_IDC, I[o.a] + o.c)
+                                  : P.ih[o.a];
+                break;
             case CHMY_P_LOAD:
                 v = load_src(P.src[o.a], i + off8(o.b, 0), j + off8(o.b, 1), k + off8(o.b, 2));
                 break;
@@ -146,10 +157,14 @@ static int check_program(const chmy_grid_t* g, const chmy_pop_t* prog, int nprog
             case CHMY_P_CONST: ok = o.a >= 0 && o.a < nconst; break;
             case CHMY_P_INDEX: ok = o.a >= 0 && o.a < g->nd && o.b >= -64 && o.b <= 64; break;
             case CHMY_P_COORD:
-                ok = o.a >= 0 && o.a < g->nd && (o.b == CHMY_CENTER || o.b == CHMY_VERTEX) && o.c >= -64 && o.c <= 64;
-                break;
             case CHMY_P_SPACING:
-            case CHMY_P_INV_SPACING: ok = o.a >= 0 && o.a < g->nd; break;
+            case CHMY_P_INV_SPACING:
+                ok = o.a >= 0 && o.a < g->nd && (o.b == CHMY_CENTER || o.b == CHMY_VERTEX) && o.c >= -64 && o.c <= 64;
+                // a FunctionAxis table covers logical -PAD .. n+1+PAD
+                if (ok && g->axis[o.a])
+                    ok = b.lo[o.a] + o.c >= -CHMY_AXIS_PAD &&
+                         b.lo[o.a] + b.len[o.a] - 1 + o.c <= g->n[o.a] + 1 + CHMY_AXIS_PAD;
+                break;
             case CHMY_P_LOAD: {
                 ok = o.a >= 0 && o.a < nsrc && o.b >= 0 && o.b < (1 << 24);
                 if (!ok) break;
@@ -190,7 +205,7 @@ extern "C" {
 
 int chmy_set_program(const chmy_grid_t* g, const chmy_field_t* dst, const chmy_box_t* box, const chmy_pop_t* prog,
                      int nprog, const double* consts, int nconst, const chmy_field_t* srcs, int nsrc, void* stream) {
-    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_grid_any(g));
     CHMY_TRY(check_field(g, dst, "dst", nullptr, -1, true));
     if (nconst < 0 || nconst > kPMaxConst || (nconst > 0 && !consts))
         return set_error(CHMY_E_SIZE, "point program: %d constants, at most %d", nconst, kPMaxConst);
@@ -218,6 +233,7 @@ int chmy_set_program(const chmy_grid_t* g, const chmy_field_t* dst, const chmy_b
         P.h[d] = d < g->nd ? g->spacing[d] : 0.0;
         P.ih[d] = d < g->nd ? g->inv_spacing[d] : 0.0;
     }
+    P.ax = axes_of(g);
     std::memcpy(P.op, prog, sizeof(chmy_pop_t) * nprog);
     if (nconst) std::memcpy(P.k, consts, sizeof(double) * nconst);
     for (int s = 0; s < nsrc; ++s) {

@@ -106,6 +106,21 @@ __global__ void __launch_bounds__(256) op_kernel(Box3 b, DFL<T> dst, DFL<T> s0, 
 // of leftv/rightv/ddv.  ∂² and ∂k∂ (partial_derivatives.jl:7-21) take
 // differences of ∂ at the two knots of the flipped location; ∂k∂ weights
 // them with lerp(k) at those knots (the coefficient is never masked).
+//
+// On a FunctionAxis (NU, non-NULL table along the dim) the metrics are the
+// reference's: ∂ scales δ by iΔ(grid, loc(f), dim, I) (partial_derivatives.jl:5,
+// field_operators.jl:20-24: the source field's location, at the output
+// index), ∂² and ∂k∂ their difference by iΔ(loc(f), I) (:7-21), and the
+// Center -> Vertex lerp weight at vertex i is 0.5 Δc(i-1) iΔv(i)
+// (interpolation.jl:29); Vertex -> Center stays 0.5 (:28).
+__device__ __forceinline__ int64_t ix_of(Ix3 I, int D) { return D == 0 ? I.i : D == 1 ? I.j : I.k; }
+
+template <class T, bool NU>
+__device__ __forceinline__ T isp_at(const Axes3& ax, const T* isp, int D, int loc, Ix3 I) {
+    if (!NU || !ax.a[D].p) return isp[D];
+    return T(ax.a[D].at(loc == 0 ? CHMY_AX_IDC : CHMY_AX_IDV, ix_of(I, D)));
+}
+
 template <class T, bool M>
 __device__ __forceinline__ T knot(const DFL<T>& f, const DFL<T>& w, Ix3 J) {
     const T v = g_at(f, J);
@@ -119,34 +134,20 @@ template <class T, bool M>
 __device__ __forceinline__ T rightk(const DFL<T>& f, const DFL<T>& w, int D, Ix3 I) {
     return knot<T, M>(f, w, f.loc[D] == 0 ? I : sh(I, D, 1));
 }
-template <class T, bool M>
-__device__ __forceinline__ T ddk(const DFL<T>& f, const DFL<T>& w, T isp, int D, Ix3 I) {
-    return (rightk<T, M>(f, w, D, I) - leftk<T, M>(f, w, D, I)) * isp;
+template <class T, bool M, bool NU>
+__device__ __forceinline__ T ddk(const DFL<T>& f, const DFL<T>& w, const Axes3& ax, const T* isp, int D, Ix3 I) {
+    return (rightk<T, M>(f, w, D, I) - leftk<T, M>(f, w, D, I)) * isp_at<T, NU>(ax, isp, D, f.loc[D], I);
 }
 // knots of the flipped location around I: Center f -> (I, I+e), Vertex f -> (I-e, I)
 __device__ __forceinline__ void flip_knots(int loc, int D, Ix3 I, Ix3& Il, Ix3& Ir) {
     Il = loc == 0 ? I : sh(I, D, -1);
     Ir = loc == 0 ? sh(I, D, 1) : I;
 }
-template <class T, bool M>
-__device__ T d2k(const DFL<T>& f, const DFL<T>& w, T isp, int D, Ix3 I) {
-    Ix3 Il, Ir;
-    flip_knots(f.loc[D], D, I, Il, Ir);
-    return (ddk<T, M>(f, w, isp, D, Ir) - ddk<T, M>(f, w, isp, D, Il)) * isp;
-}
-template <class T, bool M>
-__device__ T dkdk(const DFL<T>& f, const DFL<T>& w, const DFL<T>& k, T isp, int D, Ix3 I) {
-    Ix3 Il, Ir;
-    flip_knots(f.loc[D], D, I, Il, Ir);
-    int32_t floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
-    floc[D] = 1 - floc[D];
-    return (lerpv(k, floc, Ir) * ddk<T, M>(f, w, isp, D, Ir) - lerpv(k, floc, Il) * ddk<T, M>(f, w, isp, D, Il)) *
-           isp;
-}
-// hlerp (interpolation.jl:15,101): the lerp knots reduced with
-// inv(muladd(0.5, inv(b) - inv(a), inv(a))), first differing dim innermost
-template <class T>
-__device__ T hlerpv(const DFL<T>& f, const int32_t* to, Ix3 I) {
+// lerp and hlerp with the weights of the axis (HARM: the harmonic rule
+// inv(muladd(t, inv(b) - inv(a), inv(a))), interpolation.jl:15,101); knots
+// over the differing dims, first differing dim innermost
+template <class T, bool NU, bool HARM>
+__device__ T itpw(const DFL<T>& f, const int32_t* to, const Axes3& ax, Ix3 I) {
     int dims[3], m = 0;
     for (int d = 0; d < f.nd; ++d)
         if (f.loc[d] != to[d]) dims[m++] = d;
@@ -162,19 +163,38 @@ __device__ T hlerpv(const DFL<T>& f, const int32_t* to, Ix3 I) {
         v[q] = g_at(f, J);
     }
     for (int b = 0; b < m; ++b) {
+        const int D = dims[b];
+        const T t = (NU && ax.a[D].p && f.loc[D] == 0) ? T(ax.a[D].at(CHMY_AX_WCV, ix_of(I, D))) : T(0.5);
         int half = 1 << (m - b - 1);
         for (int q = 0; q < half; ++q) {
             T a = v[2 * q], c = v[2 * q + 1];
-            v[q] = T(1) / fma(T(0.5), T(1) / c - T(1) / a, T(1) / a);
+            v[q] = HARM ? T(1) / fma(t, T(1) / c - T(1) / a, T(1) / a) : fma(t, c - a, a);
         }
     }
     return v[0];
 }
+template <class T, bool M, bool NU>
+__device__ T d2k(const DFL<T>& f, const DFL<T>& w, const Axes3& ax, const T* isp, int D, Ix3 I) {
+    Ix3 Il, Ir;
+    flip_knots(f.loc[D], D, I, Il, Ir);
+    return (ddk<T, M, NU>(f, w, ax, isp, D, Ir) - ddk<T, M, NU>(f, w, ax, isp, D, Il)) *
+           isp_at<T, NU>(ax, isp, D, f.loc[D], I);
+}
+template <class T, bool M, bool NU>
+__device__ T dkdk(const DFL<T>& f, const DFL<T>& w, const DFL<T>& k, const Axes3& ax, const T* isp, int D, Ix3 I) {
+    Ix3 Il, Ir;
+    flip_knots(f.loc[D], D, I, Il, Ir);
+    int32_t floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
+    floc[D] = 1 - floc[D];
+    return (itpw<T, NU, false>(k, floc, ax, Ir) * ddk<T, M, NU>(f, w, ax, isp, D, Ir) -
+            itpw<T, NU, false>(k, floc, ax, Il) * ddk<T, M, NU>(f, w, ax, isp, D, Il)) *
+           isp_at<T, NU>(ax, isp, D, f.loc[D], I);
+}
 
-template <class T, bool M>
+template <class T, bool M, bool NU>
 __global__ void __launch_bounds__(256) op_ext_kernel(Box3 b, DFL<T> dst, DFL<T> s0, DFL<T> s1, DFL<T> s2, DFL<T> w0,
                                                      DFL<T> w1, DFL<T> w2, DFL<T> k, OpArgs a, T isp0, T isp1,
-                                                     T isp2) {
+                                                     T isp2, Axes3 ax) {
     bool ok;
     Ix3 I = box_index(b, ok);
     if (!ok) return;
@@ -183,31 +203,31 @@ __global__ void __launch_bounds__(256) op_ext_kernel(Box3 b, DFL<T> dst, DFL<T> 
     const T isp[3] = {isp0, isp1, isp2};
     T v = T(0);
     switch (a.op) {
-        case CHMY_OP_PARTIAL: v = ddk<T, M>(s0, w0, isp[a.D], a.D, I); break;
+        case CHMY_OP_PARTIAL: v = ddk<T, M, NU>(s0, w0, ax, isp, a.D, I); break;
         case CHMY_OP_DELTA: v = rightk<T, M>(s0, w0, a.D, I) - leftk<T, M>(s0, w0, a.D, I); break;
         case CHMY_OP_LEFT: v = leftk<T, M>(s0, w0, a.D, I); break;
         case CHMY_OP_RIGHT: v = rightk<T, M>(s0, w0, a.D, I); break;
         case CHMY_OP_DIVG: {
-            v = ddk<T, M>(*s[0], *w[0], isp[0], 0, I);
-            for (int d = 1; d < a.nd; ++d) v = v + ddk<T, M>(*s[d], *w[d], isp[d], d, I);
+            v = ddk<T, M, NU>(*s[0], *w[0], ax, isp, 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + ddk<T, M, NU>(*s[d], *w[d], ax, isp, d, I);
         } break;
-        case CHMY_OP_D2: v = d2k<T, M>(s0, w0, isp[a.D], a.D, I); break;
-        case CHMY_OP_DKD: v = dkdk<T, M>(s0, w0, k, isp[a.D], a.D, I); break;
+        case CHMY_OP_D2: v = d2k<T, M, NU>(s0, w0, ax, isp, a.D, I); break;
+        case CHMY_OP_DKD: v = dkdk<T, M, NU>(s0, w0, k, ax, isp, a.D, I); break;
         case CHMY_OP_LAPL: {
-            v = d2k<T, M>(s0, w0, isp[0], 0, I);
-            for (int d = 1; d < a.nd; ++d) v = v + d2k<T, M>(s0, w0, isp[d], d, I);
+            v = d2k<T, M, NU>(s0, w0, ax, isp, 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + d2k<T, M, NU>(s0, w0, ax, isp, d, I);
         } break;
         case CHMY_OP_DIVG_GRAD: {
-            v = dkdk<T, M>(s0, w0, k, isp[0], 0, I);
-            for (int d = 1; d < a.nd; ++d) v = v + dkdk<T, M>(s0, w0, k, isp[d], d, I);
+            v = dkdk<T, M, NU>(s0, w0, k, ax, isp, 0, I);
+            for (int d = 1; d < a.nd; ++d) v = v + dkdk<T, M, NU>(s0, w0, k, ax, isp, d, I);
         } break;
-        case CHMY_OP_LERP: v = lerpv(s0, dst.loc, I); break;
-        case CHMY_OP_HLERP: v = hlerpv(s0, dst.loc, I); break;
+        case CHMY_OP_LERP: v = itpw<T, NU, false>(s0, dst.loc, ax, I); break;
+        case CHMY_OP_HLERP: v = itpw<T, NU, true>(s0, dst.loc, ax, I); break;
         case CHMY_OP_VMAG: {
             const int32_t c[3] = {0, 0, 0};
-            T x = lerpv(*s[0], c, I);
+            T x = itpw<T, NU, false>(*s[0], c, ax, I);
             T acc = x * x;
-            for (int d = 1; d < a.nd; ++d) { T y = lerpv(*s[d], c, I); acc = acc + y * y; }
+            for (int d = 1; d < a.nd; ++d) { T y = itpw<T, NU, false>(*s[d], c, ax, I); acc = acc + y * y; }
             v = sqrt(acc);
         } break;
     }
@@ -518,7 +538,8 @@ extern "C" {
 
 int chmy_op(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, int nsrc,
             const chmy_field_t* src, int dim, void* stream) {
-    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_grid_any(g));
+    if (nonuniform(g)) return chmy_op_masked(op, g, box, dst, nsrc, src, nullptr, nullptr, dim, stream);
     chmy_box_t bx;
     CHMY_TRY(check_box(g, box, &bx));
     CHMY_TRY(check_field(g, dst, "dst", nullptr, -1));
@@ -562,7 +583,7 @@ int chmy_op(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_fiel
 
 int chmy_op_masked(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t* dst, int nsrc,
                    const chmy_field_t* src, const chmy_field_t* masks, const chmy_field_t* k, int dim, void* stream) {
-    CHMY_TRY(check_grid(g));
+    CHMY_TRY(check_grid_any(g));
     chmy_box_t bx;
     CHMY_TRY(check_box(g, box, &bx));
     CHMY_TRY(check_field(g, dst, "dst", nullptr, -1));
@@ -605,10 +626,21 @@ int chmy_op_masked(int op, const chmy_grid_t* g, const chmy_box_t* box, const ch
         DFL<T> w1 = masks && nsrc > 1 ? lview<T>(masks[1]) : w0, w2 = masks && nsrc > 2 ? lview<T>(masks[2]) : w0;
         DFL<T> kk = k ? lview<T>(*k) : s0;
         const T i0 = (T)g->inv_spacing[0], i1 = (T)g->inv_spacing[1], i2 = (T)g->inv_spacing[2];
-        if (masks)
-            hipLaunchKernelGGL((op_ext_kernel<T, true>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a, i0, i1, i2);
-        else
-            hipLaunchKernelGGL((op_ext_kernel<T, false>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a, i0, i1, i2);
+        const Axes3 ax = axes_of(g);
+        if (nonuniform(g)) {
+            if (masks)
+                hipLaunchKernelGGL((op_ext_kernel<T, true, true>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a,
+                                   i0, i1, i2, ax);
+            else
+                hipLaunchKernelGGL((op_ext_kernel<T, false, true>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a,
+                                   i0, i1, i2, ax);
+        } else if (masks) {
+            hipLaunchKernelGGL((op_ext_kernel<T, true, false>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a,
+                               i0, i1, i2, ax);
+        } else {
+            hipLaunchKernelGGL((op_ext_kernel<T, false, false>), grd, blk2(), 0, s, b, d, s0, s1, s2, w0, w1, w2, kk, a,
+                               i0, i1, i2, ax);
+        }
     };
     if (dt == CHMY_F64) run(double{});
     else run(float{});

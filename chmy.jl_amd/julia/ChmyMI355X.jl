@@ -71,6 +71,7 @@ struct ChmyGrid
     extent::NTuple{3,Float64}
     spacing::NTuple{3,Float64}
     inv_spacing::NTuple{3,Float64}
+    axis::NTuple{3,Ptr{Float64}}
 end
 
 struct ChmyBox
@@ -237,13 +238,63 @@ function desc(f::MI355XField{T,N}) where {T,N}
                      pad3(Int32.(h), Int32(0)), pad3(map(loccode, location(f)), Int32(0)))
 end
 
+const NOAXIS = Ptr{Float64}(C_NULL)
+
 function desc(g::UniformGrid{N}) where {N}
     ax = g.axes
     return ChmyGrid(Int32(N), Int32(0), pad3(ntuple(d -> Int64(ax[d].length), Val(N)), Int64(0)),
                     pad3(ntuple(d -> Float64(ax[d].origin), Val(N)), 0.0),
                     pad3(ntuple(d -> Float64(ax[d].extent), Val(N)), 0.0),
                     pad3(ntuple(d -> Float64(ax[d].spacing), Val(N)), 0.0),
-                    pad3(ntuple(d -> Float64(ax[d].inv_spacing), Val(N)), 0.0))
+                    pad3(ntuple(d -> Float64(ax[d].inv_spacing), Val(N)), 0.0),
+                    (NOAXIS, NOAXIS, NOAXIS))
+end
+
+# FunctionAxis (src/Grids/function_axis.jl): the metric table chmy_grid_t.axis
+# points to, built once per axis from vertex(ax, i) with Chmy's own formulas
+# (abstract_axis.jl:18,26-29, interpolation.jl:29) and kept on the device for
+# as long as the axis lives.
+const AXIS_PAD = 4
+const AXIS_ROWS = 7
+const AXIS_TABLES = IdDict{Any,MI355XArray{Float64,2}}()
+
+function axis_table(ax::Chmy.Grids.FunctionAxis)
+    get!(AXIS_TABLES, ax) do
+        n = ax.length
+        is = (-AXIS_PAD):(n + 1 + AXIS_PAD)
+        v(i) = Float64(Chmy.Grids.vertex(ax, i))
+        c(i) = 0.5 * (v(i) + v(i + 1))
+        dc(i) = v(i + 1) - v(i)
+        dv(i) = c(i) - c(i - 1)
+        t = Matrix{Float64}(undef, length(is), AXIS_ROWS)   # column r+1 = row r of the C table
+        for (q, i) in enumerate(is)
+            t[q, 1] = v(i)
+            t[q, 2] = c(i)
+            t[q, 3] = dc(i)
+            t[q, 4] = dv(i)
+            t[q, 5] = inv(dc(i))
+            t[q, 6] = inv(dv(i))
+            t[q, 7] = 0.5 * dc(i - 1) * inv(dv(i))
+        end
+        dev = MI355XArray{Float64}(undef, size(t))
+        copyto!(dev, t)
+        dev
+    end
+end
+axis_ptr(::Chmy.Grids.UniformAxis) = NOAXIS
+axis_ptr(ax::Chmy.Grids.FunctionAxis) = pointer(axis_table(ax))
+_uni(ax::Chmy.Grids.UniformAxis, f) = Float64(getfield(ax, f))
+_uni(::Chmy.Grids.FunctionAxis, f) = 0.0
+
+# a StructuredGrid with FunctionAxis dims: accepted by the operators, bc! and set!
+function desc(g::Chmy.Grids.StructuredGrid{N}) where {N}
+    ax = g.axes
+    return ChmyGrid(Int32(N), Int32(0), pad3(ntuple(d -> Int64(ax[d].length), Val(N)), Int64(0)),
+                    pad3(ntuple(d -> Float64(Chmy.Grids.origin(ax[d], Vertex())), Val(N)), 0.0),
+                    pad3(ntuple(d -> Float64(Chmy.Grids.extent(ax[d], Vertex())), Val(N)), 0.0),
+                    pad3(ntuple(d -> _uni(ax[d], :spacing), Val(N)), 0.0),
+                    pad3(ntuple(d -> _uni(ax[d], :inv_spacing), Val(N)), 0.0),
+                    pad3(ntuple(d -> axis_ptr(ax[d]), Val(N)), NOAXIS))
 end
 
 # a grid descriptor for kernels launched without the grid (update_old!): the
@@ -252,7 +303,7 @@ function center_grid(f::MI355XField{T,N}) where {T,N}
     n = pad3(Int64.(size(f)), Int64(0))
     o = pad3(ntuple(_ -> 0.0, Val(N)), 0.0)
     one3 = pad3(ntuple(_ -> 1.0, Val(N)), 0.0)
-    return ChmyGrid(Int32(N), Int32(0), n, o, Float64.(n), one3, one3)
+    return ChmyGrid(Int32(N), Int32(0), n, o, Float64.(n), one3, one3, (NOAXIS, NOAXIS, NOAXIS))
 end
 
 descs(fs::NamedTuple) = ChmyField[desc(f) for f in values(fs)]

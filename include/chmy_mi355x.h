@@ -62,8 +62,27 @@ typedef struct {
   int32_t loc[3];    /* CHMY_CENTER / CHMY_VERTEX per dim */
 } chmy_field_t;
 
-/* UniformGrid (src/Grids/structured_grid.jl:27-39, uniform_axis.jl:7-12):
- * spacing = extent/n, inv_spacing = 1/spacing, as the host computed them. */
+/* StructuredGrid (src/Grids/structured_grid.jl:5-9).  A dim with axis[d] ==
+ * NULL is a UniformAxis (uniform_axis.jl:1-11, UniformGrid
+ * structured_grid.jl:27-39): spacing = extent/n, inv_spacing = 1/spacing, as
+ * the host computed them.  A dim with axis[d] != NULL is a FunctionAxis
+ * (function_axis.jl:1-9): axis[d] is a device table of CHMY_AXIS_ROWS rows of
+ * n[d] + 2 + 2*CHMY_AXIS_PAD doubles; entry (row, i) of logical index i in
+ * [-CHMY_AXIS_PAD, n[d] + 1 + CHMY_AXIS_PAD] sits at
+ * axis[d][row * (n[d] + 2 + 2*CHMY_AXIS_PAD) + i + CHMY_AXIS_PAD]:
+ *   CHMY_AX_VERTEX   vertex(i) = f(i)
+ *   CHMY_AX_CENTER   center(i) = 0.5 * (vertex(i) + vertex(i+1))   (abstract_axis.jl:18)
+ *   CHMY_AX_DC       spacing(Center, i) = vertex(i+1) - vertex(i)  (abstract_axis.jl:26)
+ *   CHMY_AX_DV       spacing(Vertex, i) = center(i) - center(i-1)  (abstract_axis.jl:27)
+ *   CHMY_AX_IDC, CHMY_AX_IDV   inv(...) of the two                 (abstract_axis.jl:29)
+ *   CHMY_AX_WCV      the Center -> Vertex lerp weight at vertex i,
+ *                    0.5 * spacing(Center, i-1) * inv_spacing(Vertex, i) (interpolation.jl:29)
+ * Grids with a FunctionAxis are accepted by chmy_op, chmy_op_masked,
+ * chmy_bc_side, chmy_bc_batchset and chmy_set_program; every other entry
+ * point returns CHMY_E_ARG for them. */
+enum { CHMY_AXIS_PAD = 4, CHMY_AXIS_ROWS = 7 };
+enum { CHMY_AX_VERTEX = 0, CHMY_AX_CENTER = 1, CHMY_AX_DC = 2, CHMY_AX_DV = 3, CHMY_AX_IDC = 4, CHMY_AX_IDV = 5,
+       CHMY_AX_WCV = 6 };
 typedef struct {
   int32_t nd;
   int32_t reserved;
@@ -72,6 +91,7 @@ typedef struct {
   double  extent[3];
   double  spacing[3];
   double  inv_spacing[3];
+  const double* axis[3];   /* FunctionAxis metric tables (device), NULL = uniform */
 } chmy_grid_t;
 
 typedef struct { int64_t lo[3]; int64_t len[3]; } chmy_box_t;
@@ -110,7 +130,8 @@ int chmy_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int chmy_stream_sync(void* stream);
 
 /* UniformAxis coord (uniform_axis.jl:18-19): out[q] = coord(ax, loc, first+q),
- * vertex(i) = fma(i-1, Δ, o), center(i) = fma(i-1, Δ, fma(0.5, Δ, o)).  Host. */
+ * vertex(i) = fma(i-1, Δ, o), center(i) = fma(i-1, Δ, fma(0.5, Δ, o)).  Host;
+ * uniform dims only (a FunctionAxis's coordinates are its table's rows). */
 int chmy_coords(const chmy_grid_t* grid, int dim, int loc, int64_t first, int64_t count, double* out);
 
 /* ---------------------------------------------------------------- bc! */
@@ -409,7 +430,9 @@ int chmy_sia_reverse_noalloc(const chmy_grid_t* grid, const chmy_field_t* ckpt, 
  *   COORD a, b, c      coord(grid, loc b, index + c) along dim a: vertex
  *                      fma(i-1, Δ, o), centre fma(i-1, Δ, fma(0.5, Δ, o))
  *                      (uniform_axis.jl:18-19)
- *   SPACING a, INV_SPACING a   Δ and 1/Δ along dim a
+ *   SPACING a, b, c    spacing(grid, loc b, index + c) along dim a (uniform: Δ);
+ *   INV_SPACING a, b, c    its inverse.  On a FunctionAxis COORD and SPACING
+ *                      read the axis table and index + c must stay in its range
  *   LOAD a, b          srcs[a] at the point's logical index plus the offsets
  *                      packed in b (one signed byte per dim, x lowest); every
  *                      point of the box plus the offsets must lie in the

@@ -49,6 +49,9 @@ typedef struct {
     double  extent[3];
     double  spacing[3];
     double  inv_spacing[3];
+    // FunctionAxis (src/Grids/function_axis.jl:1-9): host array of vertex(i)
+    // for i = -kPad-1 .. n+2+kPad at axis[d][i + kPad + 1], or NULL (uniform)
+    const double* axis[3];
 } orc_grid;
 
 typedef struct { int64_t lo[3]; int64_t len[3]; } orc_box;
@@ -99,7 +102,12 @@ enum Plant {
     PL_MASK_AT_OUT = 23,     // masked left/right read the mask at the output index, not the knot
     PL_HLERP_ARITH = 24,     // hlerp with the arithmetic rule muladd(t, b - a, a)
     PL_DELTA_SIGN = 25,      // δ = left - right
+    PL_NU_DIFF_LOC = 26,     // FunctionAxis ∂ scaled by iΔ at the output location instead of the source's
+    PL_NU_WEIGHT = 27,       // FunctionAxis Center->Vertex lerp weight from Δc(i) instead of Δc(i-1)
+    PL_NU_DIRICHLET = 28,    // FunctionAxis Center Dirichlet t = 2 Δc(I_f) iΔv(I_f2) (indices swapped)
+    PL_NU_NEUMANN = 29,      // FunctionAxis Neumann h = Δ(loc) instead of Δ(flip(loc))
 };
+constexpr long kPad = 4;   // FunctionAxis arrays reach kPad indices past the halo
 int g_plant = 0;
 
 void fail(const char* m) {
@@ -153,13 +161,37 @@ struct G {
     int nd;
     long n[3];
     T sp[3], isp[3];
+    const double* xv[3];
     explicit G(const orc_grid* g) {
         nd = g->nd;
         for (int d = 0; d < 3; ++d) {
             n[d] = d < nd ? g->n[d] : 1;
             sp[d] = T(g->spacing[d]);
             isp[d] = T(g->inv_spacing[d]);
+            xv[d] = d < nd ? g->axis[d] : nullptr;
         }
+    }
+    // FunctionAxis metrics from the vertex function's values
+    // (src/Grids/abstract_axis.jl:18,26-29): center(i) = 0.5 (vertex(i) + vertex(i+1)),
+    // spacing(Center, i) = vertex(i+1) - vertex(i), spacing(Vertex, i) =
+    // center(i) - center(i-1), inv_spacing = inv(spacing).
+    double vert(int d, long i) const {
+        if (i < -kPad - 1 || i > n[d] + 2 + kPad) fail("oracle: FunctionAxis index outside its array");
+        return xv[d][i + kPad + 1];
+    }
+    double cent(int d, long i) const { return 0.5 * (vert(d, i) + vert(d, i + 1)); }
+    double dC(int d, long i) const { return vert(d, i + 1) - vert(d, i); }
+    double dV(int d, long i) const { return cent(d, i) - cent(d, i - 1); }
+    // iΔ(grid, loc, D, I) along D at index i (uniform: the grid's inv_spacing)
+    T ispat(int D, int loc, long i) const {
+        if (!xv[D]) return isp[D];
+        return T(1.0 / (loc == 0 ? dC(D, i) : dV(D, i)));
+    }
+    // itp_weight(ax, Center, Vertex, i) = 0.5 Δ(Center, i-1) iΔ(Vertex, i)
+    // (interpolation.jl:29); 0.5 on a uniform axis (:32) and Vertex -> Center (:28)
+    T wcv(int D, long i) const {
+        if (!xv[D]) return T(0.5);
+        return T((0.5 * dC(D, g_plant == PL_NU_WEIGHT ? i : i - 1)) * (1.0 / dV(D, i)));
     }
 };
 
@@ -172,50 +204,60 @@ template <class T> inline T rightv(const FV<T>& f, int D, const Ix& I) {
     return f.loc[D] == 0 ? f(I) : f(shift(I, D, +1));
 }
 // ∂ = δ * iΔ (src/GridOperators/partial_derivatives.jl:2-5)
+// iΔ(grid, loc, dim, I) with loc the source field's location, at the output
+// index I (partial_derivatives.jl:5, field_operators.jl:20-24)
+template <class T> inline T isp_of(const FV<T>& f, const G<T>& g, int D, const Ix& I) {
+    const int loc = g.xv[D] && g_plant == PL_NU_DIFF_LOC ? 1 - f.loc[D] : f.loc[D];
+    return g.ispat(D, loc, I.v[D]);
+}
 template <class T> inline T dd(const FV<T>& f, const G<T>& g, int D, const Ix& I) {
     if (g_plant == PL_VERTEX_DIFF && f.loc[D] == 1) return (f(I) - f(shift(I, D, -1))) * g.isp[D];
-    return (rightv(f, D, I) - leftv(f, D, I)) * g.isp[D];
+    return (rightv(f, D, I) - leftv(f, D, I)) * isp_of(f, g, D, I);
 }
 
 // lerp (src/GridOperators/interpolation.jl:14-94): for every dim where the
 // location differs, knots (I-e, I) for Center->Vertex and (I, I+e) for
 // Vertex->Center, rule fma(0.5, b-a, a), last differing dim outermost.
+// With a grid g, the weight of a Center->Vertex dim is g's (G::wcv).
 template <class T>
-T itp_rec(const FV<T>& f, const int* dims, int level, const int* to, const Ix& I) {
+T itp_rec(const FV<T>& f, const int* dims, int level, const int* to, const Ix& I, const G<T>* g) {
     if (level == 0) return f(I);
     int D = dims[level - 1];
     Ix Il = I, Ir = I;
     if (f.loc[D] == 0 && to[D] == 1 && g_plant != PL_LERP_KNOTS) { Il = shift(I, D, -1); }
     else { Ir = shift(I, D, +1); }
-    T a = itp_rec(f, dims, level - 1, to, Il);
-    T b = itp_rec(f, dims, level - 1, to, Ir);
-    return std::fma(T(0.5), b - a, a);
+    T a = itp_rec(f, dims, level - 1, to, Il, g);
+    T b = itp_rec(f, dims, level - 1, to, Ir, g);
+    const T t = g && f.loc[D] == 0 && to[D] == 1 ? g->wcv(D, I.v[D]) : T(0.5);
+    return std::fma(t, b - a, a);
 }
-template <class T> T lerp(const FV<T>& f, const int* to, const Ix& I) {
+template <class T> T lerp(const FV<T>& f, const int* to, const Ix& I, const G<T>* g = nullptr) {
     int dims[3], m = 0;
     for (int d = 0; d < f.nd; ++d)
         if (f.loc[d] != to[d]) dims[m++] = d;
     if (m == 0) return f(I);
-    return itp_rec(f, dims, m, to, I);
+    return itp_rec(f, dims, m, to, I, g);
 }
-template <class T> T hlerp_rec(const FV<T>& f, const int* dims, int level, const int* to, const Ix& I) {
+template <class T> T hlerp_rec(const FV<T>& f, const int* dims, int level, const int* to, const Ix& I,
+                                const G<T>* g) {
     if (level == 0) return f(I);
     int D = dims[level - 1];
     Ix Il = I, Ir = I;
     if (f.loc[D] == 0 && to[D] == 1) { Il = shift(I, D, -1); }
     else { Ir = shift(I, D, +1); }
-    T a = hlerp_rec(f, dims, level - 1, to, Il);
-    T b = hlerp_rec(f, dims, level - 1, to, Ir);
+    T a = hlerp_rec(f, dims, level - 1, to, Il, g);
+    T b = hlerp_rec(f, dims, level - 1, to, Ir, g);
+    const T t = g && f.loc[D] == 0 && to[D] == 1 ? g->wcv(D, I.v[D]) : T(0.5);
     // itp_rule(::HarmonicLinear, t, a, b) = inv(muladd(t, inv(b) - inv(a), inv(a)))
-    if (g_plant == PL_HLERP_ARITH) return std::fma(T(0.5), b - a, a);
-    return T(1) / std::fma(T(0.5), T(1) / b - T(1) / a, T(1) / a);
+    if (g_plant == PL_HLERP_ARITH) return std::fma(t, b - a, a);
+    return T(1) / std::fma(t, T(1) / b - T(1) / a, T(1) / a);
 }
-template <class T> T hlerp(const FV<T>& f, const int* to, const Ix& I) {
+template <class T> T hlerp(const FV<T>& f, const int* to, const Ix& I, const G<T>* g = nullptr) {
     int dims[3], m = 0;
     for (int d = 0; d < f.nd; ++d)
         if (f.loc[d] != to[d]) dims[m++] = d;
     if (m == 0) return f(I);
-    return hlerp_rec(f, dims, m, to, I);
+    return hlerp_rec(f, dims, m, to, I, g);
 }
 
 // divg: left-associated Σ ∂_D V_D (src/GridOperators/field_operators.jl:50-59)
@@ -227,9 +269,9 @@ template <class T> T divg(const FV<T>* V, const G<T>& g, const Ix& I) {
 // vmag = sqrt(Σ lerp(V_D, Center)^2) (field_operators.jl:116-125)
 template <class T> T vmag(const FV<T>* V, const G<T>& g, const Ix& I) {
     const int c[3] = {0, 0, 0};
-    T a = lerp(V[0], c, I);
+    T a = lerp(V[0], c, I, &g);
     T s = a * a;
-    for (int d = 1; d < g.nd; ++d) { T b = lerp(V[d], c, I); s = s + b * b; }
+    for (int d = 1; d < g.nd; ++d) { T b = lerp(V[d], c, I, &g); s = s + b * b; }
     return g_plant == PL_VMAG ? s : std::sqrt(s);
 }
 // ∂² (partial_derivatives.jl:7-12): from = loc = location(f).
@@ -238,7 +280,7 @@ template <class T> T d2(const FV<T>& f, const G<T>& g, int D, const Ix& I) {
     Ix Ir = I, Il = I;
     if (f.loc[D] == 0) { Ir = shift(I, D, +1); }   // ir(Vertex, Center) = I+e, il = I
     else { Il = shift(I, D, -1); }                 // ir(Center, Vertex) = I,   il = I-e
-    return (dd(f, g, D, Ir) - dd(f, g, D, Il)) * g.isp[D];
+    return (dd(f, g, D, Ir) - dd(f, g, D, Il)) * isp_of(f, g, D, I);
 }
 // ∂k∂ (partial_derivatives.jl:14-21)
 template <class T> T dkd(const FV<T>& f, const FV<T>& k, const G<T>& g, int D, const Ix& I) {
@@ -246,7 +288,7 @@ template <class T> T dkd(const FV<T>& f, const FV<T>& k, const G<T>& g, int D, c
     if (f.loc[D] == 0) { Ir = shift(I, D, +1); } else { Il = shift(I, D, -1); }
     int floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
     floc[D] = 1 - floc[D];
-    return (lerp(k, floc, Ir) * dd(f, g, D, Ir) - lerp(k, floc, Il) * dd(f, g, D, Il)) * g.isp[D];
+    return (lerp(k, floc, Ir, &g) * dd(f, g, D, Ir) - lerp(k, floc, Il, &g) * dd(f, g, D, Il)) * isp_of(f, g, D, I);
 }
 
 // Julia max/min against 0.0 for floats (NaN propagates, max(-0.0,0.0)=0.0).
@@ -300,14 +342,23 @@ void bc_one(const G<T>& g, int D, int side, const FV<T>& f, const orc_bc& bc, Ix
     }
     If.v[D] = side == 1 ? 0 : m + 1;              // ghost
     Ix If2 = If; If2.v[D] = side == 1 ? 1 : m;    // neighbour
+    const long ig = If.v[D], in = If2.v[D];
     if (bc.kind == 0) {                           // Center Dirichlet, t = 2 on uniform grids
         T a = value(If);
         T b = f(If2);
-        f(If) = std::fma(g_plant == PL_CENTER_DIRICHLET ? T(1) : T(2), a - b, b);
-    } else {                                      // Neumann, any location
+        T t = g_plant == PL_CENTER_DIRICHLET ? T(1) : T(2);
+        if (g.xv[D])                              // t = 2 Δ(Center, I_f2) iΔ(Vertex, I_f) (:64-66)
+            t = g_plant == PL_NU_DIRICHLET ? T((2.0 * g.dC(D, ig)) * (1.0 / g.dV(D, in)))
+                                           : T((2.0 * g.dC(D, in)) * (1.0 / g.dV(D, ig)));
+        f(If) = std::fma(t, a - b, b);
+    } else {                                      // Neumann, any location: h = Δ(grid, flip(loc), dim, I_f)
         T q = value(If);
         T qs = side == 1 && g_plant != PL_NEUMANN_SIGN ? -q : q;
         T h = g.sp[D];
+        if (g.xv[D]) {
+            const int fl = g_plant == PL_NU_NEUMANN ? f.loc[D] : 1 - f.loc[D];
+            h = T(fl == 0 ? g.dC(D, ig) : g.dV(D, ig));
+        }
         f(If) = std::fma(h, qs, f(If2));
     }
 }
@@ -737,7 +788,7 @@ template <class T> void op_t(int op, const orc_grid* gg, const orc_box* b, const
         T v = 0;
         switch (op) {
             case 0: v = dd(s[0], g, D, I); break;
-            case 1: v = lerp(s[0], d.loc, I); break;
+            case 1: v = lerp(s[0], d.loc, I, &g); break;
             case 2: v = divg(s, g, I); break;
             case 3: v = vmag(s, g, I); break;
             case 4: v = d2(s[0], g, D, I); break;
@@ -746,7 +797,7 @@ template <class T> void op_t(int op, const orc_grid* gg, const orc_box* b, const
             case 7: { v = dkd(s[0], kk, g, 0, I); for (int q = 1; q < g.nd; ++q) v = v + dkd(s[0], kk, g, q, I); } break;
             case 8: v = leftv(s[0], D, I); break;
             case 9: v = rightv(s[0], D, I); break;
-            case 10: v = hlerp(s[0], d.loc, I); break;
+            case 10: v = hlerp(s[0], d.loc, I, &g); break;
             case 11:   // δ (partial_derivatives.jl:2)
                 v = g_plant == PL_DELTA_SIGN ? leftv(s[0], D, I) - rightv(s[0], D, I) : rightv(s[0], D, I) - leftv(s[0], D, I);
                 break;
@@ -775,19 +826,20 @@ template <class T> inline T deltam(const FV<T>& f, const FV<T>& w, int D, const 
     return rightm(f, w, D, I) - leftm(f, w, D, I);
 }
 template <class T> inline T ddm(const FV<T>& f, const FV<T>& w, const G<T>& g, int D, const Ix& I) {
-    return deltam(f, w, D, I) * g.isp[D];
+    return deltam(f, w, D, I) * isp_of(f, g, D, I);
 }
 template <class T> T d2m(const FV<T>& f, const FV<T>& w, const G<T>& g, int D, const Ix& I) {
     Ix Ir = I, Il = I;
     if (f.loc[D] == 0) { Ir = shift(I, D, +1); } else { Il = shift(I, D, -1); }
-    return (ddm(f, w, g, D, Ir) - ddm(f, w, g, D, Il)) * g.isp[D];
+    return (ddm(f, w, g, D, Ir) - ddm(f, w, g, D, Il)) * isp_of(f, g, D, I);
 }
 template <class T> T dkdm(const FV<T>& f, const FV<T>& w, const FV<T>& k, const G<T>& g, int D, const Ix& I) {
     Ix Ir = I, Il = I;
     if (f.loc[D] == 0) { Ir = shift(I, D, +1); } else { Il = shift(I, D, -1); }
     int floc[3] = {f.loc[0], f.loc[1], f.loc[2]};
     floc[D] = 1 - floc[D];
-    return (lerp(k, floc, Ir) * ddm(f, w, g, D, Ir) - lerp(k, floc, Il) * ddm(f, w, g, D, Il)) * g.isp[D];
+    return (lerp(k, floc, Ir, &g) * ddm(f, w, g, D, Ir) - lerp(k, floc, Il, &g) * ddm(f, w, g, D, Il)) *
+           isp_of(f, g, D, I);
 }
 
 // op as op_t (0 ∂, 2 divg, 4 ∂², 5 ∂k∂, 6 lapl, 7 divg_grad, 8 left, 9 right, 11 δ);
@@ -929,6 +981,14 @@ double orc_center(double origin, double spacing, long i) {
 void orc_coords(double origin, double spacing, int loc, long first, long count, double* out) {
     for (long q = 0; q < count; ++q)
         out[q] = loc == 0 ? orc_center(origin, spacing, first + q) : orc_vertex(origin, spacing, first + q);
+}
+// FunctionAxis coordinates (function_axis.jl:9, abstract_axis.jl:18,20-21): xv holds
+// vertex(i) for i = -kPad-1 .. at xv[i + kPad + 1]
+void orc_axis_coords(const double* xv, int loc, long first, long count, double* out) {
+    for (long q = 0; q < count; ++q) {
+        const long i = first + q + kPad + 1;
+        out[q] = loc == 0 ? 0.5 * (xv[i] + xv[i + 1]) : xv[i];
+    }
 }
 float orc_vertex_f32(float origin, float spacing, long i) { return std::fma(float(i - 1), spacing, origin); }
 float orc_center_f32(float origin, float spacing, long i) {

@@ -44,7 +44,11 @@ class OField(C.Structure):
 class OGrid(C.Structure):
     _fields_ = [("nd", C.c_int32), ("reserved", C.c_int32), ("n", C.c_int64 * 3),
                 ("origin", C.c_double * 3), ("extent", C.c_double * 3),
-                ("spacing", C.c_double * 3), ("inv_spacing", C.c_double * 3)]
+                ("spacing", C.c_double * 3), ("inv_spacing", C.c_double * 3),
+                ("axis", C.c_void_p * 3)]
+
+
+AXIS_PAD = 4   # kPad in chmy_oracle.cpp
 
 
 class OBox(C.Structure):
@@ -69,6 +73,7 @@ def _declare(L):
     L.orc_center.restype = C.c_double
     L.orc_center.argtypes = [C.c_double, C.c_double, C.c_long]
     L.orc_coords.argtypes = [C.c_double, C.c_double, C.c_int, C.c_long, C.c_long, P(C.c_double)]
+    L.orc_axis_coords.argtypes = [P(C.c_double), C.c_int, C.c_long, C.c_long, P(C.c_double)]
     L.orc_bc_side.argtypes = [P(OGrid), C.c_int, C.c_int, C.c_int, C.c_int, P(OField), P(OBC)]
     L.orc_bc_set.argtypes = [P(OGrid), C.c_int, P(C.c_int), P(OField), P(OBC)]
     L.orc_compute_q.argtypes = [P(OGrid), P(OBox), P(OField), P(OField), C.c_double]
@@ -122,7 +127,8 @@ PLANTS = {
     "diff_flux": 6, "stress_dev": 7, "edge_swap": 8, "rv_swap": 9, "vel_ml_side": 10, "maxloc_2n": 11,
     "thermal_upwind": 12, "sia_exp": 13, "sia_grad": 14, "sia_m": 15, "resid_rp": 16, "pr_update": 17,
     "grad_edge": 18, "pt_params": 19, "vmag": 20, "divg": 21, "thermal_sign": 22, "mask_at_out": 23,
-    "hlerp_arith": 24, "delta_sign": 25,
+    "hlerp_arith": 24, "delta_sign": 25, "nu_diff_loc": 26, "nu_weight": 27, "nu_dirichlet": 28,
+    "nu_neumann": 29,
 }
 
 
@@ -146,15 +152,25 @@ def _plant():
 
 
 class Grid:
-    """UniformGrid restated (src/Grids/uniform_axis.jl:7-12, structured_grid.jl:27-39)."""
+    """UniformGrid restated (src/Grids/uniform_axis.jl:7-12, structured_grid.jl:27-39).
 
-    def __init__(self, origin, extent, dims):
+    vertex_funcs[d] (optional) makes dim d a FunctionAxis
+    (function_axis.jl:1-9): vertex(i) = f(i), evaluated here for
+    i = -AXIS_PAD-1 .. n+2+AXIS_PAD; the oracle derives centres, spacings and
+    weights from those values itself."""
+
+    def __init__(self, origin, extent, dims, vertex_funcs=None):
         self.nd = len(dims)
         self.origin = tuple(float(o) for o in origin)
         self.extent = tuple(float(e) for e in extent)
         self.dims = tuple(int(d) for d in dims)
         self.spacing = tuple(e / n for e, n in zip(self.extent, self.dims))
         self.inv_spacing = tuple(1.0 / s for s in self.spacing)
+        self.xv = [None] * self.nd
+        for d, fn in enumerate(vertex_funcs or [None] * self.nd):
+            if fn is not None:
+                idx = range(-AXIS_PAD - 1, self.dims[d] + 3 + AXIS_PAD)
+                self.xv[d] = np.array([float(fn(i)) for i in idx], dtype=np.float64)
 
     def size(self, loc):
         loc = _locs(loc, self.nd)
@@ -169,14 +185,21 @@ class Grid:
             g.extent[d] = self.extent[d]
             g.spacing[d] = self.spacing[d]
             g.inv_spacing[d] = self.inv_spacing[d]
+            if self.xv[d] is not None:
+                g.axis[d] = self.xv[d].ctypes.data
         return g
 
     def coords(self, loc, dim, first=1, count=None):
-        """coord(ax, loc, i) for i in first:first+count-1 (uniform_axis.jl:18-19)."""
+        """coord(ax, loc, i) for i in first:first+count-1 (uniform_axis.jl:18-19;
+        a FunctionAxis: abstract_axis.jl:18,20-21)."""
         if count is None:
             count = self.size(loc)[dim] if isinstance(loc, tuple) else self.dims[dim] + loc
         lc = loc[dim] if isinstance(loc, tuple) else loc
         out = np.empty(count, np.float64)
+        if self.xv[dim] is not None:
+            lib().orc_axis_coords(self.xv[dim].ctypes.data_as(P(C.c_double)), lc, first, count,
+                                  out.ctypes.data_as(P(C.c_double)))
+            return out
         lib().orc_coords(self.origin[dim], self.spacing[dim], lc, first, count,
                          out.ctypes.data_as(P(C.c_double)))
         return out

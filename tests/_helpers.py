@@ -13,7 +13,11 @@ def locs_codes(f):
 
 
 def ogrid(grid):
-    return O.Grid(grid.origin, grid.extent, grid.dims)
+    """The oracle's grid for a product grid; a FunctionAxis hands the oracle
+    its vertex function, which the oracle evaluates and derives from itself."""
+    axes = getattr(grid, "axes", None)
+    funcs = [getattr(a, "f", None) for a in axes] if axes else None
+    return O.Grid(grid.origin, grid.extent, grid.dims, vertex_funcs=funcs)
 
 
 def pair(arch, grid, og, loc, T=np.float64, fill=None, rng=None, lo=-1.0, hi=1.0):

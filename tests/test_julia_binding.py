@@ -206,7 +206,7 @@ def test_every_example_kernel_has_a_binding():
 
 
 C2JL_FIELD = {"void*": "Ptr{Cvoid}", "int32_t": "Int32", "int64_t": "Int64", "double": "Float64",
-              "const chmy_field_t*": "Ptr{ChmyField}"}
+              "const chmy_field_t*": "Ptr{ChmyField}", "const double*": "Ptr{Float64}"}
 
 
 @pytest.mark.parametrize("cname,jname", [("chmy_field_t", "ChmyField"), ("chmy_grid_t", "ChmyGrid"),

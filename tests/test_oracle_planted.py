@@ -9,6 +9,7 @@ import numpy as np
 import oracle as O
 import test_oracle_closed_forms as CF
 import test_oracle_masked as MK
+import test_oracle_nonuniform as NU
 import test_oracle_pins as PN
 import test_oracle_reference_units as RU
 import test_oracle_stokes_adjoint as SA
@@ -48,6 +49,11 @@ CASES = [
     ("hlerp_arith", PN.test_hlerp_is_the_harmonic_rule, (1,)),
     ("hlerp_arith", PN.test_hlerp_is_the_harmonic_rule, (3,)),
     ("delta_sign", PN.test_delta_is_the_partial_before_its_inverse_spacing, (2,)),
+    ("nu_diff_loc", NU.test_partial_scales_by_the_source_location_spacing, ()),
+    ("nu_weight", NU.test_lerp_reproduces_affine_and_bilinear_fields, (1,)),
+    ("nu_weight", NU.test_lerp_reproduces_affine_and_bilinear_fields, (2,)),
+    ("nu_dirichlet", NU.test_boundary_ghosts_use_the_reference_indices, ()),
+    ("nu_neumann", NU.test_boundary_ghosts_use_the_reference_indices, ()),
 ]
 
 
