@@ -993,6 +993,22 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
     __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW];   // the plane's stresses (A, then B)
     __shared__ T s_tyz[2][BY][TW], b_tyz[2][BY][TW];         // tau_yz at the plane and the next
     __shared__ T q_st[7][BY][TW];   // A's stresses of the previous step, one slot per thread (B's inputs)
+    // FP32: B's tau_old (and forcing, rho g) are the values A loaded one step
+    // earlier; they pass through per-thread LDS slots by step parity (bf16
+    // forcing as its 16 bits) instead of being loaded again.  FP64 has no
+    // LDS left for them; FP32 forcing has none unless it is bf16.
+    constexpr bool HO = sizeof(T) == 4 && (!HF || BF);
+    constexpr int NO = 6 + (HF ? 4 : RG ? 1 : 0);
+    using HS = typename std::conditional<BF, uint16_t, T>::type;
+    __shared__ HS q_old[HO ? 2 : 1][HO ? NO : 1][HO ? BY : 1][TW];
+    auto hpack = [](T v) -> HS {
+        if constexpr (BF) return (uint16_t)(__float_as_uint(v) >> 16);   // exact: v came from bf16
+        else return v;
+    };
+    auto hunpack = [](HS h) -> T {
+        if constexpr (BF) return bf16_to_f32(h);
+        else return h;
+    };
 
     const int tx = threadIdx.x;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -1225,6 +1241,15 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             s_tyy[ty][tx] = tyy;
             s_txy[ty][tx] = txy1;
             s_tyz[pa][ty][tx] = tyz1;
+            if constexpr (HO) {
+                q_old[pa][0][ty][tx] = hpack(oxx);
+                q_old[pa][1][ty][tx] = hpack(oyy);
+                q_old[pa][2][ty][tx] = hpack(ozz);
+                q_old[pa][3][ty][tx] = hpack(oxy);
+                q_old[pa][4][ty][tx] = hpack(oxz);
+                q_old[pa][5][ty][tx] = hpack(oyz);
+                if constexpr (HF) q_old[pa][6][ty][tx] = hpack(fP);
+            }
             const T txxl = xm1(txx), txzr = xp1(Atxz);
             __syncthreads();
             // A's V at plane s
@@ -1240,6 +1265,12 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 ry = ry + fVy;
                 rz = rz + fVz;
             }
+            if constexpr (HO && HF) {
+                q_old[pa][7][ty][tx] = hpack(fVx);
+                q_old[pa][8][ty][tx] = hpack(fVy);
+                q_old[pa][9][ty][tx] = hpack(fVz);
+            }
+            if constexpr (HO && RG) q_old[pa][6][ty][tx] = hpack(rgz);
             if constexpr (FVISC) {
                 ex = AR::vcoef_field(jmax(xm1(eml), eml), K);
                 ey = AR::vcoef_field(jmax(s_eml[tym][tx], eml), K);
@@ -1321,19 +1352,40 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             s_vx[ty][tx] = Bv[0];
             b_vy[pb ^ 1][ty][tx] = V1[1];
             s_vz1[ty][tx] = V1[2];
-            // tau_old (and forcing) of plane r: o[] sits at plane s
+            // tau_old (and forcing) of plane r (Center, xy) and r+1 (xz, yz): o[] sits at
+            // plane s; with HO the slots A filled one step earlier
             const uint32_t oC = o[LC] - dz[LC];
-            const T oxx = ldf<T, BF>(a.Oxx, oC), oyy = ldf<T, BF>(a.Oyy, oC), ozz = ldf<T, BF>(a.Ozz, oC);
-            const T oxy = ldf<T, BF>(a.Oxy, o[LXY] - dz[LXY] + dy[LXY]);
-            const T oxz = ldf<T, BF>(a.Oxz, o[LXZ]), oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
-            T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
-            if constexpr (HF) {
-                fP = ldf<T, BF>(a.fP, oC);
-                fVx = ldf<T, BF>(a.fVx, o[LVX] - dz[LVX]);
-                fVy = ldf<T, BF>(a.fVy, o[LVY] - dz[LVY]);
-                fVz = ldf<T, BF>(a.fVz, o[LVZ] - dz[LVZ]);
+            T oxx, oyy, ozz, oxy, oxz, oyz, fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0), rgz = T(0);
+            if constexpr (HO) {
+                const int qb = pa ^ 1;
+                oxx = hunpack(q_old[qb][0][ty][tx]);
+                oyy = hunpack(q_old[qb][1][ty][tx]);
+                ozz = hunpack(q_old[qb][2][ty][tx]);
+                oxy = hunpack(q_old[qb][3][ty][tx]);
+                oxz = hunpack(q_old[qb][4][ty][tx]);
+                oyz = hunpack(q_old[qb][5][ty][tx]);
+                if constexpr (HF) {
+                    fP = hunpack(q_old[qb][6][ty][tx]);
+                    fVx = hunpack(q_old[qb][7][ty][tx]);
+                    fVy = hunpack(q_old[qb][8][ty][tx]);
+                    fVz = hunpack(q_old[qb][9][ty][tx]);
+                }
+                if constexpr (RG) rgz = hunpack(q_old[qb][6][ty][tx]);
+            } else {
+                oxx = ldf<T, BF>(a.Oxx, oC);
+                oyy = ldf<T, BF>(a.Oyy, oC);
+                ozz = ldf<T, BF>(a.Ozz, oC);
+                oxy = ldf<T, BF>(a.Oxy, o[LXY] - dz[LXY] + dy[LXY]);
+                oxz = ldf<T, BF>(a.Oxz, o[LXZ]);
+                oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
+                if constexpr (HF) {
+                    fP = ldf<T, BF>(a.fP, oC);
+                    fVx = ldf<T, BF>(a.fVx, o[LVX] - dz[LVX]);
+                    fVy = ldf<T, BF>(a.fVy, o[LVY] - dz[LVY]);
+                    fVz = ldf<T, BF>(a.fVz, o[LVZ] - dz[LVZ]);
+                }
+                if constexpr (RG) rgz = ld(a.rg, o[LVZ] - dz[LVZ]);
             }
-            const T rgz = RG ? ld(a.rg, o[LVZ] - dz[LVZ]) : T(0);
             __syncthreads();
             const bool in_k = r >= 0 && r <= nb2;
             const bool in_k1 = r + 1 >= 0 && r + 1 <= nb2;

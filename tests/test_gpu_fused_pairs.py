@@ -189,12 +189,15 @@ GENERAL3 = [[[0.7, -1.3], [0.25, -0.6], [0.1, 0.35]],
 @pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("visc", ["field", "scalar"])
 @pytest.mark.parametrize("dims", [(61, 13, 17), (20, 14, 35), (130, 30, 41), (5, 4, 3), (2, 2, 2), (59, 13, 11),
-                                  (60, 10, 9)])
+                                  (60, 10, 9), (118, 20, 9), (40, 40, 7), (15, 40, 9), (75, 13, 9), (76, 70, 12),
+                                  (86, 20, 7)])
 def test_pair_pass_3d_equals_single_iterations(arch, T, exact, visc, dims):
     """3D: x spans up to three 60-column tiles, y several 9-row tiles, z
     several chunks (the host picks chunks of 2 to 6 planes at these sizes,
     37 at 255^3), down to the smallest grid the pass takes; 59 and 60 cells
-    put the right x-ghost column on its tile's first and second owned lane."""
+    put the right x-ghost column on its tile's first and second owned lane,
+    118 fills two tiles exactly, and 15 to 86 leave 17 to 42 columns in the
+    last tile."""
     g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=dims)
     for vals in (SHEAR3, GENERAL3):
         for niter in (2, 3):
