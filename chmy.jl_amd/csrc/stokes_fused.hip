@@ -992,15 +992,19 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
     __shared__ T s_eml[FVISC ? BY : 1][TW];
     __shared__ T s_P[BY][TW], s_tyy[BY][TW], s_txy[BY][TW];   // the plane's stresses (A, then B)
     __shared__ T s_tyz[2][BY][TW], b_tyz[2][BY][TW];         // tau_yz at the plane and the next
-    __shared__ T q_st[7][BY][TW];   // A's stresses of the previous step, one slot per thread (B's inputs)
-    // FP32: B's tau_old (and forcing, rho g) are the values A loaded one step
-    // earlier; they pass through per-thread LDS slots by step parity (bf16
-    // forcing as its 16 bits) instead of being loaded again.  FP64 has no
-    // LDS left for them; FP32 forcing has none unless it is bf16.
+    // A's stresses of the previous step, one slot per thread (B's inputs);
+    // tau_yz is read from A's s_tyz plane of that step instead
+    __shared__ T q_st[6][BY][TW];
+    // B's tau_old (and forcing, rho g) are the values A loaded one step
+    // earlier; they pass through per-thread LDS slots by step parity instead
+    // of being loaded again: in FP32 all of them (bf16 forcing as its 16
+    // bits), in FP64 the two the remaining LDS holds (tau_old xz and yz; the
+    // block then uses all 160 KB).  FP32 forcing stored in FP32 has no room.
     constexpr bool HO = sizeof(T) == 4 && (!HF || BF);
-    constexpr int NO = 6 + (HF ? 4 : RG ? 1 : 0);
+    constexpr bool HO64 = sizeof(T) == 8;
+    constexpr int NO = HO ? 6 + (HF ? 4 : RG ? 1 : 0) : 2;
     using HS = typename std::conditional<BF, uint16_t, T>::type;
-    __shared__ HS q_old[HO ? 2 : 1][HO ? NO : 1][HO ? BY : 1][TW];
+    __shared__ HS q_old[HO || HO64 ? 2 : 1][NO][HO || HO64 ? BY : 1][TW];
     auto hpack = [](T v) -> HS {
         if constexpr (BF) return (uint16_t)(__float_as_uint(v) >> 16);   // exact: v came from bf16
         else return v;
@@ -1241,6 +1245,10 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             s_tyy[ty][tx] = tyy;
             s_txy[ty][tx] = txy1;
             s_tyz[pa][ty][tx] = tyz1;
+            if constexpr (HO64) {
+                q_old[pa][0][ty][tx] = oxz;
+                q_old[pa][1][ty][tx] = oyz;
+            }
             if constexpr (HO) {
                 q_old[pa][0][ty][tx] = hpack(oxx);
                 q_old[pa][1][ty][tx] = hpack(oyy);
@@ -1303,14 +1311,13 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
         if (s >= sb) {
             // B's inputs are A's stresses of the previous step; this step's take their slots
             T bP = q_st[0][ty][tx], btxx = q_st[1][ty][tx], btyy = q_st[2][ty][tx], btzz = q_st[3][ty][tx];
-            T btxy = q_st[4][ty][tx], btxz = q_st[5][ty][tx], btyz = q_st[6][ty][tx];
+            T btxy = q_st[4][ty][tx], btxz = q_st[5][ty][tx], btyz = s_tyz[pa ^ 1][ty][tx];
             q_st[0][ty][tx] = P;
             q_st[1][ty][tx] = txx;
             q_st[2][ty][tx] = tyy;
             q_st[3][ty][tx] = tzz;
             q_st[4][ty][tx] = txy1;
             q_st[5][ty][tx] = txz1;
-            q_st[6][ty][tx] = tyz1;
             T V1[3];
             if (s == sb) {   // B's first step: V after the batch at plane r as well
                 if (kb == 0) {
@@ -1376,8 +1383,13 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 oyy = ldf<T, BF>(a.Oyy, oC);
                 ozz = ldf<T, BF>(a.Ozz, oC);
                 oxy = ldf<T, BF>(a.Oxy, o[LXY] - dz[LXY] + dy[LXY]);
-                oxz = ldf<T, BF>(a.Oxz, o[LXZ]);
-                oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
+                if constexpr (HO64) {
+                    oxz = q_old[pa ^ 1][0][ty][tx];
+                    oyz = q_old[pa ^ 1][1][ty][tx];
+                } else {
+                    oxz = ldf<T, BF>(a.Oxz, o[LXZ]);
+                    oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
+                }
                 if constexpr (HF) {
                     fP = ldf<T, BF>(a.fP, oC);
                     fVx = ldf<T, BF>(a.fVx, o[LVX] - dz[LVX]);
@@ -1470,7 +1482,6 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             q_st[3][ty][tx] = tzz;
             q_st[4][ty][tx] = txy1;
             q_st[5][ty][tx] = txz1;
-            q_st[6][ty][tx] = tyz1;
         }
         qv[0] = va[0];
         qv[1] = va[1];
