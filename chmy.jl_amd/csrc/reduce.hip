@@ -27,36 +27,51 @@ __global__ void zero_u64(unsigned long long* p) {
     if (threadIdx.x == 0) p[0] = 0ull;
 }
 
-// One wave per x-row of the box at a time (rows strided over the grid's
-// waves): the row's base offset is formed once, every lane's loads of the
-// row are issued before the max, and no per-element division is needed.
-constexpr int kRowChunk = 8;   // 64-wide slices of a row loaded before the max
+// Work items are (x-row, 256-element segment) pairs of the box; a wave takes
+// kRW consecutive items at a time and issues all their loads (kRW x 4 per
+// lane) before the max, so that each wave keeps 16 loads in flight.  Item
+// bases are formed once per item; no per-element division is needed.
+constexpr int kRowChunk = 4;   // 64-wide slices of an item
+constexpr int kSeg = 64 * kRowChunk;
+constexpr int kRW = 4;         // items of a wave in flight
 
 template <class T>
 __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned long long* out) {
     __shared__ double part[kRB / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t nx = b.len[0], ny = b.len[1];
-    const int64_t rows = ny * b.len[2];
+    const int64_t nseg = (nx + kSeg - 1) / kSeg;
+    const int64_t items = ny * b.len[2] * nseg;
     const int64_t nw = (int64_t)gridDim.x * (kRB / 64);
     double m = 0.0;
     bool isnan_ = false;
-    for (int64_t r = (int64_t)blockIdx.x * (kRB / 64) + w; r < rows; r += nw) {
-        const int64_t j = r % ny, k = r / ny;
-        const T* row = &f(b.lo[0], b.lo[1] + j, b.lo[2] + k);
-        for (int64_t x0 = 0; x0 < nx; x0 += 64 * kRowChunk) {
-            double a[kRowChunk];
+    for (int64_t i0 = ((int64_t)blockIdx.x * (kRB / 64) + w) * kRW; i0 < items; i0 += nw * kRW) {
+        const T* base[kRW];
+        int64_t len[kRW];
 #pragma unroll
-            for (int q = 0; q < kRowChunk; ++q) {
-                const int64_t x = x0 + 64 * q + lane;
-                a[q] = x < nx ? fabs(as_double(row[x])) : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < kRowChunk; ++q) {
-                isnan_ |= a[q] != a[q];
-                m = fmax(m, a[q]);
-            }
+        for (int u = 0; u < kRW; ++u) {
+            const int64_t it = i0 + u;
+            const int64_t r = it / nseg, sg = it - r * nseg;
+            const int64_t j = r % ny, k = r / ny;
+            const bool ok = it < items;
+            base[u] = ok ? &f(b.lo[0] + sg * kSeg, b.lo[1] + j, b.lo[2] + k) : nullptr;
+            len[u] = ok ? (nx - sg * kSeg < kSeg ? nx - sg * kSeg : kSeg) : 0;
         }
+        double a[kRW][kRowChunk];
+#pragma unroll
+        for (int u = 0; u < kRW; ++u)
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) {
+                const int x = 64 * q + lane;
+                a[u][q] = x < len[u] ? fabs(as_double(base[u][x])) : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < kRW; ++u)
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) {
+                isnan_ |= a[u][q] != a[u][q];
+                m = fmax(m, a[u][q]);
+            }
     }
     m = wave_max(m);
     const bool anynan = __any(isnan_);
@@ -157,10 +172,10 @@ static unsigned grid_blocks(const Box3& b) {
     return (unsigned)nb;
 }
 
-// maxabs: one wave per row, about 8 rows per wave at most
+// maxabs: (row, segment) items, two rounds of kRW items per wave at most
 static unsigned row_blocks(const Box3& b) {
-    int64_t rows = b.len[1] * b.len[2];
-    int64_t nb = (rows + 4 * 8 - 1) / (4 * 8);
+    const int64_t items = b.len[1] * b.len[2] * ((b.len[0] + kSeg - 1) / kSeg);
+    int64_t nb = (items + (kRB / 64) * kRW * 2 - 1) / ((kRB / 64) * kRW * 2);
     if (nb > 4096) nb = 4096;
     if (nb < 1) nb = 1;
     return (unsigned)nb;

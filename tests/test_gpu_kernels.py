@@ -574,3 +574,31 @@ def test_reductions(arch, T):
     # NaN propagates
     f.interior()[3, 2, 1] = float("nan")
     assert np.isnan(chmy.kernels.maxabs(f))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("dims", [(600, 3, 2), (257,), (1000, 7), (5, 5, 5), (256, 9), (513, 1, 3)])
+def test_maxabs_items(arch, dtype, dims):
+    """max|x| over the (row, 256-element segment) items the kernel splits the
+    box into: rows of one to four segments, item counts that are not a
+    multiple of the items a wave takes, and the maximum planted at the first
+    and last interior element and in a later segment of a middle row; halo
+    values above the maximum are ignored; exact against numpy."""
+    nd = len(dims)
+    g = chmy.UniformGrid(arch, origin=(0.0,) * nd, extent=(1.0,) * nd, dims=dims)
+    f = chmy.Field(arch, g, (Cc(),) * nd, dtype)
+    rng = np.random.default_rng(sum(dims))
+    n = dims   # interior() indexes (x, y, z)
+    spots = [(0,) * nd, tuple(m - 1 for m in n), (min(n[0] - 1, 300),) + tuple(m // 2 for m in n[1:])]
+    for q, spot in enumerate(spots):
+        base = torch.from_numpy(rng.uniform(-1.0, 1.0, tuple(f.data.shape))).to(dtype)
+        base.fill_(100.0)                                # halo
+        inner = torch.from_numpy(rng.uniform(-1.0, 1.0, n)).to(dtype)
+        inner[spot] = -7.5 if q % 2 else 7.5
+        f.data.copy_(base)
+        f.interior().copy_(inner.to(f.data.device))
+        want = float(np.max(np.abs(inner.double().numpy())))
+        assert want == 7.5
+        assert chmy.kernels.maxabs(f) == want, (dims, spot)
+    f.interior()[spots[-1]] = float("nan")
+    assert np.isnan(chmy.kernels.maxabs(f))
