@@ -1042,13 +1042,23 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
     const bool xb = i0 <= 1 || (i0 <= nb0 && nb0 <= i0 + TW - 1);
 
     // byte offsets at plane kb - 2 (clamped to -1); loads of plane s+1 past
-    // the storage (s = n2+2) take plane s instead
+    // the storage (s = n2+2) take plane s instead.  FP64: a location's offset
+    // is xo + uo[c], the column part xo per lane and shared by all seven
+    // locations, the row and plane part uo[c] wave-uniform (SGPRs), so the
+    // offsets hold one VGPR instead of seven (the unsigned sum wraps to the
+    // non-negative offset when ic = -1): VGPR spills 9 -> 2 in the fast form.
+    // uo[c] is recomputed from the plane index at each step (see the loop).
+    // FP32 keeps uo[c] as the whole per-lane offset (the split cost its
+    // correction sweep 1%).
+    constexpr bool SPLIT = sizeof(T) == 8;
     const int s0 = max(kb - 2, -1);
-    uint32_t o[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
+    uint32_t uo[NLOC], dz[NLOC], dy[NLOC], d8[NLOC];
+    const uint32_t xo = SPLIT ? (uint32_t)(ic * (int)E) : 0u;
+    const auto o = [&](int c) -> uint32_t { return SPLIT ? xo + uo[c] : uo[c]; };
 #pragma unroll
     for (int c = 0; c < NLOC; ++c) {
         const LocStride sl = a.L[c];
-        o[c] = (uint32_t)((int64_t)sl.o0 + ic + (int64_t)jc * sl.sy + (int64_t)s0 * sl.sz) * E;
+        uo[c] = (uint32_t)(((int64_t)sl.o0 + (SPLIT ? 0 : ic) + (int64_t)jc * sl.sy + (int64_t)s0 * sl.sz) * E);
         dz[c] = sl.sz * E;
         dy[c] = sl.sy * E;
         d8[c] = row8 ? dy[c] : 0u;
@@ -1110,16 +1120,16 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
 
     // ---- A: carried state at plane s0
     int pa = 0, pb = 0;
-    T Avx = ld(a.Vx, o[LVX]), Avy = ld(a.Vy, o[LVY]), Avz = ld(a.Vz, o[LVZ]);
+    T Avx = ld(a.Vx, o(LVX)), Avy = ld(a.Vy, o(LVY)), Avz = ld(a.Vz, o(LVZ));
     T Aeta = T(0), Aeml_m = T(0), AP_m = T(0), Atzz_m = T(0), Atxz = T(0), Atyz = T(0);
     s_vy[0][ty][tx] = Avy;
     if constexpr (FVISC) {
-        Aeta = ld(a.eta, o[LC]);
+        Aeta = ld(a.eta, o(LC));
         s_eta[0][ty][tx] = Aeta;
     }
     if (last) {
-        s_vy[0][BY][tx] = ld(a.Vy, o[LVY] + d8[LVY]);
-        if constexpr (FVISC) s_eta[0][BY][tx] = ld(a.eta, o[LC] + d8[LC]);
+        s_vy[0][BY][tx] = ld(a.Vy, o(LVY) + d8[LVY]);
+        if constexpr (FVISC) s_eta[0][BY][tx] = ld(a.eta, o(LC) + d8[LC]);
     }
     // A's results of the previous step (plane s-1), for B (its stresses are in q_st)
     T qv[3] = {T(0), T(0), T(0)};   // A's raw V at s-1
@@ -1130,11 +1140,22 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
     T BP_m = T(0), Btzz_m = T(0), Btxz = T(0), Btyz = T(0);
     const int sb = kb == 0 ? 1 : kb;   // B's first step (plane sb - 1)
 
+    uint32_t ub[NLOC];   // uo at plane 0 (modulo 2^32)
+#pragma unroll
+    for (int c = 0; c < NLOC; ++c) ub[c] = uo[c] - (uint32_t)s0 * dz[c];
     for (int s = s0; s <= ke; ++s) {
+        if constexpr (SPLIT) {
+            // formed from the plane index through readfirstlane, so that the
+            // compiler keeps one offset per location rather than one induction
+            // variable per derived offset (o + dy, o - dz, ...): SGPR spill
+            // reloads in the loop 546 -> 132, pass 1.257 -> 1.230 ms at 255^3
+#pragma unroll
+            for (int c = 0; c < NLOC; ++c) uo[c] = __builtin_amdgcn_readfirstlane(ub[c] + (uint32_t)s * dz[c]);
+        }
         const bool top = s + 1 > n2 + 2;   // plane s+1 is past the storage
         uint32_t z1[NLOC];
 #pragma unroll
-        for (int c = 0; c < NLOC; ++c) z1[c] = top ? o[c] : o[c] + dz[c];
+        for (int c = 0; c < NLOC; ++c) z1[c] = top ? o(c) : o(c) + dz[c];
         // ================= A: plane s
         T va[3], vx1, vy1, vz1, P, txx, tyy, tzz, txy1, txz1, tyz1;
         Loc Lc, Le, L4, L5;
@@ -1144,7 +1165,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             // writes (before the first barrier) then need not wait for the rest
             T vx8 = T(0), vy8 = T(0), vz8 = T(0), eta8 = T(0);
             if (last) {
-                vx8 = ld(a.Vx, o[LVX] + d8[LVX]);
+                vx8 = ld(a.Vx, o(LVX) + d8[LVX]);
                 vy8 = ld(a.Vy, z1[LVY] + d8[LVY]);
                 vz8 = ld(a.Vz, z1[LVZ] + d8[LVZ]);
                 if constexpr (FVISC) eta8 = ld(a.eta, z1[LC] + d8[LC]);
@@ -1155,26 +1176,26 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             T eta1 = T(0), eml = T(0);
             if constexpr (FVISC) {
                 eta1 = ld(a.eta, z1[LC]);
-                eml = ld(a.eml, o[LC]);
+                eml = ld(a.eml, o(LC));
             }
-            P = ld(a.Pr, o[LC]);
-            txx = ld(a.txx, o[LC]);
-            tyy = ld(a.tyy, o[LC]);
-            tzz = ld(a.tzz, o[LC]);
-            txy1 = ld(a.txy, o[LXY] + dy[LXY]);
+            P = ld(a.Pr, o(LC));
+            txx = ld(a.txx, o(LC));
+            tyy = ld(a.tyy, o(LC));
+            tzz = ld(a.tzz, o(LC));
+            txy1 = ld(a.txy, o(LXY) + dy[LXY]);
             txz1 = ld(a.txz, z1[LXZ]);
             tyz1 = ld(a.tyz, z1[LYZ] + dy[LYZ]);
-            const T oxx = ldf<T, BF>(a.Oxx, o[LC]), oyy = ldf<T, BF>(a.Oyy, o[LC]), ozz = ldf<T, BF>(a.Ozz, o[LC]);
-            const T oxy = ldf<T, BF>(a.Oxy, o[LXY] + dy[LXY]);
+            const T oxx = ldf<T, BF>(a.Oxx, o(LC)), oyy = ldf<T, BF>(a.Oyy, o(LC)), ozz = ldf<T, BF>(a.Ozz, o(LC));
+            const T oxy = ldf<T, BF>(a.Oxy, o(LXY) + dy[LXY]);
             const T oxz = ldf<T, BF>(a.Oxz, z1[LXZ]), oyz = ldf<T, BF>(a.Oyz, z1[LYZ] + dy[LYZ]);
             T fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0);
             if constexpr (HF) {
-                fP = ldf<T, BF>(a.fP, o[LC]);
-                fVx = ldf<T, BF>(a.fVx, o[LVX]);
-                fVy = ldf<T, BF>(a.fVy, o[LVY]);
-                fVz = ldf<T, BF>(a.fVz, o[LVZ]);
+                fP = ldf<T, BF>(a.fP, o(LC));
+                fVx = ldf<T, BF>(a.fVx, o(LVX));
+                fVy = ldf<T, BF>(a.fVy, o(LVY));
+                fVz = ldf<T, BF>(a.fVz, o(LVZ));
             }
-            const T rgz = RG ? ld(a.rg, o[LVZ]) : T(0);
+            const T rgz = RG ? ld(a.rg, o(LVZ)) : T(0);
             const bool in_s = s >= 0 && s <= nb2;
             const bool in_s1 = s + 1 >= 0 && s + 1 <= nb2;
             s_vx[ty][tx] = Avx;
@@ -1337,12 +1358,12 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                     __syncthreads();
                     // B's tau_xz at (i, j, 0) and tau_yz at (i, j+1, 0); V at plane -1 is the
                     // input, since no iteration and no rule changes that plane
-                    const T vxm = ld(a.Vx, o[LVX] - 2 * dz[LVX]);
-                    const T vym = ld(a.Vy, o[LVY] - 2 * dz[LVY] + dy[LVY]);
-                    const T oxz = ldf<T, BF>(a.Oxz, o[LXZ] - dz[LXZ]);
-                    const T oyz = ldf<T, BF>(a.Oyz, o[LYZ] - dz[LYZ] + dy[LYZ]);
-                    BP_m = ld(a.Pr, o[LC] - 2 * dz[LC]);
-                    Btzz_m = ld(a.tzz, o[LC] - 2 * dz[LC]);
+                    const T vxm = ld(a.Vx, o(LVX) - 2 * dz[LVX]);
+                    const T vym = ld(a.Vy, o(LVY) - 2 * dz[LVY] + dy[LVY]);
+                    const T oxz = ldf<T, BF>(a.Oxz, o(LXZ) - dz[LXZ]);
+                    const T oyz = ldf<T, BF>(a.Oyz, o(LYZ) - dz[LYZ] + dy[LYZ]);
+                    BP_m = ld(a.Pr, o(LC) - 2 * dz[LC]);
+                    Btzz_m = ld(a.tzz, o(LC) - 2 * dz[LC]);
                     const T exz = T(0.5) * ((Bv[0] - vxm) * iz + (Bv[2] - xm1(Bv[2])) * ix);
                     const T eyz = T(0.5) * ((b_vy[pb][typ][tx] - vym) * iz + (s_vz1[typ][tx] - Bv[2]) * iy);
                     Btxz = inc ? AR::off(ztxz, oxz, exz, zL4, K) : ztxz;
@@ -1361,7 +1382,7 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             s_vz1[ty][tx] = V1[2];
             // tau_old (and forcing) of plane r (Center, xy) and r+1 (xz, yz): o[] sits at
             // plane s; with HO the slots A filled one step earlier
-            const uint32_t oC = o[LC] - dz[LC];
+            const uint32_t oC = o(LC) - dz[LC];
             T oxx, oyy, ozz, oxy, oxz, oyz, fP = T(0), fVx = T(0), fVy = T(0), fVz = T(0), rgz = T(0);
             if constexpr (HO) {
                 const int qb = pa ^ 1;
@@ -1382,21 +1403,21 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 oxx = ldf<T, BF>(a.Oxx, oC);
                 oyy = ldf<T, BF>(a.Oyy, oC);
                 ozz = ldf<T, BF>(a.Ozz, oC);
-                oxy = ldf<T, BF>(a.Oxy, o[LXY] - dz[LXY] + dy[LXY]);
+                oxy = ldf<T, BF>(a.Oxy, o(LXY) - dz[LXY] + dy[LXY]);
                 if constexpr (HO64) {
                     oxz = q_old[pa ^ 1][0][ty][tx];
                     oyz = q_old[pa ^ 1][1][ty][tx];
                 } else {
-                    oxz = ldf<T, BF>(a.Oxz, o[LXZ]);
-                    oyz = ldf<T, BF>(a.Oyz, o[LYZ] + dy[LYZ]);
+                    oxz = ldf<T, BF>(a.Oxz, o(LXZ));
+                    oyz = ldf<T, BF>(a.Oyz, o(LYZ) + dy[LYZ]);
                 }
                 if constexpr (HF) {
                     fP = ldf<T, BF>(a.fP, oC);
-                    fVx = ldf<T, BF>(a.fVx, o[LVX] - dz[LVX]);
-                    fVy = ldf<T, BF>(a.fVy, o[LVY] - dz[LVY]);
-                    fVz = ldf<T, BF>(a.fVz, o[LVZ] - dz[LVZ]);
+                    fVx = ldf<T, BF>(a.fVx, o(LVX) - dz[LVX]);
+                    fVy = ldf<T, BF>(a.fVy, o(LVY) - dz[LVY]);
+                    fVz = ldf<T, BF>(a.fVz, o(LVZ) - dz[LVZ]);
                 }
-                if constexpr (RG) rgz = ld(a.rg, o[LVZ] - dz[LVZ]);
+                if constexpr (RG) rgz = ld(a.rg, o(LVZ) - dz[LVZ]);
             }
             __syncthreads();
             const bool in_k = r >= 0 && r <= nb2;
@@ -1436,11 +1457,11 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                     sto(a.otxx, oC, btxx);
                     sto(a.otyy, oC, btyy);
                     sto(a.otzz, oC, btzz);
-                    sto(a.otxz, o[LXZ] - dz[LXZ], Btxz);
+                    sto(a.otxz, o(LXZ) - dz[LXZ], Btxz);
                 }
                 if (owne) {
-                    sto(a.otxy, o[LXY] - dz[LXY] + dy[LXY], btxy);
-                    sto(a.otyz, o[LYZ] - dz[LYZ] + dy[LYZ], Btyz);
+                    sto(a.otxy, o(LXY) - dz[LXY] + dy[LXY], btxy);
+                    sto(a.otyz, o(LYZ) - dz[LYZ] + dy[LYZ], Btyz);
                 }
             }
             const T txxl = xm1(btxx), txzr = xp1(Btxz);
@@ -1462,9 +1483,9 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
                 rz = rz + fVz;
             }
             if (st && ownc) {
-                sto(a.oVx, o[LVX] - dz[LVX], AR::vel(Bv[0], rx, qex, K));
-                sto(a.oVy, o[LVY] - dz[LVY], AR::vel(Bv[1], ry, qey, K));
-                sto(a.oVz, o[LVZ] - dz[LVZ], AR::vel(Bv[2], rz, qez, K));
+                sto(a.oVx, o(LVX) - dz[LVX], AR::vel(Bv[0], rx, qex, K));
+                sto(a.oVy, o(LVY) - dz[LVY], AR::vel(Bv[1], ry, qey, K));
+                sto(a.oVz, o(LVZ) - dz[LVZ], AR::vel(Bv[2], rz, qez, K));
             }
             BP_m = bP;
             Btzz_m = btzz;
@@ -1494,8 +1515,9 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
         qey = ey;
         qez = ez;
         pa ^= 1;
+        if constexpr (!SPLIT)
 #pragma unroll
-        for (int c = 0; c < NLOC; ++c) o[c] += dz[c];
+            for (int c = 0; c < NLOC; ++c) uo[c] += dz[c];
     }
 }
 
