@@ -89,24 +89,49 @@ __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned l
     }
 }
 
+// Sum of squares over the same (row, segment) items, in a fixed order: the
+// grid is always kSumBlocks blocks, so which thread adds which element, and
+// the order of every addition, depend on the box only.
 template <class T>
 __global__ void __launch_bounds__(kRB) sumsq_kernel(Box3 b, DF<T> f, double* partial) {
     __shared__ double part[kRB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t nx = b.len[0], ny = b.len[1];
-    const int64_t tot = nx * ny * b.len[2];
+    const int64_t nseg = (nx + kSeg - 1) / kSeg;
+    const int64_t items = ny * b.len[2] * nseg;
+    const int64_t nw = (int64_t)gridDim.x * (kRB / 64);
     double s = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * kRB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * kRB) {
-        int64_t i = t % nx, r = t / nx;
-        int64_t j = r % ny, k = r / ny;
-        double a = as_double(f(b.lo[0] + i, b.lo[1] + j, b.lo[2] + k));
-        s += a * a;
+    for (int64_t i0 = ((int64_t)blockIdx.x * (kRB / 64) + w) * kRW; i0 < items; i0 += nw * kRW) {
+        const T* base[kRW];
+        int64_t len[kRW];
+#pragma unroll
+        for (int u = 0; u < kRW; ++u) {
+            const int64_t it = i0 + u;
+            const int64_t r = it / nseg, sg = it - r * nseg;
+            const int64_t j = r % ny, k = r / ny;
+            const bool ok = it < items;
+            base[u] = ok ? &f(b.lo[0] + sg * kSeg, b.lo[1] + j, b.lo[2] + k) : nullptr;
+            len[u] = ok ? (nx - sg * kSeg < kSeg ? nx - sg * kSeg : kSeg) : 0;
+        }
+        double a[kRW][kRowChunk];
+#pragma unroll
+        for (int u = 0; u < kRW; ++u)
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) {
+                const int x = 64 * q + lane;
+                a[u][q] = x < len[u] ? as_double(base[u][x]) : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < kRW; ++u)
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) s += a[u][q] * a[u][q];
     }
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    if (lane == 0) part[w] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         double v = part[0];
-        for (int w = 1; w < kRB / 64; ++w) v += part[w];
+        for (int q = 1; q < kRB / 64; ++q) v += part[q];
         partial[blockIdx.x] = v;
     }
 }
@@ -162,14 +187,6 @@ static int reduce_box(const chmy_field_t* f, const chmy_box_t* box, Box3* out) {
     }
     *out = box3(b, f->nd);
     return 0;
-}
-
-static unsigned grid_blocks(const Box3& b) {
-    int64_t tot = b.len[0] * b.len[1] * b.len[2];
-    int64_t nb = (tot + kRB - 1) / kRB;
-    if (nb > 2048) nb = 2048;
-    if (nb < 1) nb = 1;
-    return (unsigned)nb;
 }
 
 // maxabs: (row, segment) items, two rounds of kRW items per wave at most

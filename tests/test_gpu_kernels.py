@@ -579,8 +579,8 @@ def test_reductions(arch, T):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("dims", [(600, 3, 2), (257,), (1000, 7), (5, 5, 5), (256, 9), (513, 1, 3)])
 def test_maxabs_items(arch, dtype, dims):
-    """max|x| over the (row, 256-element segment) items the kernel splits the
-    box into: rows of one to four segments, item counts that are not a
+    """max|x| and the sum of squares over the (row, 256-element segment)
+    items the kernels split the box into: rows of one to four segments, item counts that are not a
     multiple of the items a wave takes, and the maximum planted at the first
     and last interior element and in a later segment of a middle row; halo
     values above the maximum are ignored; exact against numpy."""
@@ -600,5 +600,8 @@ def test_maxabs_items(arch, dtype, dims):
         want = float(np.max(np.abs(inner.double().numpy())))
         assert want == 7.5
         assert chmy.kernels.maxabs(f) == want, (dims, spot)
+        ss = chmy.kernels.sumsq(f)   # same items; a fixed order, so repeatable bit for bit
+        assert np.isclose(ss, float(np.sum(inner.double().numpy() ** 2)), rtol=1e-12, atol=0), (dims, spot)
+        assert ss == chmy.kernels.sumsq(f)
     f.interior()[spots[-1]] = float("nan")
     assert np.isnan(chmy.kernels.maxabs(f))
