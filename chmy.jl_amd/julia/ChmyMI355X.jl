@@ -25,6 +25,11 @@ ChmyAMDGPUExt is replaced:
     `SingleDeviceArchitecture{MI355XBackend}` (src/BoundaryConditions/batch.jl:20-29,180-184);
   - `set!` on MI355X fields: Chmy's own CPU `set!` into a host field, then one
     upload (bitwise Chmy's CPU values);
+  - `iterate!` (fused PT iterations with the V batch, in two-iteration passes
+    for free-slip batches) and `iter2!` (one such pass); `stokes_residual!` and
+    `stokes_accumulate!` for FP32 sweeps with FP64 defect correction;
+    `maxabs` for the residual norms; `apply_op!` for any GridOperator (plain,
+    masked by a FieldMask component, on uniform or FunctionAxis grids);
   - `rrule`s for the SIA time loop and the steady Stokes solve.
 
 Usage, against examples/stokes_3d_inc_ve_T.jl:
@@ -628,9 +633,9 @@ batch, stokes_3d_inc_ve_T.jl:164-165) in one call, double-buffered between
 state.  `bc_V` is the example's `(V.x => (x=Dirichlet(..), ...), ...)` list;
 `p` holds G, dt, dτ_Pr, dτ_r, νdτ and, for a scalar viscosity, η and η_ve.
 """
-function iterate!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V, niter::Integer, p;
-                  flags::Integer=0, flags_last::Integer=flags) where {N}
-    (V0, τ0, P0), (V1, τ1, P1) = state0, state1
+# the V batch of bc_V as (counts per (dim, side), V component per condition,
+# condition descriptors, value fields to keep alive)
+function vbatch(grid::UniformGrid{N}, bc_V, V0) where {N}
     comps = Int32[]
     cs = ChmyBC[]
     counts = zeros(Int32, 6)
@@ -645,6 +650,13 @@ function iterate!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V,
         end
         counts[2D+S-2] = length(b.fields)
     end
+    return counts, comps, cs, keep
+end
+
+function iterate!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V, niter::Integer, p;
+                  flags::Integer=0, flags_last::Integer=flags) where {N}
+    (V0, τ0, P0), (V1, τ1, P1) = state0, state1
+    counts, comps, cs, keep = vbatch(grid, bc_V, V0)
     par = Ref{Cint}(0)
     ηd = η isa Number ? nothing : Ref(desc(η))
     md = η_ml === nothing ? nothing : Ref(desc(η_ml))
@@ -663,6 +675,153 @@ function iterate!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V,
                     Float64(p.dτ_r), Float64(p.νdτ), par, current_stream()))
     end
     return Int(par[])
+end
+
+"""
+    iter2!(state0, state1, τ_old, η, η_ml, grid, bc_V, p; flags=0)
+
+Two PT iterations in one pass from `state0` into `state1`, the free-slip V
+batch applied in-kernel between them (bitwise two `iterate!` steps); the
+caller applies the V batch to `state1` and swaps, as with one iteration.
+`bc_V` must be the free-slip batch (each V component Dirichlet on its own
+dim, Neumann on the others, scalar values).
+"""
+function iter2!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V, p; flags::Integer=0) where {N}
+    (V0, τ0, P0), (V1, τ1, P1) = state0, state1
+    counts, comps, cs, keep = vbatch(grid, bc_V, V0)
+    ηd = η isa Number ? nothing : Ref(desc(η))
+    md = η_ml === nothing ? nothing : Ref(desc(η_ml))
+    GC.@preserve keep ηd md begin
+        ηp = ηd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, ηd)
+        mp = md === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, md)
+        check(ccall((:chmy_stokes_iter2, LIB), Cint,
+                    (Ref{ChmyGrid}, Ptr{ChmyField}, Ptr{ChmyField}, Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ptr{ChmyField}, Ptr{Int32}, Ptr{Int32}, Ptr{ChmyBC}, Cint, Cdouble, Cdouble, Cdouble, Cdouble,
+                     Cdouble, Cdouble, Cdouble, Ptr{Cvoid}),
+                    desc(grid), descs(V0), descs(τ0), desc(P0), descs(V1), descs(τ1), desc(P1), descs(τ_old), ηp, mp,
+                    Ptr{ChmyField}(C_NULL), Ptr{ChmyField}(C_NULL), Ptr{ChmyField}(C_NULL), counts, comps, cs,
+                    Cint(flags), _eta_s(η), Float64(p.η_ve), Float64(p.G), Float64(p.dt), Float64(p.dτ_Pr),
+                    Float64(p.dτ_r), Float64(p.νdτ), current_stream()))
+    end
+    return state1
+end
+
+"""
+    stokes_residual!(x, τ_old, η, grid, p, forcing; rhog=nothing, divV=nothing, r_V=nothing)
+
+Defect correction, first half (build extension): the f64 residual of the
+state `x = (V, τ, Pr)` in the reference's expression order
+(stokes_3d_inc_ve_T.jl:23-57), written as the correction sweeps' forcing
+`forcing = (τ_old_c, f_V, f_P)` (τ_old_c = G dt R_τ, f_V = R_V, f_P = R_P; f32
+or bf16 fields); with `divV` and `r_V` also ∇V and r_V in f64 for the
+convergence check.  The sweeps are `iterate!` on a correction state from 0
+with that forcing (flags as usual, the forcing in `τ_old`).
+"""
+function stokes_residual!(x, τ_old, η, grid::UniformGrid, p, forcing; rhog=nothing, divV=nothing, r_V=nothing)
+    V, τ, Pr = x
+    τc, fV, fP = forcing
+    ηd = η isa Number ? nothing : Ref(desc(η))
+    gd = rhog === nothing ? nothing : Ref(desc(rhog))
+    dd = divV === nothing ? nothing : Ref(desc(divV))
+    rd = r_V === nothing ? nothing : descs(r_V)
+    GC.@preserve ηd gd dd rd begin
+        ηp = ηd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, ηd)
+        gp = gd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, gd)
+        dp = dd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, dd)
+        rp = rd === nothing ? Ptr{ChmyField}(C_NULL) : pointer(rd)
+        check(ccall((:chmy_stokes_residual, LIB), Cint,
+                    (Ref{ChmyGrid}, Ptr{ChmyField}, Ptr{ChmyField}, Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ptr{ChmyField}, Ptr{ChmyField}, Cdouble, Cdouble, Cdouble, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{Cvoid}),
+                    desc(grid), descs(V), descs(τ), desc(Pr), descs(τ_old), ηp, gp, Ptr{ChmyField}(C_NULL),
+                    _eta_s(η), Float64(p.G), Float64(p.dt), descs(τc), descs(fV), desc(fP), dp, rp,
+                    current_stream()))
+    end
+    return forcing
+end
+
+"""
+    stokes_accumulate!(x, d, grid)
+
+Defect correction, second half: `x .+= d` over the Launcher box for
+`x = (V, τ, Pr)` in f64 and the correction `d` (f32 or f64).  The real V
+batch on `x` follows, as after any iteration.
+"""
+function stokes_accumulate!(x, d, grid::UniformGrid)
+    (V, τ, Pr), (Vc, τc, Pc) = x, d
+    check(ccall((:chmy_stokes_accumulate, LIB), Cint,
+                (Ref{ChmyGrid}, Ptr{ChmyField}, Ptr{ChmyField}, Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                 Ref{ChmyField}, Ptr{Cvoid}),
+                desc(grid), descs(V), descs(τ), desc(Pr), descs(Vc), descs(τc), desc(Pc), current_stream()))
+    return x
+end
+
+"""
+    maxabs(f; box=nothing) -> Float64
+
+`maximum(abs.(interior(f)))` on the device (the examples' residual norms,
+stokes_3d_inc_ve_T.jl:172-175), or over a box of logical indices.
+"""
+function maxabs(f::MI355XField; box=nothing)
+    out = Ref{Cdouble}(0.0)
+    bd = box === nothing ? nothing : Ref(box)
+    GC.@preserve bd begin
+        bp = bd === nothing ? Ptr{ChmyBox}(C_NULL) : Base.unsafe_convert(Ptr{ChmyBox}, bd)
+        check(ccall((:chmy_maxabs, LIB), Cint, (Ref{ChmyField}, Ptr{ChmyBox}, Ref{Cdouble}, Ptr{Cvoid}),
+                    desc(f), bp, out, current_stream()))
+    end
+    return out[]
+end
+
+# ------------------------------------------------------------------ operators
+const CHMY_OP_PARTIAL = Cint(0)
+const CHMY_OP_LERP = Cint(1)
+const CHMY_OP_DIVG = Cint(2)
+const CHMY_OP_VMAG = Cint(3)
+const CHMY_OP_D2 = Cint(4)
+const CHMY_OP_DKD = Cint(5)
+const CHMY_OP_LAPL = Cint(6)
+const CHMY_OP_DIVG_GRAD = Cint(7)
+const CHMY_OP_LEFT = Cint(8)
+const CHMY_OP_RIGHT = Cint(9)
+const CHMY_OP_HLERP = Cint(10)
+const CHMY_OP_DELTA = Cint(11)
+
+"""
+    apply_op!(op, dst, src, grid; dim=1, masks=nothing, k=nothing, box=nothing)
+
+One GridOperator evaluated into `dst` over a box (the GPU form of the
+operator evaluations of test/test_grid_operators.jl): `op` a `CHMY_OP_*`
+code, `src` a Field or a tuple of Fields (divg, vmag: the vector components),
+`dim` 1-based, `masks` the FieldMask component at each source's location
+(`at(ω, location(src))`, masked_operators.jl:11-19) for the masked forms,
+`k` the coefficient of ∂k∂ and divg_grad.  `grid` may be a UniformGrid or a
+StructuredGrid with FunctionAxis dims.  The default box is the Launcher's
+[0, n+1] per dim.
+"""
+function apply_op!(op::Cint, dst::MI355XField, src, grid; dim::Integer=1, masks=nothing, k=nothing, box=nothing)
+    sd = ChmyField[desc(f) for f in (src isa Tuple ? src : (src,))]
+    md = masks === nothing ? nothing : ChmyField[desc(m) for m in (masks isa Tuple ? masks : (masks,))]
+    kd = k === nothing ? nothing : Ref(desc(k))
+    bd = box === nothing ? nothing : Ref(box)
+    GC.@preserve md kd bd begin
+        bp = bd === nothing ? Ptr{ChmyBox}(C_NULL) : Base.unsafe_convert(Ptr{ChmyBox}, bd)
+        if md === nothing && kd === nothing
+            check(ccall((:chmy_op, LIB), Cint,
+                        (Cint, Ref{ChmyGrid}, Ptr{ChmyBox}, Ref{ChmyField}, Cint, Ptr{ChmyField}, Cint, Ptr{Cvoid}),
+                        op, desc(grid), bp, desc(dst), Cint(length(sd)), sd, Cint(dim - 1), current_stream()))
+        else
+            mp = md === nothing ? Ptr{ChmyField}(C_NULL) : pointer(md)
+            kp = kd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, kd)
+            check(ccall((:chmy_op_masked, LIB), Cint,
+                        (Cint, Ref{ChmyGrid}, Ptr{ChmyBox}, Ref{ChmyField}, Cint, Ptr{ChmyField}, Ptr{ChmyField},
+                         Ptr{ChmyField}, Cint, Ptr{Cvoid}),
+                        op, desc(grid), bp, desc(dst), Cint(length(sd)), sd, mp, kp, Cint(dim - 1),
+                        current_stream()))
+        end
+    end
+    return dst
 end
 
 """
