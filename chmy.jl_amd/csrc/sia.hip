@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <mutex>
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace chmy {
@@ -302,6 +304,8 @@ struct SiaStep {   // one forward step H -> Ho (+ the H Neumann batch when bc !=
     DF<T> Ho;
     T dt;
     int bc;
+    int wd;        // also store D(H) of the tile's cells into Dout (for a later adjoint step at H)
+    DF<T> Dout;
 };
 
 template <class T>
@@ -318,7 +322,10 @@ __device__ __forceinline__ void sia_step_block(const SiaStep<T>& p, int64_t nx, 
     for (int q = tid; q < W * Hh; q += kST) {
         const int a = q % W, b = q / W;
         const int64_t i = i0 - 1 + a, j = j0 - 1 + b;
-        Ds[b][a] = (i >= 1 && i <= nx && j >= 1 && j <= ny) ? L.D(i, j) : T(0);
+        const bool in = i >= 1 && i <= nx && j >= 1 && j <= ny;
+        const T d = in ? L.D(i, j) : T(0);
+        Ds[b][a] = d;
+        if (p.wd && in && a >= 1 && a <= kSX && b >= 1 && b <= kSY) p.Dout(i, j, 0) = d;
     }
     __syncthreads();
     for (int q = tid; q < kSX * kSY; q += kST) {
@@ -363,6 +370,8 @@ struct SiaAdj {    // one adjoint step at state s.H: lam -> lo, gA += dJ/dlog A
     SiaF<T> s;
     DF<T> lam, lo, gA;
     T dt;
+    int rd;        // D(s.H) read from Din (stored by the forward step at s.H) instead of evaluated
+    DF<T> Din;
 };
 
 template <class T>
@@ -383,11 +392,12 @@ __device__ __forceinline__ void sia_adj_block(const SiaAdj<T>& p, int64_t nx, in
     const LS L{lds, i0, j0, &s};
     __syncthreads();
     auto inside = [nx, ny](int64_t i, int64_t j) { return i >= 1 && i <= nx && j >= 1 && j <= ny; };
-    // D on the tile + 3 ring; cell (i, j) at Ds[j - j0 + 3][i - i0 + 3]
+    // D on the tile + 3 ring; cell (i, j) at Ds[j - j0 + 3][i - i0 + 3]; the same
+    // bits whether evaluated here or stored by the forward step at this state
     for (int q = tid; q < W3 * H3; q += kST) {
         const int a = q % W3, b = q / W3;
         const int64_t i = i0 - 3 + a, j = j0 - 3 + b;
-        Ds[b][a] = inside(i, j) ? L.D(i, j) : T(0);
+        Ds[b][a] = inside(i, j) ? (p.rd ? p.Din(i, j, 0) : L.D(i, j)) : T(0);
     }
     __syncthreads();
     // pass A1: mu on the tile + 2 ring; cell at Ms[j - j0 + 2][i - i0 + 2]
@@ -580,6 +590,8 @@ static SiaStep<double> sia_stepper(const chmy_grid_t* g, const chmy_field_t* H, 
     f.Ho = view<double>(*Ho);
     f.dt = dt;
     f.bc = 1;
+    f.wd = 0;
+    f.Dout = f.Ho;
     return f;
 }
 
@@ -592,6 +604,8 @@ static SiaAdj<double> sia_adjointer(const chmy_grid_t* g, const chmy_field_t* H,
     a.lo = view<double>(*lo);
     a.gA = gA ? view<double>(*gA) : a.lo;
     a.dt = dt;
+    a.rd = 0;
+    a.Din = a.lo;
     return a;
 }
 
@@ -618,24 +632,33 @@ static int sia_reverse_check(const chmy_grid_t* g, const chmy_field_t* ckpt, int
 }
 
 // The reverse sweep on the stream: seg(i) gives segment state buffer i
-// (0 <= i < 2 (stride - 1)) by value.  Returns the slot that holds dJ/dH_0
+// (0 <= i < 2 (stride - 1)) by value; with dbuf (2 stride Center fields, or
+// NULL) the recompute steps also store D of the state they start from, and
+// the adjoint step at that state reads it instead of evaluating it (every
+// state of a segment but its last).  Returns the slot that holds dJ/dH_0
 // (0 = lam_a, 1 = lam_b).
 template <class SegF>
 static int sia_reverse_sweep(const chmy_grid_t* g, const chmy_field_t* ckpt, int stride, int nsteps, SegF seg,
                              const chmy_field_t* lam_a, const chmy_field_t* lam_b, const chmy_field_t* grad_logA,
                              const chmy_field_t* B, const chmy_field_t* A, const SiaPar& p, const double* dts,
-                             hipStream_t st) {
+                             hipStream_t st, const chmy_field_t* dbuf = nullptr) {
     const int nseg = (nsteps + stride - 1) / stride;
     const int64_t nx = g->n[0], ny = g->n[1];
     // state q of segment s: q = 0 its checkpoint, q >= 1 seg((s % 2)(stride - 1) + q - 1)
     auto state = [&](int sg, int q) -> chmy_field_t {
         return q == 0 ? ckpt[sg] : seg((sg % 2) * (stride - 1) + q - 1);
     };
+    // D of state q of segment s: dbuf[(s % 2) stride + q]
+    auto dfield = [&](int sg, int q) -> DF<double> { return view<double>(dbuf[(sg % 2) * stride + q]); };
     auto len = [&](int sg) { return std::min(stride, nsteps - sg * stride); };
     // recompute the last segment on its own
     for (int q = 1; q < len(nseg - 1); ++q) {
         const chmy_field_t s0 = state(nseg - 1, q - 1), s1 = state(nseg - 1, q);
         SiaStep<double> f = sia_stepper(g, &s0, &s1, B, A, p, dts[(nseg - 1) * stride + q - 1]);
+        if (dbuf) {
+            f.wd = 1;
+            f.Dout = dfield(nseg - 1, q - 1);
+        }
         launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
     }
     int slot = 0;   // lam_in of the next adjoint step: 0 = lam_a, 1 = lam_b
@@ -649,6 +672,10 @@ static int sia_reverse_sweep(const chmy_grid_t* g, const chmy_field_t* ckpt, int
             if (fw) {
                 const chmy_field_t s0 = state(sg - 1, r), s1 = state(sg - 1, r + 1);
                 f = sia_stepper(g, &s0, &s1, B, A, p, dts[(sg - 1) * stride + r]);
+                if (dbuf) {
+                    f.wd = 1;
+                    f.Dout = dfield(sg - 1, r);
+                }
             }
             if (r >= L) {
                 launch_step_tile(sia_tiles(nx, ny, 1), st, f, nx, ny);
@@ -660,6 +687,10 @@ static int sia_reverse_sweep(const chmy_grid_t* g, const chmy_field_t* ckpt, int
             const chmy_field_t sq = state(sg, q);
             SiaAdj<double> a = sia_adjointer(g, &sq, li, lo, A ? grad_logA : nullptr, B, A, p,
                                              dts[sg * stride + q]);
+            if (dbuf && q + 1 < L) {   // stored by the recompute step q -> q + 1
+                a.rd = 1;
+                a.Din = dfield(sg, q);
+            }
             launch_adj_tile(sia_tiles(nx, ny, fw ? 2 : 1), st, a, f, nx, ny);
             slot ^= 1;
         }
@@ -704,6 +735,29 @@ int chmy::sia_scratch_ptr(void** p) {
 }
 
 namespace chmy {
+
+// Device memory for the D fields of chmy_sia_reverse: one buffer per device,
+// grown on demand and kept until the process ends (calls on one device that
+// overlap race, as with any shared workspace).
+static std::mutex g_dcache_mu;
+static void* g_dcache[64] = {nullptr};
+static size_t g_dcache_sz[64] = {0};
+
+static int sia_dcache(size_t bytes, void** p) {
+    int dev = 0;
+    CHMY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(CHMY_E_ARG, "device index %d", dev);
+    std::lock_guard<std::mutex> lk(g_dcache_mu);
+    if (g_dcache_sz[dev] < bytes) {
+        if (g_dcache[dev]) CHMY_HIP(hipFree(g_dcache[dev]));
+        g_dcache[dev] = nullptr;
+        g_dcache_sz[dev] = 0;
+        CHMY_HIP(hipMalloc(&g_dcache[dev], bytes));
+        g_dcache_sz[dev] = bytes;
+    }
+    *p = g_dcache[dev];
+    return 0;
+}
 
 static dim3 sia_blk() { return dim3(64, 4, 1); }
 static dim3 sia_grid(int64_t nx, int64_t ny) {
@@ -836,8 +890,22 @@ int chmy_sia_reverse(const chmy_grid_t* g, const chmy_field_t* ckpt, int nckpt, 
     CHMY_TRY(check_center(g, lam_b, "lam_b", CHMY_F64));
     if (lam_a->data == lam_b->data) return set_error(CHMY_E_ARG, "lam_a and lam_b must differ");
     const SiaPar p{As, npow, mb_a, mb_b, mb_c, 0.0};
+    // D of the segments' states (2 stride Center fields) in library-held device
+    // memory, so that the adjoint steps read what the recompute steps evaluated
+    std::vector<chmy_field_t> dbuf;
+    if (stride > 1) {
+        const size_t fbytes = (size_t)(pad_of(*lam_a, 0) * pad_of(*lam_a, 1)) * sizeof(double);
+        const size_t fstride = (fbytes + 255) / 256 * 256;
+        char* base = nullptr;
+        CHMY_TRY(sia_dcache(2 * (size_t)stride * fstride, reinterpret_cast<void**>(&base)));
+        for (int i = 0; i < 2 * stride; ++i) {
+            chmy_field_t f = *lam_a;
+            f.data = base + (size_t)i * fstride;
+            dbuf.push_back(f);
+        }
+    }
     *lam_slot = sia_reverse_sweep(g, ckpt, stride, nsteps, [&](int i) { return seg[i]; }, lam_a, lam_b, grad_logA,
-                                  B, A, p, dts, (hipStream_t)stream);
+                                  B, A, p, dts, (hipStream_t)stream, dbuf.empty() ? nullptr : dbuf.data());
     CHMY_HIP(hipGetLastError());
     return 0;
 }
