@@ -390,9 +390,13 @@ int chmy_sia_run(const chmy_grid_t* grid, const chmy_field_t* states, int nstate
  * state before step s*stride (nckpt = ceil(nsteps/stride)); each segment is
  * recomputed once into seg (2*(stride-1) Center fields, two alternating
  * halves), the recompute of segment s-1 riding in the same launches as the
- * adjoint steps of segment s.  lam_a holds dJ/dH_N on entry; on return the
- * field named by *lam_slot (0 = lam_a, 1 = lam_b) holds dJ/dH_0, and
- * grad_logA has dJ/dlog(A) added (A != NULL). */
+ * adjoint steps of segment s.  The recompute steps also store the
+ * diffusivity D of their starting state, which the adjoint steps read, in
+ * device memory the library holds (2*stride Center fields per device, grown
+ * on demand, kept until the process ends; calls that overlap on one device
+ * race).  lam_a holds dJ/dH_N on entry; on return the field named by
+ * *lam_slot (0 = lam_a, 1 = lam_b) holds dJ/dH_0, and grad_logA has
+ * dJ/dlog(A) added (A != NULL). */
 int chmy_sia_reverse(const chmy_grid_t* grid, const chmy_field_t* ckpt, int nckpt, int stride, int nsteps,
                      const chmy_field_t* seg, const chmy_field_t* lam_a, const chmy_field_t* lam_b,
                      const chmy_field_t* grad_logA, const chmy_field_t* B, const chmy_field_t* A, double As,
