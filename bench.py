@@ -518,6 +518,68 @@ def stokes_gradient(args):
     print(json.dumps(res))
 
 
+def thermal_loop(args):
+    """The physical time loop's PT iterations with the thermal update inside
+    (examples/stokes_3d_inc_ve_T.jl:161-169, time steps after the first): per
+    iteration update_stress!, update_velocity! + V batch, update_thermal_flux!,
+    update_thermal! + T batch.  3D FP64 at --n, rho g_z and T inclusions of
+    radius 0.1 L (the example's setup), eta = 10, --steps iterations with no
+    residual check (the example checks every 2 n_x iterations in its second
+    step).  Timed two ways from the same state: one fused iteration per
+    launch, and two iterations per pass with the first iteration's V kept for
+    its thermal update (chmy_stokes_iter2_first); the final V, Pr, tau and T
+    of the two are compared bitwise."""
+    import numpy as np
+    import torch
+    import chmy
+    from chmy import Center as Cc, Vertex as Vv
+    from chmy.stokes_thermal import StokesThermal
+    arch = chmy.Arch(chmy.MI355XBackend())
+    n, K = args.n, args.steps
+    L = 1.0
+    g = chmy.UniformGrid(arch, origin=(-L / 2,) * 3, extent=(L,) * 3, dims=(n, n, n))
+
+    def incl(locs, inside, outside):
+        xs = [g.coords(Vv() if c == "V" else Cc(), d, first=1, count=n + (1 if c == "V" else 0))
+              for d, c in enumerate(locs)]
+        Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+        arr = np.zeros(tuple(len(x) + 4 for x in xs[::-1]))
+        arr[2:-2, 2:-2, 2:-2] = np.where(X * X + Y * Y + Z * Z < (0.1 * L) ** 2, inside, outside)
+        return arr
+
+    rg_arr, T_arr = incl("CCV", 1.0, 0.0), incl("CCC", 1.0, 0.1)
+    res, keep = {}, None
+    for mode in ("single", "pairs"):
+        rhog = chmy.Field(arch, g, (Cc(), Cc(), Vv()))
+        chmy.set_storage(rhog, rg_arr)
+        M = StokesThermal(arch, g, eta=10.0, G=1.0, rhog=rhog, exact=args.exact, pairs=mode == "pairs")
+        chmy.set_storage(M.T, T_arr)
+        M.apply_bc()
+        M.time_step(2, 4, 2 * n, 0.0)   # warm-up: one step of 4 iterations
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        M.time_step(2, K, 2 * n, 0.0)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        fields = [f.data for f in list(M.S.V) + [M.S.Pr] + list(M.S.tau) + [M.T]]
+        if keep is None:
+            keep = [f.cpu() for f in fields]
+            same = None
+        else:
+            same = all(torch.equal(a, b.cpu()) for a, b in zip(keep, fields))
+        res[mode] = {"ms_per_iteration": round(sec * 1e3 / K, 4), "seconds": round(sec, 3)}
+        print(f"# thermal loop {mode}: {res[mode]['ms_per_iteration']} ms per iteration", file=sys.stderr, flush=True)
+        del M, rhog, fields
+        torch.cuda.empty_cache()
+    res["pairs"]["bitwise_equal_to_single"] = same
+    speed = res["single"]["ms_per_iteration"] / res["pairs"]["ms_per_iteration"]
+    print(json.dumps({"metric": "3D PT Stokes + thermal loop, ms per PT iteration", "value": res["pairs"]["ms_per_iteration"],
+                      "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+                      "config": {"workload": f"3D VE Stokes PT with the thermal update per iteration, {n}^3 cells, "
+                                             f"{K} iterations, {'exact' if args.exact else 'fast'} form"},
+                      "runs": res, "speedup_pairs_vs_single": round(speed, 3)}))
+
+
 def pairs_rate(S, args, isz):
     """3D PT iterations two per pass (chmy_stokes_iterate with the pairs flag:
     fused3d_tb_kernel, then the V batch); one step = one pass = two
@@ -584,6 +646,9 @@ def main():
     ap.add_argument("--fd-eps", type=float, default=1e-11)
     ap.add_argument("--stokes2d", action="store_true",
                     help="2D fused PT iteration rates at --n (default 2047): FP64, FP32, defect-correction sweep")
+    ap.add_argument("--thermal-loop", action="store_true",
+                    help="3D PT iterations with the thermal update inside (the time loop after its first step): "
+                         "one iteration per launch vs two per pass, --steps iterations at --n")
     ap.add_argument("--diffusion", action="store_true",
                     help="2D diffusion 256^2 x 1000 steps: reference launches vs fused step vs native loop, vs the oracle")
     args = ap.parse_args()
@@ -601,6 +666,9 @@ def main():
         return
     if args.diffusion:
         diffusion_c1(args)
+        return
+    if args.thermal_loop:
+        thermal_loop(args)
         return
     if args.stokes2d:
         if args.n == 255:

@@ -100,6 +100,8 @@ SIGNATURES = {
                             VP],
     "chmy_stokes_iter2": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32), P(C.c_int32),
                           P(ChmyBC), C.c_int, D, D, D, D, D, D, D, VP],
+    "chmy_stokes_iter2_first": [PG, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, PF, P(C.c_int32), P(C.c_int32),
+                                P(ChmyBC), C.c_int, D, D, D, D, D, D, D, VP],
     "chmy_stokes_residual": [PG, PF, PF, PF, PF, PF, PF, PF, D, D, D, PF, PF, PF, PF, PF, VP],
     "chmy_stokes_accumulate": [PG, PF, PF, PF, PF, PF, PF, VP],
     "chmy_stokes_grad_eta": [PG, PF, PF, PF, PF, C.c_int, VP],

@@ -217,6 +217,25 @@ class PTStokes:
         self._swap()
         self.iters += 2
 
+    def pass2_first(self, V_first):
+        """pass2() that also leaves the first iteration's V, after its V batch,
+        in the VectorField V_first (chmy_stokes_iter2_first; 3D, FP64, no
+        forcing): bitwise V after the first of two iteration() calls at every
+        point either writes.  V_first's other points keep their values."""
+        g = self.grid
+        counts, comp, conds, _keep = self._vbatch()
+        _lib.call("chmy_stokes_iter2_first", C.byref(g.desc()), _arr(self.V), _arr(self.tau),
+                  C.byref(self.Pr.desc()), _arr(self.V2), _arr(self.tau2), C.byref(self.Pr2.desc()), _arr(V_first),
+                  _arr(self.tau_old), C.byref(self.eta.desc()) if self.eta is not None else None,
+                  C.byref(self.eta_ml.desc()) if self.eta_ml is not None else None,
+                  C.byref(self.rhog.desc()) if self.rhog is not None else None,
+                  counts, comp, conds, ITER_EXACT if self.exact else 0, self.eta_s, self.eta_ve_s, self.G,
+                  self.dt, self.p["dtau_Pr"], self.p["dtau_r"], self.p["nudtau"], current_stream())
+        self.apply_bc(V_first)
+        self.apply_bc(self.V2)
+        self._swap()
+        self.iters += 2
+
     def iteration_unfused(self):
         """The reference's sequence: update_stress!, update_velocity! (per-point
         kernels, single buffers, r_V and ∇V written), then the V batch."""

@@ -8,7 +8,9 @@ update_stress! + update_velocity! with the V batch, followed from the second
 time step on by update_thermal_flux!, update_thermal! and the T Neumann
 batch (one launch into the second of two T buffers, then the batch on it:
 the same values as the two launches in place; in the first step, with no
-thermal update, the iterations between checks run in one native call);
+thermal update, the iterations between checks run in one native call; in 3D
+two iterations between checks run as one pass that also keeps the first
+iteration's V for its thermal update, chmy_stokes_iter2_first);
 every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
 boundary vertices, scaled by τsc, lz/psc) decides convergence, and from the
 third step on ncheck = ceil(0.5 n_x).  With exact=True the states are
@@ -30,7 +32,7 @@ from .stokes import PTStokes
 
 class StokesThermal:
     def __init__(self, arch, grid, *, eta=10.0, G=1.0, lam_rhoCp=None, rhog=None, normal_values=None,
-                 re_m=2.3 * math.pi, r=0.5, exact=False):
+                 re_m=2.3 * math.pi, r=0.5, exact=False, pairs=True):
         self.arch, self.grid = arch, grid
         self.eta, self.G = float(eta), float(G)
         L = grid.extent
@@ -46,6 +48,10 @@ class StokesThermal:
         self.qT = VectorField(arch, grid)
         self.launch = Launcher(arch, grid, sync=False)
         self.scales = dict(tau_sc=tau_sc, lz=L[-1], psc=self.G)
+        # 3D: two PT iterations per pass inside the thermal PT loop, with the
+        # first iteration's V kept in _V1 for its thermal update
+        self.pairs = bool(pairs) and grid.nd == 3
+        self._V1 = VectorField(arch, grid) if self.pairs else None
 
     @property
     def T(self):
@@ -56,6 +62,9 @@ class StokesThermal:
         self.S.apply_bc()
         self._Tbcs[self._tp].apply()
         self.S.sync_buffers()
+        if self._V1 is not None:   # points no pass writes agree with V's
+            for a, b in zip(self.S.V, self._V1):
+                b.data.copy_(a.data)
 
     def dt_rule(self):
         g = self.grid
@@ -93,20 +102,36 @@ class StokesThermal:
             return niter, err
         # both T buffers start equal, so points no launch writes agree
         self._Tbuf[1 - self._tp].data.copy_(self.T.data)
-        for q in range(1, niter + 1):
+        q = 1
+        while q <= niter:
             check = q % ncheck == 0
+            if self.pairs and not check and q < niter and (q + 1) % ncheck != 0:
+                # iterations q and q+1 in one pass; each thermal update reads
+                # the V of its own iteration (Stokes does not read T)
+                S.pass2_first(self._V1)
+                self._thermal(self._V1, dt)
+                self._thermal(S.V, dt)
+                q += 2
+                continue
             S.iteration(write_res=check)
-            src, dst = self._tp, 1 - self._tp
-            self.launch(self.arch, g, (thermal_step, (self._Tbuf[src], self._Tbuf[dst], self.T_old, self.qT,
-                                                      S.V, self.lam, dt, g)), bc=self._Tbcs[dst])
-            self._tp = dst
+            self._thermal(S.V, dt)
             if check:
                 err = S.residual(**self.scales)
                 if all(e < eps for e in err):
                     return q, err
                 if not all(math.isfinite(e) for e in err):
                     raise FloatingPointError(f"simulation failed, err = {err}")
+            q += 1
         return niter, err
+
+    def _thermal(self, V, dt):
+        """update_thermal_flux! + update_thermal! (one launch into the other T
+        buffer) and the T Neumann batch on it."""
+        g = self.grid
+        src, dst = self._tp, 1 - self._tp
+        self.launch(self.arch, g, (thermal_step, (self._Tbuf[src], self._Tbuf[dst], self.T_old, self.qT,
+                                                  V, self.lam, dt, g)), bc=self._Tbcs[dst])
+        self._tp = dst
 
     def run(self, nt=2, niter=None, ncheck=None, eps=1e-6):
         """The reference's defaults: niter = 50 n_x, ncheck = 2 n_x, then

@@ -980,9 +980,12 @@ __device__ __forceinline__ float xp1(float v) { return __int_as_float(dpp_shl1(_
 // bitwise those of two fused3d_kernel launches with the batch between them.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF>
 __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a, FreeSlip3 fs) {
-    static_assert(OPT == 0 || OPT == 1 || OPT == 2, "two-iteration pass: plain, forcing-only or rho g-only");
+    static_assert(OPT >= 0 && OPT <= 4, "two-iteration pass: plain, forcing, rho g; 3, 4: plain, rho g + first V");
+    static_assert(OPT < 3 || sizeof(T) == 8, "the first iteration's V is written in FP64 only");
     constexpr int BY = kTB3Y<T>, NR = BY + 1, TW = kFX;
-    constexpr bool HF = OPT == 1, RG = OPT == 2;
+    // WA: A's V (the first iteration's, before its V batch) is stored into
+    // a.rVx/rVy/rVz at the cells and planes B stores its own V
+    constexpr bool HF = OPT == 1, RG = OPT == 2 || OPT == 4, WA = OPT >= 3;
     using AR = Arith<T, EXACT>;
     using Loc = typename AR::Loc;
     const Kc<T>& K = a.k;
@@ -1311,6 +1314,13 @@ __global__ void __launch_bounds__(kFX * kTB3Y<T>) fused3d_tb_kernel(Fused3<T> a,
             const T vxa = keep ? AR::vel(Avx, rx, ex, K) : Avx;
             const T vya = keep ? AR::vel(Avy, ry, ey, K) : Avy;
             const T vza = keep ? AR::vel(Avz, rz, ez, K) : Avz;
+            if constexpr (WA) {
+                if (s >= kb && s < ke && ownc) {
+                    sto(a.rVx, o(LVX), vxa);
+                    sto(a.rVy, o(LVY), vya);
+                    sto(a.rVz, o(LVZ), vza);
+                }
+            }
             // A's carried state for step s+1
             AP_m = P;
             Atzz_m = tzz;
@@ -1603,7 +1613,8 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
                         const chmy_field_t* rg, const chmy_field_t* fV, const chmy_field_t* fP,
                         const chmy_field_t* dV, const chmy_field_t* rV, int wres, bool exact,
                         double eta_s, double eve_s, double G, double dt, double dPr, double dr, double nu,
-                        hipStream_t s, const FreeSlip2* tb = nullptr, const FreeSlip3* tb3 = nullptr) {
+                        hipStream_t s, const FreeSlip2* tb = nullptr, const FreeSlip3* tb3 = nullptr,
+                        const chmy_field_t* vfirst = nullptr) {
     const bool fv = eta != nullptr;
     const bool bf = told[0].dtype == CHMY_BF16;   // validated: FP32 sweep with bf16 forcing
     // every byte offset a thread forms must fit in 32 bits
@@ -1663,6 +1674,27 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
             const unsigned nb = (unsigned)a.ntx * a.nty * a.nchunks;
             dim3 blk(kFX, kTB3Y<T>, 1), grd(nb, 1, 1);
             const bool hf = a.has_f, hr = a.has_rg;
+            if (vfirst) {   // validated: FP64, no forcing
+                if constexpr (std::is_same<T, double>::value) {
+                    a.rVx = P(vfirst[0]);
+                    a.rVy = P(vfirst[1]);
+                    a.rVz = P(vfirst[2]);
+#define CHMY_TB3F(FV, EX)                                                                              \
+    do {                                                                                               \
+        if (hr) hipLaunchKernelGGL((fused3d_tb_kernel<double, FV, EX, 4, false>), grd, blk, 0, s, a, *tb3); \
+        else hipLaunchKernelGGL((fused3d_tb_kernel<double, FV, EX, 3, false>), grd, blk, 0, s, a, *tb3);    \
+    } while (0)
+                    if (fv && exact) CHMY_TB3F(true, true);
+                    else if (fv) CHMY_TB3F(true, false);
+                    else if (exact) CHMY_TB3F(false, true);
+                    else CHMY_TB3F(false, false);
+#undef CHMY_TB3F
+                    CHMY_HIP(hipGetLastError());
+                    return 0;
+                } else {
+                    return set_error(CHMY_E_DTYPE, "the first iteration's V is written in FP64 only");
+                }
+            }
 #define CHMY_TB3(FV, EX)                                                                                    \
     do {                                                                                                    \
         constexpr bool F32 = std::is_same<T, float>::value;                                                 \
@@ -1816,7 +1848,8 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
                             const chmy_field_t* eta_ml, const chmy_field_t* rhog, const chmy_field_t* f_V,
                             const chmy_field_t* f_P, const chmy_field_t* divV, const chmy_field_t* r_V, int flags,
                             double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
-                            double nudtau, void* stream, const FreeSlip2* tb, const FreeSlip3* tb3) {
+                            double nudtau, void* stream, const FreeSlip2* tb, const FreeSlip3* tb3,
+                            const chmy_field_t* V_first = nullptr) {
     CHMY_TRY(check_grid(g));
     if ((tb && g->nd != 2) || (tb3 && g->nd != 3) || (tb3 && rhog && f_V) || ((tb || tb3) && (flags & CHMY_ITER_RESIDUAL)))
         return set_error(CHMY_E_ARG, "two-iteration pass: free-slip batch of the grid's dims, no residual fields "
@@ -1877,10 +1910,21 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
     for (int c = 0; c < nt; ++c)
         if (tau[c].data == tau_out[c].data) return set_error(CHMY_E_ARG, "tau and tau_out must be distinct buffers");
     if (Pr->data == Pr_out->data) return set_error(CHMY_E_ARG, "Pr and Pr_out must be distinct buffers");
+    if (V_first) {   // the first iteration's V: FP64 3D passes without forcing
+        if (!tb3 || dt_ != CHMY_F64 || f_V)
+            return set_error(CHMY_E_ARG, "the first iteration's V is written by FP64 3D passes without forcing");
+        CHMY_TRY(check_vector(g, V_first, "V_first", dt_));
+        for (int d = 0; d < nd; ++d) {
+            if (!halo1(&V_first[d])) return set_error(CHMY_E_HALO, "fused iteration needs halo 1 on every field");
+            for (int e = 0; e < nd; ++e)
+                if (V_first[d].data == V[e].data || V_first[d].data == V_out[e].data)
+                    return set_error(CHMY_E_ARG, "V_first must be distinct from V and V_out");
+        }
+    }
     return dt_ == CHMY_F64
                ? fused_launch<double>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                       write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
-                                      (hipStream_t)stream, tb, tb3)
+                                      (hipStream_t)stream, tb, tb3, V_first)
                : fused_launch<float>(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, divV, r_V,
                                      write_residual, exact, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau,
                                      (hipStream_t)stream, tb, tb3);
@@ -2065,6 +2109,39 @@ int chmy_stokes_iter2(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fi
     return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, f_V, f_P, nullptr,
                             nullptr, flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream,
                             tb2 ? &fs : nullptr, tb3 ? &fs3 : nullptr);
+}
+
+// chmy_stokes_iter2 that also writes the first iteration's V (before its V
+// batch) into V_first, at the points the pass writes V_out (build extension):
+// in the physical time loop update_thermal_flux! reads V after every PT
+// iteration.  3D, FP64, no forcing; the caller applies the V batch to V_first
+// and to V_out.
+int chmy_stokes_iter2_first(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau,
+                            const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
+                            const chmy_field_t* Pr_out, const chmy_field_t* V_first, const chmy_field_t* tau_old,
+                            const chmy_field_t* eta, const chmy_field_t* eta_ml, const chmy_field_t* rhog,
+                            const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int flags,
+                            double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                            double nudtau, void* stream) {
+    CHMY_TRY(check_grid(g));
+    if (!V_first) return set_error(CHMY_E_ARG, "V_first is NULL");
+    if (!bc_counts || !bc_comp || !bcs) return set_error(CHMY_E_ARG, "V batch is NULL");
+    if (flags & ~CHMY_ITER_EXACT) return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first takes CHMY_ITER_EXACT only");
+    if (g->nd != 3) return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first is 3D only");
+    int total = 0;
+    for (int q = 0; q < 6; ++q) {
+        if (bc_counts[q] < 0) return set_error(CHMY_E_ARG, "negative batch count");
+        total += bc_counts[q];
+    }
+    for (int q = 0; q < total; ++q)
+        if (bc_comp[q] < 0 || bc_comp[q] >= 3) return set_error(CHMY_E_ARG, "bc_comp out of range");
+    FreeSlip3 fs3;
+    if (!free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3))
+        return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first needs the free-slip V batch (each component Dirichlet "
+                                     "on its own dim, Neumann on the others, scalar values, >= 2 cells per dim)");
+    return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, nullptr, nullptr,
+                            nullptr, nullptr, flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream, nullptr,
+                            &fs3, V_first);
 }
 
 }  // extern "C"

@@ -26,7 +26,9 @@ ChmyAMDGPUExt is replaced:
   - `set!` on MI355X fields: Chmy's own CPU `set!` into a host field, then one
     upload (bitwise Chmy's CPU values);
   - `iterate!` (fused PT iterations with the V batch, in two-iteration passes
-    for free-slip batches) and `iter2!` (one such pass); `stokes_residual!` and
+    for free-slip batches), `iter2!` (one such pass) and `iter2_first!` (one
+    such pass that keeps the first iteration's V, for the thermal updates
+    inside the PT loop); `stokes_residual!` and
     `stokes_accumulate!` for FP32 sweeps with FP64 defect correction;
     `maxabs` for the residual norms; `apply_op!` for any GridOperator (plain,
     masked by a FieldMask component, on uniform or FunctionAxis grids);
@@ -703,6 +705,40 @@ function iter2!(state0, state1, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V, p
                     Ptr{ChmyField}(C_NULL), Ptr{ChmyField}(C_NULL), Ptr{ChmyField}(C_NULL), counts, comps, cs,
                     Cint(flags), _eta_s(η), Float64(p.η_ve), Float64(p.G), Float64(p.dt), Float64(p.dτ_Pr),
                     Float64(p.dτ_r), Float64(p.νdτ), current_stream()))
+    end
+    return state1
+end
+
+"""
+    iter2_first!(state0, state1, V_first, τ_old, η, η_ml, grid, bc_V, p; rhog=nothing, flags=0)
+
+`iter2!` that also stores the first iteration's V (before its V batch) into
+`V_first` (3D, Float64; build extension).  After the V batch on `V_first`
+and on `state1`'s V, `V_first` is bitwise V after the first of the two
+iterations: the V that the first iteration's `update_thermal_flux!` reads in
+the physical time loop (stokes_3d_inc_ve_T.jl:164-169).  `rhog` is the
+`ρgz` field update_velocity! subtracts.
+"""
+function iter2_first!(state0, state1, V_first, τ_old, η, η_ml, grid::UniformGrid{3}, bc_V, p;
+                      rhog=nothing, flags::Integer=0)
+    (V0, τ0, P0), (V1, τ1, P1) = state0, state1
+    counts, comps, cs, keep = vbatch(grid, bc_V, V0)
+    ηd = η isa Number ? nothing : Ref(desc(η))
+    md = η_ml === nothing ? nothing : Ref(desc(η_ml))
+    rd = rhog === nothing ? nothing : Ref(desc(rhog))
+    GC.@preserve keep ηd md rd begin
+        ηp = ηd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, ηd)
+        mp = md === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, md)
+        rp = rd === nothing ? Ptr{ChmyField}(C_NULL) : Base.unsafe_convert(Ptr{ChmyField}, rd)
+        check(ccall((:chmy_stokes_iter2_first, LIB), Cint,
+                    (Ref{ChmyGrid}, Ptr{ChmyField}, Ptr{ChmyField}, Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ref{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField}, Ptr{ChmyField},
+                     Ptr{Int32}, Ptr{Int32}, Ptr{ChmyBC}, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble,
+                     Cdouble, Ptr{Cvoid}),
+                    desc(grid), descs(V0), descs(τ0), desc(P0), descs(V1), descs(τ1), desc(P1), descs(V_first),
+                    descs(τ_old), ηp, mp, rp, counts, comps, cs, Cint(flags), _eta_s(η), Float64(p.η_ve),
+                    Float64(p.G), Float64(p.dt), Float64(p.dτ_Pr), Float64(p.dτ_r), Float64(p.νdτ),
+                    current_stream()))
     end
     return state1
 end

@@ -317,6 +317,24 @@ int chmy_stokes_iter2(const chmy_grid_t* grid, const chmy_field_t* V, const chmy
                       const chmy_bc_t* bcs, int flags, double eta_s, double eta_ve_s, double G, double dt,
                       double dtau_Pr, double dtau_r, double nudtau, void* stream);
 
+/* chmy_stokes_iter2 that also stores the first iteration's V, before its V
+ * batch, into V_first (build extension).  In the physical time loop the
+ * reference runs update_thermal_flux! and update_thermal! after every PT
+ * iteration, reading V (examples/stokes_3d_inc_ve_T.jl:164-169); with V_first
+ * two PT iterations can run as one pass and both thermal updates still see
+ * their own V.  V_first is written at exactly the points the pass writes
+ * V_out; the caller applies the V batch to both.  Then V_first is bitwise
+ * V after the first chmy_stokes_iter + batch.  3D, FP64, no f_V/f_P (else
+ * CHMY_E_ARG); V_first distinct from V and V_out; other arguments as in
+ * chmy_stokes_iter2. */
+int chmy_stokes_iter2_first(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
+                            const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
+                            const chmy_field_t* Pr_out, const chmy_field_t* V_first, const chmy_field_t* tau_old,
+                            const chmy_field_t* eta, const chmy_field_t* eta_ml, const chmy_field_t* rhog,
+                            const int32_t* bc_counts, const int32_t* bc_comp, const chmy_bc_t* bcs, int flags,
+                            double eta_s, double eta_ve_s, double G, double dt, double dtau_Pr, double dtau_r,
+                            double nudtau, void* stream);
+
 /* Defect correction (build extension; the reference has no mixed-precision
  * path).  Residuals of an f64 state in the reference's expression order
  * (update_stress!/update_velocity!, examples/stokes_3d_inc_ve_T.jl:23-57):

@@ -9,7 +9,8 @@ rho g and forcing together); the reference here is the single-iteration path,
 driven one iteration at a time (chmy_stokes_iter + chmy_bc_batchset), which
 tests/test_gpu_fused.py pins to the oracle.  Grids cover partial x tiles and
 y chunks, chunks of one row, and the smallest grid the pass takes (2x2).
-chmy_stokes_iter2 (one pass, no batch after it) is checked the same way.
+chmy_stokes_iter2 (one pass, no batch after it) is checked the same way, and
+chmy_stokes_iter2_first's V of the first iteration against one iteration.
 """
 import ctypes as C
 
@@ -112,6 +113,22 @@ def _run(arch, g, T, exact, visc, forcing, vals, niter, pairs, seed, rhog=False,
                          _arr(list(fV)) if fV is not None else None,
                          C.byref(fP.desc()) if fP is not None else None, C.byref(divV.desc()), _arr(list(rV)))
     scal = (eta_s, eve, G, dtp, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
+    if mode == "iter2_first":   # one pass that also writes the first iteration's V, then the batch on both
+        assert niter == 2
+        Vf = chmy.VectorField(arch, g, dt)
+        for f, v in zip(Vf, sets[0].V):
+            f.data.copy_(v.data)
+        a, b = sets[0], sets[1]
+        _lib.call("chmy_stokes_iter2_first", C.byref(g.desc()), _arr(list(a.V)), _arr(list(a.tau)),
+                  C.byref(a.Pr.desc()), _arr(list(b.V)), _arr(list(b.tau)), C.byref(b.Pr.desc()), _arr(list(Vf)),
+                  _arr(list(told)), C.byref(eta.desc()) if eta is not None else None,
+                  C.byref(eml.desc()) if eml is not None else None, C.byref(rg.desc()) if rg is not None else None,
+                  counts, comp, conds, flags, *scal, chmy.current_stream())
+        for V in (b.V, Vf):
+            _lib.call("chmy_bc_batchset", C.byref(g.desc()), counts, _arr([V[c] for c in comp]), conds,
+                      chmy.current_stream())
+        chmy.synchronize()
+        return [f.data.cpu().numpy() for f in b.fields()], [f.data.cpu().numpy() for f in Vf]
     if mode == "iter2":   # one pass per two iterations, the batch after each pass
         assert niter % 2 == 0
         for _ in range(niter // 2):
@@ -254,6 +271,47 @@ def test_iter2_equals_two_single_iterations(arch, nd, T, visc):
         a = _run(arch, g, T, exact, visc, None, vals, 4, True, 41, rhog=True, mode="iter2")
         b = _run(arch, g, T, exact, visc, None, vals, 4, False, 41, rhog=True)
         _compare(a, b, f"iter2 {nd}D exact {exact}")
+
+
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("visc", ["field", "scalar"])
+@pytest.mark.parametrize("rhog", [True, False])
+@pytest.mark.parametrize("dims", [(61, 13, 17), (2, 2, 2), (59, 13, 11), (130, 30, 41)])
+def test_iter2_first_keeps_the_first_iteration(arch, exact, visc, rhog, dims):
+    """chmy_stokes_iter2_first + the batch on both outputs: the pass's state
+    equals two single iterations and V_first equals V after the first."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5, -0.5), extent=(1.0, 1.1, 0.9), dims=dims)
+    for vals in (SHEAR3, GENERAL3):
+        state, vf = _run(arch, g, np.float64, exact, visc, None, vals, 2, True, 51, rhog=rhog, mode="iter2_first")
+        _compare(state, _run(arch, g, np.float64, exact, visc, None, vals, 2, False, 51, rhog=rhog),
+                 f"iter2_first state, dims {dims}")
+        one = _run(arch, g, np.float64, exact, visc, None, vals, 1, False, 51, rhog=rhog)
+        for n, x, y in zip(("Vx", "Vy", "Vz"), vf, one[:3]):
+            if not np.array_equal(x.view(np.int64), y.view(np.int64)):
+                bad = np.argwhere(x.view(np.int64) != y.view(np.int64))
+                raise AssertionError(f"V_first {n} differs at {len(bad)} points, first {bad[:4].tolist()}")
+
+
+def test_iter2_first_rejects_what_it_does_not_run(arch):
+    """An FP32 pass, or an FP32 V_first beside FP64 fields, is refused before any launch."""
+    g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(8, 8, 8))
+    counts, comp, conds = _batch3(GENERAL3)
+    p = pt_params(g)
+    scal = (1.0, 0.5, 1.0, 0.3, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
+
+    def call(dt, Vf_dt=None):
+        a, b = _Set(arch, g, dt), _Set(arch, g, dt)
+        Vf = chmy.VectorField(arch, g, Vf_dt or dt)
+        told = chmy.TensorField(arch, g, dt)
+        return _lib.lib().chmy_stokes_iter2_first(
+            C.byref(g.desc()), _arr(list(a.V)), _arr(list(a.tau)), C.byref(a.Pr.desc()), _arr(list(b.V)),
+            _arr(list(b.tau)), C.byref(b.Pr.desc()), _arr(list(Vf)), _arr(list(told)), None, None, None,
+            counts, comp, conds, 0, *scal, chmy.current_stream())
+
+    assert call(torch.float32) != 0
+    assert call(torch.float64, torch.float32) != 0
+    assert call(torch.float64) == 0
+    chmy.synchronize()
 
 
 def test_single_flag_keeps_results(arch):

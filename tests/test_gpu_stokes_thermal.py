@@ -110,3 +110,31 @@ def test_time_loop_matches_reference_sequence(arch, nd):
     sl = (slice(1, -1),) * nd     # logical 0..n+2: interior, halos, boundary nodes
     for a, b in zip(list(M.S.V) + [M.S.Pr] + list(M.S.tau) + [M.T], Vf + [Pr] + tau + [T]):
         assert np.array_equal(dev(a)[sl], b.arr[sl])
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_time_loop_pairs_equal_single_iterations(arch, exact):
+    """3D: the thermal PT loop in passes of two iterations (the first
+    iteration's V kept for its thermal update, chmy_stokes_iter2_first)
+    against one iteration per launch: same iteration counts and residuals,
+    bitwise equal V, Pr, tau and T.  An odd ncheck mixes passes with the
+    single iterations before and at each check."""
+    n = (21, 17, 14)
+    L = 2.0
+    g = chmy.UniformGrid(arch, origin=(-L / 2,) * 3, extent=(L,) * 3, dims=n)
+    rg_arr = _incl(g, ("C", "C", "V"), 1.0, 0.0, 0.15 * L)
+    T_arr = _incl(g, ("C", "C", "C"), 1.0, 0.1, 0.15 * L)
+    runs = []
+    for pairs in (True, False):
+        rhog = chmy.Field(arch, g, (Cc(), Cc(), Vv()))
+        chmy.set_storage(rhog, rg_arr)
+        M = StokesThermal(arch, g, eta=10.0, G=1.0, rhog=rhog, exact=exact, pairs=pairs)
+        chmy.set_storage(M.T, T_arr)
+        M.apply_bc()
+        res = M.run(nt=3, niter=160, ncheck=7, eps=1e-9)
+        runs.append((res, [dev(f) for f in list(M.S.V) + [M.S.Pr] + list(M.S.tau) + [M.T]]))
+    (ra, fa), (rb, fb) = runs
+    assert ra == rb
+    assert not np.array_equal(fa[-1], T_arr)
+    for k, (x, y) in enumerate(zip(fa, fb)):
+        assert np.array_equal(x.view(np.int64), y.view(np.int64)), k
