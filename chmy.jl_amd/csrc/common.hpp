@@ -167,6 +167,15 @@ inline dim3 grid_for(const Box3& b, dim3 blk) {
                 (unsigned)((b.len[2] + blk.z - 1) / blk.z));
 }
 
+// Block id -> tile, so that the blocks an XCD receives (ids congruent mod 8,
+// dispatched round-robin over the 8 XCDs) form one contiguous run of tiles and
+// share halo rows in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int per = nb / 8;
+    if (bid >= 8 * per) return bid;
+    return (bid % 8) * per + bid / 8;
+}
+
 // Julia max/min against zero for floats (NaN propagates, max(-0.0, 0.0) = 0.0).
 template <class T>
 __device__ __forceinline__ T jmax0(T x) { return x != x ? x : (x > T(0) ? x : T(0)); }

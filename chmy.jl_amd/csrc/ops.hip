@@ -442,6 +442,74 @@ __global__ void __launch_bounds__(256) thermal_step_kernel(Box3 b, DFL<T> Tf, DF
     g_at(Tout, I) = g_at(To, I) - dt * dv;
 }
 
+// The same step in 3D as a z-march: a block is 64 (x, one wave) x kThY (y)
+// points and walks a chunk of at most 16 planes, carrying T at the plane and z's flux
+// (qT_z at k+1 computed at plane k is qT_z at plane k+1, the same
+// expression on the same operands), so T and V_z stream from HBM once
+// instead of being fetched again by the blocks of the neighbouring planes.
+// x and y neighbours are loads of the same or adjacent rows, found in L2:
+// blocks are remapped so that each XCD walks a contiguous run of tiles.
+// Per point: the same expressions, association and stored qT as
+// thermal_step_kernel; fields of one staggered location share strides
+// (the host checks it).  Lanes outside the box load at clamped indices and
+// store nothing.
+constexpr int kThY = 4;   // rows per block; with 16-plane chunks the fastest of 1-16 rows x 8-64 planes
+
+template <class T>
+__device__ __forceinline__ T th_flux(T tc, T tl, T v, T nlam, T isp) {
+    return (nlam * ((tc - tl) * isp) + jmax0(v) * tl) + jmin0(v) * tc;
+}
+
+template <class T>
+__global__ void __launch_bounds__(64 * kThY) thermal_step3_kernel(Box3 b, DF<T> Tf, DF<T> Tout, DF<T> To, DF<T> V0,
+                                                                  DF<T> V1, DF<T> V2, DF<T> q0, DF<T> q1, DF<T> q2,
+                                                                  T nlam, T dt, T i0, T i1, T i2, int nbx, int nby,
+                                                                  int kchunk) {
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = lt % nbx, by = (lt / nbx) % nby, bz = lt / (nbx * nby);
+    const int64_t hi0 = b.lo[0] + b.len[0], hi1 = b.lo[1] + b.len[1], hi2 = b.lo[2] + b.len[2];
+    const int64_t i = b.lo[0] + (int64_t)bx * 64 + lane, j = b.lo[1] + (int64_t)by * kThY + r;
+    const int64_t kb = b.lo[2] + (int64_t)bz * kchunk;
+    if (kb >= hi2) return;   // a last chunk past the box (whole block: no barrier follows)
+    const int64_t ke = kb + kchunk < hi2 ? kb + kchunk : hi2;
+    const bool act = i < hi0 && j < hi1;
+    const int64_t ic = i < hi0 ? i : hi0 - 1, jc = j < hi1 ? j : hi1 - 1;
+    const bool xr = ic + 1 < hi0, yr = jc + 1 < hi1;   // the right face's flux is computed, else read
+    const int64_t sC = Tf.sz, sX = V0.sz, sY = V1.sz, sZ = V2.sz;
+    const int64_t yC = Tf.sy, yY = V1.sy;
+    int64_t oC = ic + jc * yC + kb * sC, oX = ic + jc * V0.sy + kb * sX;
+    int64_t oY = ic + jc * yY + kb * sY, oZ = ic + jc * V2.sy + kb * sZ;
+    T tc = Tf.p[oC];
+    T qlz = th_flux(tc, Tf.p[oC - sC], V2.p[oZ], nlam, i2);
+    for (int64_t k = kb; k < ke; ++k) {
+        const T tp = Tf.p[oC + sC], vz1 = V2.p[oZ + sZ];
+        const T txm = Tf.p[oC - 1], txp = Tf.p[oC + 1];
+        const T tym = Tf.p[oC - yC], typ = Tf.p[oC + yC];
+        const T vx = V0.p[oX], vx1 = V0.p[oX + 1];
+        const T vy = V1.p[oY], vy1 = V1.p[oY + yY];
+        const T to = To.p[oC];
+        const T qlx = th_flux(tc, txm, vx, nlam, i0);
+        const T qly = th_flux(tc, tym, vy, nlam, i1);
+        const T qrx = xr ? th_flux(txp, tc, vx1, nlam, i0) : q0.p[oX + 1];
+        const T qry = yr ? th_flux(typ, tc, vy1, nlam, i1) : q1.p[oY + yY];
+        const T qrz = k + 1 < hi2 ? th_flux(tp, tc, vz1, nlam, i2) : q2.p[oZ + sZ];
+        const T dv = ((qrx - qlx) * i0 + (qry - qly) * i1) + (qrz - qlz) * i2;
+        if (act) {
+            q0.p[oX] = qlx;
+            q1.p[oY] = qly;
+            q2.p[oZ] = qlz;
+            Tout.p[oC] = to - dt * dv;
+        }
+        tc = tp;
+        qlz = qrz;
+        oC += sC;
+        oX += sX;
+        oY += sY;
+        oZ += sZ;
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) thermal_update_kernel(Box3 b, DFL<T> Tf, DFL<T> To, DFL<T> q0, DFL<T> q1,
                                                              DFL<T> q2, int nd, T dt, T i0, T i1, T i2) {
@@ -823,6 +891,29 @@ int chmy_thermal_step(const chmy_grid_t* g, const chmy_box_t* box, const chmy_fi
         DFL<T> v0 = lview<T>(V[0]), q0 = lview<T>(qT[0]);
         DFL<T> v1 = g->nd > 1 ? lview<T>(V[1]) : v0, v2 = g->nd > 2 ? lview<T>(V[2]) : v0;
         DFL<T> q1 = g->nd > 1 ? lview<T>(qT[1]) : q0, q2 = g->nd > 2 ? lview<T>(qT[2]) : q0;
+        // 3D with one stride set per location (Center; each V_D with qT_D): the z-march
+        bool march = g->nd == 3;
+        if (march) {
+            const DF<T> c = view<T>(*Tf), c1 = view<T>(*T_out), c2 = view<T>(*To);
+            march = c1.sy == c.sy && c1.sz == c.sz && c2.sy == c.sy && c2.sz == c.sz;
+            for (int d = 0; d < 3; ++d) {
+                const DF<T> a = view<T>(V[d]), q = view<T>(qT[d]);
+                march = march && a.sy == q.sy && a.sz == q.sz;
+            }
+        }
+        if (march) {
+            const int nbx = (int)((b.len[0] + 63) / 64), nby = (int)((b.len[1] + kThY - 1) / kThY);
+            const int64_t nch = (b.len[2] + 15) / 16;
+            const int kchunk = (int)((b.len[2] + nch - 1) / nch);
+            const int64_t nb = (int64_t)nbx * nby * nch;
+            if (nb > INT32_MAX) return set_error(CHMY_E_SIZE, "thermal step: box too large");
+            hipLaunchKernelGGL(thermal_step3_kernel<T>, dim3((unsigned)nb), dim3(64, kThY, 1), 0, (hipStream_t)stream,
+                               b, view<T>(*Tf), view<T>(*T_out), view<T>(*To), view<T>(V[0]), view<T>(V[1]),
+                               view<T>(V[2]), view<T>(qT[0]), view<T>(qT[1]), view<T>(qT[2]), -T(lam), T(dt),
+                               T(g->inv_spacing[0]), T(g->inv_spacing[1]), T(g->inv_spacing[2]), nbx, nby, kchunk);
+            CHMY_HIP(hipGetLastError());
+            return 0;
+        }
         dim3 grd = grid_for(b, blk2());
         hipLaunchKernelGGL(thermal_step_kernel<T>, grd, blk2(), 0, (hipStream_t)stream, b, t, to, o, v0, v1, v2, q0,
                            q1, q2, g->nd, -T(lam), T(dt), T(g->inv_spacing[0]), T(g->inv_spacing[1]),

@@ -164,14 +164,6 @@ __device__ __forceinline__ T lerp4(T a0, T a1, T b0, T b1) {
     return fma(T(0.5), B - A, A);
 }
 
-// Block id -> tile, so that the blocks an XCD receives (ids congruent mod 8)
-// form one contiguous run of tiles and share halo rows in that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-    const int per = nb / 8;
-    if (bid >= 8 * per) return bid;
-    return (bid % 8) * per + bid / 8;
-}
-
 // Tuning knobs fixed at build time: rows per block (FP64 / FP32 sweeps) and
 // the minimum waves per SIMD the register allocation must allow (0 = no bound).
 #ifndef CHMY_FUSED_TW64

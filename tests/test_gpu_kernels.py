@@ -441,6 +441,47 @@ def test_thermal_step_equals_two_kernels(arch, T, nd):
         chmy.kernels.thermal_step.launch(g, lo, ws, Tf, Tf, To, q, V, lam, dt, g)
 
 
+@pytest.mark.parametrize("T", TYPES)
+@pytest.mark.parametrize("dims", [(130, 19, 45), (64, 6, 30), (65, 9, 2), (3, 2, 70)])
+def test_thermal_step_3d_march_equals_two_kernels(arch, T, dims):
+    """The 3D thermal step marches z over chunks of up to 16 planes in 64 x 4
+    blocks: several x tiles, y tiles and z chunks, exactly full and one-over
+    tiles, boxes that start inside the grid and end before its last index
+    (the right faces' fluxes then come from qT's storage), against the two
+    launches, bitwise, T and every qT point."""
+    g = chmy.UniformGrid(origin=(-0.5, -0.5, -0.5), extent=(1.0, 0.9, 1.2), dims=dims)
+    og = ogrid(g)
+    rng = np.random.default_rng(sum(dims))
+    Tf, _ = pair(arch, g, og, (Cc(),) * 3, T, rng=rng)
+    To, _ = pair(arch, g, og, (Cc(),) * 3, T, rng=rng)
+    V = [pair(arch, g, og, vloc(D, 3), T, rng=rng)[0] for D in range(3)]
+    qs = [pair(arch, g, og, vloc(D, 3), T, rng=rng)[0] for D in range(3)]
+    lam, dt = 0.7, 0.21 * min(chmy.spacing(g)) ** 2
+    bits = np.int64 if T == np.float64 else np.int32
+    n = g.dims
+    boxes = [((0, 0, 0), tuple(m + 2 for m in n)), ((1, 1, 1), tuple(m for m in n)),
+             ((2, 0, 1), (n[0] - 1, n[1] + 1, n[2]))]
+    for lo, ws in boxes:
+        Tn = chmy.Field(arch, g, Cc(), Tf.data.dtype)
+        q = [chmy.Field(arch, g, f.loc, f.data.dtype) for f in qs]
+        for a, b in zip(q, qs):
+            a.data.copy_(b.data)
+        chmy.kernels.thermal_step.launch(g, lo, ws, Tf, Tn, To, q, V, lam, dt, g)
+        T2 = chmy.Field(arch, g, Cc(), Tf.data.dtype)
+        T2.data.copy_(Tf.data)
+        q2 = [chmy.Field(arch, g, f.loc, f.data.dtype) for f in qs]
+        for a, b in zip(q2, qs):
+            a.data.copy_(b.data)
+        chmy.kernels.update_thermal_flux.launch(g, lo, ws, q2, T2, V, lam, g)
+        chmy.kernels.update_thermal.launch(g, lo, ws, T2, To, q2, dt, g)
+        chmy.synchronize()
+        sl = tuple(slice(1 + l, 1 + l + w) for l, w in zip(reversed(lo), reversed(ws)))
+        assert np.array_equal(dev(Tn)[sl].view(bits), dev(T2)[sl].view(bits)), f"box {lo} {ws}"
+        assert not np.any(dev(Tn)[sl] == 0)
+        for a, b in zip(q, q2):
+            assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"qT box {lo} {ws}"
+
+
 @pytest.mark.parametrize("ow", [(16, 8), (3, 5)])
 def test_launcher_outer_width_matches(arch, ow):
     """Launcher(outer_width): inner box + 2N slabs with per-side bc
