@@ -219,7 +219,7 @@ class PTStokes:
 
     def pass2_first(self, V_first):
         """pass2() that also leaves the first iteration's V, after its V batch,
-        in the VectorField V_first (chmy_stokes_iter2_first; 3D, FP64, no
+        in the VectorField V_first (chmy_stokes_iter2_first; FP64, no
         forcing): bitwise V after the first of two iteration() calls at every
         point either writes.  V_first's other points keep their values."""
         g = self.grid

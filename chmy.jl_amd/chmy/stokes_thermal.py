@@ -8,9 +8,9 @@ update_stress! + update_velocity! with the V batch, followed from the second
 time step on by update_thermal_flux!, update_thermal! and the T Neumann
 batch (one launch into the second of two T buffers, then the batch on it:
 the same values as the two launches in place; in the first step, with no
-thermal update, the iterations between checks run in one native call; in 3D
-two iterations between checks run as one pass that also keeps the first
-iteration's V for its thermal update, chmy_stokes_iter2_first);
+thermal update, the iterations between checks run in one native call; with
+the thermal update, two iterations between checks run as one pass that also
+keeps the first iteration's V for its thermal update, chmy_stokes_iter2_first);
 every ncheck iterations the residual (∇V, r_V with Dirichlet-zeroed
 boundary vertices, scaled by τsc, lz/psc) decides convergence, and from the
 third step on ncheck = ceil(0.5 n_x).  With exact=True the states are
@@ -48,9 +48,9 @@ class StokesThermal:
         self.qT = VectorField(arch, grid)
         self.launch = Launcher(arch, grid, sync=False)
         self.scales = dict(tau_sc=tau_sc, lz=L[-1], psc=self.G)
-        # 3D: two PT iterations per pass inside the thermal PT loop, with the
+        # two PT iterations per pass inside the thermal PT loop, with the
         # first iteration's V kept in _V1 for its thermal update
-        self.pairs = bool(pairs) and grid.nd == 3
+        self.pairs = bool(pairs) and grid.nd in (2, 3)
         self._V1 = VectorField(arch, grid) if self.pairs else None
 
     @property

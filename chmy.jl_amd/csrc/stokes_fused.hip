@@ -768,8 +768,12 @@ __device__ __forceinline__ void row2(const Kc<T>& K, T ix, T iy, bool in_j, bool
 // batch between them.
 template <class T, bool FVISC, bool EXACT, int OPT, bool BF, bool RG>
 __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, FreeSlip2 fs) {
-    static_assert(OPT == 0 || OPT == 1, "two-iteration pass: plain or forcing-only (rho g by RG)");
-    constexpr bool HF = OPT == 1;
+    static_assert(OPT == 0 || OPT == 1 || OPT == 2, "two-iteration pass: plain, forcing-only (rho g by RG), "
+                                                    "2: plain + first V");
+    static_assert(OPT < 2 || sizeof(T) == 8, "the first iteration's V is written in FP64 only");
+    // WA: A's V (the first iteration's, before its V batch) is stored into
+    // a.rVx/rVy at the points and rows B stores its own V
+    constexpr bool HF = OPT == 1, WA = OPT == 2;
     const Kc<T>& K = a.k;
     const int lane = threadIdx.x;
     const int xt = blockIdx.x * kTBW + threadIdx.y;
@@ -860,6 +864,14 @@ __global__ void __launch_bounds__(kFX * kTBW) fused2d_tb_kernel(Fused2<T> a, Fre
             row2<T, FVISC, EXACT>(K, ix, iy, inA, inA1, HF, RG, A, cur.vx1, cur.vy1, P, txx, tyy, txy1, oxx, oyy,
                                   oxy, eta1, eml, fP, fVx, fVy, rg, vxn, vyn, txy_prev);
             const bool keep = inA && t > 0;          // the first step has no velocity
+            if constexpr (WA) {
+                if (ra >= jb && ra < je && own && inA) {
+                    uint32_t q[NLOC2];
+                    rowoff(ra, q);
+                    sto(a.rVx, q[L2X], vxn);
+                    sto(a.rVy, q[L2Y], vyn);
+                }
+            }
             rP[t] = P;
             rtxx[t] = txx;
             rtyy[t] = tyy;
@@ -1772,6 +1784,26 @@ static int fused_launch(const chmy_grid_t* g, const chmy_field_t* V, const chmy_
             dim3 grd((unsigned)((a.ntx + kTBW - 1) / kTBW), (unsigned)a.nchunks, 1);
             const bool hf = a.has_f;
             const bool hr = rg != nullptr;
+            if (vfirst) {   // validated: FP64, no forcing
+                if constexpr (std::is_same<T, double>::value) {
+                    a.rVx = P(vfirst[0]);
+                    a.rVy = P(vfirst[1]);
+#define CHMY_TB2F(FV, EX)                                                                                          \
+    do {                                                                                                           \
+        if (hr) hipLaunchKernelGGL((fused2d_tb_kernel<double, FV, EX, 2, false, true>), grd, blk, 0, s, a, *tb);    \
+        else hipLaunchKernelGGL((fused2d_tb_kernel<double, FV, EX, 2, false, false>), grd, blk, 0, s, a, *tb);      \
+    } while (0)
+                    if (fv && exact) CHMY_TB2F(true, true);
+                    else if (fv) CHMY_TB2F(true, false);
+                    else if (exact) CHMY_TB2F(false, true);
+                    else CHMY_TB2F(false, false);
+#undef CHMY_TB2F
+                    CHMY_HIP(hipGetLastError());
+                    return 0;
+                } else {
+                    return set_error(CHMY_E_DTYPE, "the first iteration's V is written in FP64 only");
+                }
+            }
 #define CHMY_TB2R(FV, EX, RG)                                                                                 \
     do {                                                                                                      \
         constexpr bool F32 = std::is_same<T, float>::value;                                                   \
@@ -1902,9 +1934,9 @@ static int stokes_iter_impl(const chmy_grid_t* g, const chmy_field_t* V, const c
     for (int c = 0; c < nt; ++c)
         if (tau[c].data == tau_out[c].data) return set_error(CHMY_E_ARG, "tau and tau_out must be distinct buffers");
     if (Pr->data == Pr_out->data) return set_error(CHMY_E_ARG, "Pr and Pr_out must be distinct buffers");
-    if (V_first) {   // the first iteration's V: FP64 3D passes without forcing
-        if (!tb3 || dt_ != CHMY_F64 || f_V)
-            return set_error(CHMY_E_ARG, "the first iteration's V is written by FP64 3D passes without forcing");
+    if (V_first) {   // the first iteration's V: FP64 passes without forcing
+        if (!(tb || tb3) || dt_ != CHMY_F64 || f_V)
+            return set_error(CHMY_E_ARG, "the first iteration's V is written by FP64 passes without forcing");
         CHMY_TRY(check_vector(g, V_first, "V_first", dt_));
         for (int d = 0; d < nd; ++d) {
             if (!halo1(&V_first[d])) return set_error(CHMY_E_HALO, "fused iteration needs halo 1 on every field");
@@ -2106,8 +2138,8 @@ int chmy_stokes_iter2(const chmy_grid_t* g, const chmy_field_t* V, const chmy_fi
 // chmy_stokes_iter2 that also writes the first iteration's V (before its V
 // batch) into V_first, at the points the pass writes V_out (build extension):
 // in the physical time loop update_thermal_flux! reads V after every PT
-// iteration.  3D, FP64, no forcing; the caller applies the V batch to V_first
-// and to V_out.
+// iteration.  2D and 3D, FP64, no forcing; the caller applies the V batch to
+// V_first and to V_out.
 int chmy_stokes_iter2_first(const chmy_grid_t* g, const chmy_field_t* V, const chmy_field_t* tau,
                             const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                             const chmy_field_t* Pr_out, const chmy_field_t* V_first, const chmy_field_t* tau_old,
@@ -2119,21 +2151,25 @@ int chmy_stokes_iter2_first(const chmy_grid_t* g, const chmy_field_t* V, const c
     if (!V_first) return set_error(CHMY_E_ARG, "V_first is NULL");
     if (!bc_counts || !bc_comp || !bcs) return set_error(CHMY_E_ARG, "V batch is NULL");
     if (flags & ~CHMY_ITER_EXACT) return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first takes CHMY_ITER_EXACT only");
-    if (g->nd != 3) return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first is 3D only");
+    const int nd = g->nd;
+    if (nd != 2 && nd != 3) return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first is 2D or 3D");
     int total = 0;
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < 2 * nd; ++q) {
         if (bc_counts[q] < 0) return set_error(CHMY_E_ARG, "negative batch count");
         total += bc_counts[q];
     }
     for (int q = 0; q < total; ++q)
-        if (bc_comp[q] < 0 || bc_comp[q] >= 3) return set_error(CHMY_E_ARG, "bc_comp out of range");
+        if (bc_comp[q] < 0 || bc_comp[q] >= nd) return set_error(CHMY_E_ARG, "bc_comp out of range");
+    FreeSlip2 fs;
     FreeSlip3 fs3;
-    if (!free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3))
+    const bool tb2 = nd == 2 && free_slip_2d(g, bc_counts, bc_comp, bcs, &fs);
+    const bool tb3 = nd == 3 && free_slip_3d(g, bc_counts, bc_comp, bcs, &fs3);
+    if (!tb2 && !tb3)
         return set_error(CHMY_E_ARG, "chmy_stokes_iter2_first needs the free-slip V batch (each component Dirichlet "
                                      "on its own dim, Neumann on the others, scalar values, >= 2 cells per dim)");
     return stokes_iter_impl(g, V, tau, Pr, V_out, tau_out, Pr_out, tau_old, eta, eta_ml, rhog, nullptr, nullptr,
-                            nullptr, nullptr, flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream, nullptr,
-                            &fs3, V_first);
+                            nullptr, nullptr, flags, eta_s, eta_ve_s, G, dt, dtau_Pr, dtau_r, nudtau, stream,
+                            tb2 ? &fs : nullptr, tb3 ? &fs3 : nullptr, V_first);
 }
 
 }  // extern "C"

@@ -713,14 +713,14 @@ end
     iter2_first!(state0, state1, V_first, τ_old, η, η_ml, grid, bc_V, p; rhog=nothing, flags=0)
 
 `iter2!` that also stores the first iteration's V (before its V batch) into
-`V_first` (3D, Float64; build extension).  After the V batch on `V_first`
+`V_first` (2D or 3D, Float64; build extension).  After the V batch on `V_first`
 and on `state1`'s V, `V_first` is bitwise V after the first of the two
 iterations: the V that the first iteration's `update_thermal_flux!` reads in
-the physical time loop (stokes_3d_inc_ve_T.jl:164-169).  `rhog` is the
-`ρgz` field update_velocity! subtracts.
+the physical time loop (stokes_3d_inc_ve_T.jl:164-169, stokes_2d_inc_ve_T.jl:145-151).
+`rhog` is the `ρgz` (2D: `ρgy`) field update_velocity! subtracts.
 """
-function iter2_first!(state0, state1, V_first, τ_old, η, η_ml, grid::UniformGrid{3}, bc_V, p;
-                      rhog=nothing, flags::Integer=0)
+function iter2_first!(state0, state1, V_first, τ_old, η, η_ml, grid::UniformGrid{N}, bc_V, p;
+                      rhog=nothing, flags::Integer=0) where {N}
     (V0, τ0, P0), (V1, τ1, P1) = state0, state1
     counts, comps, cs, keep = vbatch(grid, bc_V, V0)
     ηd = η isa Number ? nothing : Ref(desc(η))

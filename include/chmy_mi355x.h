@@ -320,13 +320,14 @@ int chmy_stokes_iter2(const chmy_grid_t* grid, const chmy_field_t* V, const chmy
 /* chmy_stokes_iter2 that also stores the first iteration's V, before its V
  * batch, into V_first (build extension).  In the physical time loop the
  * reference runs update_thermal_flux! and update_thermal! after every PT
- * iteration, reading V (examples/stokes_3d_inc_ve_T.jl:164-169); with V_first
+ * iteration, reading V (examples/stokes_3d_inc_ve_T.jl:164-169,
+ * stokes_2d_inc_ve_T.jl:145-151); with V_first
  * two PT iterations can run as one pass and both thermal updates still see
  * their own V.  V_first is written at exactly the points the pass writes
  * V_out; the caller applies the V batch to both.  Then V_first is bitwise
- * V after the first chmy_stokes_iter + batch.  3D, FP64, no f_V/f_P (else
- * CHMY_E_ARG); V_first distinct from V and V_out; other arguments as in
- * chmy_stokes_iter2. */
+ * V after the first chmy_stokes_iter + batch.  2D or 3D, FP64, no f_V/f_P
+ * (else CHMY_E_ARG or CHMY_E_DTYPE); V_first distinct from V and V_out; other
+ * arguments as in chmy_stokes_iter2. */
 int chmy_stokes_iter2_first(const chmy_grid_t* grid, const chmy_field_t* V, const chmy_field_t* tau,
                             const chmy_field_t* Pr, const chmy_field_t* V_out, const chmy_field_t* tau_out,
                             const chmy_field_t* Pr_out, const chmy_field_t* V_first, const chmy_field_t* tau_old,

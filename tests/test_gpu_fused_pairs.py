@@ -292,6 +292,24 @@ def test_iter2_first_keeps_the_first_iteration(arch, exact, visc, rhog, dims):
                 raise AssertionError(f"V_first {n} differs at {len(bad)} points, first {bad[:4].tolist()}")
 
 
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("visc", ["field", "scalar"])
+@pytest.mark.parametrize("rhog", [True, False])
+@pytest.mark.parametrize("dims", [(70, 45), (2, 2), (129, 23)])
+def test_iter2_first_2d_keeps_the_first_iteration(arch, exact, visc, rhog, dims):
+    """2D: the same as test_iter2_first_keeps_the_first_iteration."""
+    g = chmy.UniformGrid(arch, origin=(-0.5, -0.5), extent=(1.0, 1.3), dims=dims)
+    for vals in (SHEAR, GENERAL):
+        state, vf = _run(arch, g, np.float64, exact, visc, None, vals, 2, True, 52, rhog=rhog, mode="iter2_first")
+        _compare(state, _run(arch, g, np.float64, exact, visc, None, vals, 2, False, 52, rhog=rhog),
+                 f"iter2_first state, dims {dims}")
+        one = _run(arch, g, np.float64, exact, visc, None, vals, 1, False, 52, rhog=rhog)
+        for n, x, y in zip(("Vx", "Vy"), vf, one[:2]):
+            if not np.array_equal(x.view(np.int64), y.view(np.int64)):
+                bad = np.argwhere(x.view(np.int64) != y.view(np.int64))
+                raise AssertionError(f"V_first {n} differs at {len(bad)} points, first {bad[:4].tolist()}")
+
+
 def test_iter2_first_rejects_what_it_does_not_run(arch):
     """An FP32 pass, or an FP32 V_first beside FP64 fields, is refused before any launch."""
     g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(8, 8, 8))

@@ -113,20 +113,22 @@ def test_time_loop_matches_reference_sequence(arch, nd):
 
 
 @pytest.mark.parametrize("exact", [True, False])
-def test_time_loop_pairs_equal_single_iterations(arch, exact):
-    """3D: the thermal PT loop in passes of two iterations (the first
+@pytest.mark.parametrize("nd", [2, 3])
+def test_time_loop_pairs_equal_single_iterations(arch, exact, nd):
+    """The thermal PT loop in passes of two iterations (the first
     iteration's V kept for its thermal update, chmy_stokes_iter2_first)
     against one iteration per launch: same iteration counts and residuals,
     bitwise equal V, Pr, tau and T.  An odd ncheck mixes passes with the
     single iterations before and at each check."""
-    n = (21, 17, 14)
+    n = (21, 17, 14) if nd == 3 else (45, 38)
     L = 2.0
-    g = chmy.UniformGrid(arch, origin=(-L / 2,) * 3, extent=(L,) * 3, dims=n)
-    rg_arr = _incl(g, ("C", "C", "V"), 1.0, 0.0, 0.15 * L)
-    T_arr = _incl(g, ("C", "C", "C"), 1.0, 0.1, 0.15 * L)
+    g = chmy.UniformGrid(arch, origin=(-L / 2,) * nd, extent=(L,) * nd, dims=n)
+    rg_loc = ("C",) * (nd - 1) + ("V",)
+    rg_arr = _incl(g, rg_loc, 1.0, 0.0, 0.15 * L)
+    T_arr = _incl(g, ("C",) * nd, 1.0, 0.1, 0.15 * L)
     runs = []
     for pairs in (True, False):
-        rhog = chmy.Field(arch, g, (Cc(), Cc(), Vv()))
+        rhog = chmy.Field(arch, g, tuple(Vv() if c == "V" else Cc() for c in rg_loc))
         chmy.set_storage(rhog, rg_arr)
         M = StokesThermal(arch, g, eta=10.0, G=1.0, rhog=rhog, exact=exact, pairs=pairs)
         chmy.set_storage(M.T, T_arr)
