@@ -263,7 +263,10 @@ __global__ void __launch_bounds__(256) sia_adj5(SiaF<T> s, SiaW<T> w, DF<T> lo, 
 #ifndef CHMY_SIA_TY
 #define CHMY_SIA_TY 8
 #endif
-constexpr int kSX = CHMY_SIA_TX, kSY = CHMY_SIA_TY, kST = 256;
+#ifndef CHMY_SIA_THREADS
+#define CHMY_SIA_THREADS 256
+#endif
+constexpr int kSX = CHMY_SIA_TX, kSY = CHMY_SIA_TY, kST = CHMY_SIA_THREADS;
 
 // S = B + H staged in LDS on the tile + R ring (zero outside cells 0..n+1,
 // which no evaluation reads); the formulas are those of SiaF.
