@@ -35,6 +35,19 @@ constexpr int kRowChunk = 4;   // 64-wide slices of an item
 constexpr int kSeg = 64 * kRowChunk;
 constexpr int kRW = 4;         // items of a wave in flight
 
+// item -> (segment, row j, plane k) in 32-bit arithmetic (the host checks
+// that the items of a box number below 2^32): 64-bit divisions per item
+// cost more than the item's loads; one segment per row needs none for it.
+__device__ __forceinline__ void item_rc(int64_t it, uint32_t nseg, uint32_t ny, int64_t& sg, int64_t& j,
+                                        int64_t& k) {
+    const uint32_t u = (uint32_t)it;
+    const uint32_t r = nseg == 1 ? u : u / nseg;
+    const uint32_t kk = r / ny;
+    sg = (int64_t)(u - r * nseg);
+    j = (int64_t)(r - kk * ny);
+    k = (int64_t)kk;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned long long* out) {
     __shared__ double part[kRB / 64];
@@ -51,8 +64,8 @@ __global__ void __launch_bounds__(kRB) maxabs_kernel(Box3 b, DF<T> f, unsigned l
 #pragma unroll
         for (int u = 0; u < kRW; ++u) {
             const int64_t it = i0 + u;
-            const int64_t r = it / nseg, sg = it - r * nseg;
-            const int64_t j = r % ny, k = r / ny;
+            int64_t sg, j, k;
+            item_rc(it, (uint32_t)nseg, (uint32_t)ny, sg, j, k);
             const bool ok = it < items;
             base[u] = ok ? &f(b.lo[0] + sg * kSeg, b.lo[1] + j, b.lo[2] + k) : nullptr;
             len[u] = ok ? (nx - sg * kSeg < kSeg ? nx - sg * kSeg : kSeg) : 0;
@@ -107,8 +120,8 @@ __global__ void __launch_bounds__(kRB) sumsq_kernel(Box3 b, DF<T> f, double* par
 #pragma unroll
         for (int u = 0; u < kRW; ++u) {
             const int64_t it = i0 + u;
-            const int64_t r = it / nseg, sg = it - r * nseg;
-            const int64_t j = r % ny, k = r / ny;
+            int64_t sg, j, k;
+            item_rc(it, (uint32_t)nseg, (uint32_t)ny, sg, j, k);
             const bool ok = it < items;
             base[u] = ok ? &f(b.lo[0] + sg * kSeg, b.lo[1] + j, b.lo[2] + k) : nullptr;
             len[u] = ok ? (nx - sg * kSeg < kSeg ? nx - sg * kSeg : kSeg) : 0;
@@ -186,6 +199,8 @@ static int reduce_box(const chmy_field_t* f, const chmy_box_t* box, Box3* out) {
             return set_error(CHMY_E_BOX, "reduction box outside the storage of the field along dim %d", d);
     }
     *out = box3(b, f->nd);
+    const int64_t items = out->len[1] * out->len[2] * ((out->len[0] + kSeg - 1) / kSeg);
+    if (items >= (int64_t(1) << 32)) return set_error(CHMY_E_SIZE, "reduction box of 2^32 rows or more");
     return 0;
 }
 
