@@ -369,7 +369,7 @@ void bc_side(const orc_grid* gg, int D, int side, int K, const orc_field* fs, co
     // tangential range 0..n_t+2 (worksize = remove_dim(dim, size(grid, Vertex()) .+ 2), I = J .- 1)
     long hi[3] = {1, 1, 1};
     int td[3], nt = 0;
-    for (int d = 0; d < g.nd; ++d) if (d != D) { td[nt] = d; hi[nt] = g.n[d] + 3; ++nt; }
+    for (int d = 0; d < gg->nd; ++d) if (d != D) { td[nt] = d; hi[nt] = gg->n[d] + 3; ++nt; }
     for (long b = 0; b < hi[1]; ++b)
         for (long a = 0; a < hi[0]; ++a) {
             Ix It{{1, 1, 1}};
