@@ -161,6 +161,10 @@ inline Box3 box3(const chmy_box_t& b, int nd) {
     return r;
 }
 
+// A box with no points: the entry point launches nothing and returns 0 (a
+// KernelAbstractions launch over an empty worksize does no work).
+inline bool no_work(const Box3& b) { return b.len[0] <= 0 || b.len[1] <= 0 || b.len[2] <= 0; }
+
 // Block of 256 threads as 64 (x, one wave) x 4 (y); grid covers the box.
 inline dim3 grid_for(const Box3& b, dim3 blk) {
     return dim3((unsigned)((b.len[0] + blk.x - 1) / blk.x), (unsigned)((b.len[1] + blk.y - 1) / blk.y),

@@ -550,6 +550,7 @@ static int diffusion_launch(const chmy_grid_t* g, const chmy_box_t* box, const c
     chmy_box_t bx;
     CHMY_TRY(check_box(g, box, &bx));
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     DFL<T> c = lview<T>(*Cf);
     DFL<T> q0 = lview<T>(q[0]);
     DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
@@ -573,6 +574,7 @@ static int copy_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_fi
     if (Tf) { c.src[c.n] = lview<T>(*Tf); c.dst[c.n] = lview<T>(*To); ++c.n; }
     for (int q = 0; q < nt; ++q) { c.src[c.n] = lview<T>(tau[q]); c.dst[c.n] = lview<T>(tauo[q]); ++c.n; }
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     hipLaunchKernelGGL(copy_kernel<T>, grid_for(b, blk2()), blk2(), 0, s, b, c);
     CHMY_HIP(hipGetLastError());
     return 0;
@@ -582,6 +584,7 @@ template <class T>
 static int thermal_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_field_t* q, const chmy_field_t* Tf,
                           const chmy_field_t* other, double scalar, bool flux, hipStream_t s) {
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     DFL<T> t = lview<T>(*Tf), o = lview<T>(other[0]);
     DFL<T> q0 = lview<T>(q[0]);
     DFL<T> q1 = g->nd > 1 ? lview<T>(q[1]) : q0, q2 = g->nd > 2 ? lview<T>(q[2]) : q0;
@@ -631,6 +634,7 @@ int chmy_op(int op, const chmy_grid_t* g, const chmy_box_t* box, const chmy_fiel
         CHMY_TRY(check_field(g, &src[q], "src", want, dt));
     }
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     OpArgs a{op, g->nd, dim, nsrc};
     hipStream_t s = (hipStream_t)stream;
     dim3 grd = grid_for(b, blk2());
@@ -683,6 +687,7 @@ int chmy_op_masked(int op, const chmy_grid_t* g, const chmy_box_t* box, const ch
     }
     if (k) CHMY_TRY(check_field(g, k, "k", nullptr, dt));
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     OpArgs a{op, g->nd, dim, nsrc};
     hipStream_t s = (hipStream_t)stream;
     dim3 grd = grid_for(b, blk2());
@@ -744,6 +749,7 @@ int chmy_diffusion_step(const chmy_grid_t* g, const chmy_box_t* box, const chmy_
     chmy_box_t bx;
     CHMY_TRY(check_box(g, box, &bx));
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     auto launch = [&](auto zero) -> int {
         using T = decltype(zero);
         DFL<T> c = lview<T>(*C), co = lview<T>(*C_out);
@@ -885,6 +891,7 @@ int chmy_thermal_step(const chmy_grid_t* g, const chmy_box_t* box, const chmy_fi
     CHMY_TRY(check_vector(g, V, "V", Tf->dtype));
     if (Tf->data == T_out->data) return set_error(CHMY_E_ARG, "T and T_out must be distinct buffers");
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     auto launch = [&](auto zero) -> int {
         using T = decltype(zero);
         DFL<T> t = lview<T>(*Tf), to = lview<T>(*T_out), o = lview<T>(*To);
@@ -932,6 +939,7 @@ int chmy_maxloc(const chmy_grid_t* g, const chmy_box_t* box, const chmy_field_t*
     CHMY_TRY(check_center(g, src, "src", -1));
     CHMY_TRY(check_center(g, dst, "dst", src->dtype));
     Box3 b = box3(bx, g->nd);
+    if (no_work(b)) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (src->dtype == CHMY_F64)
         hipLaunchKernelGGL(maxloc_kernel<double>, grid_for(b, blk2()), blk2(), 0, s, b, lview<double>(*dst),

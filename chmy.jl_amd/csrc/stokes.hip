@@ -198,6 +198,7 @@ static int stress_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chmy_
     a.dPr = T(dPr);
     a.dr = T(dr);
     Box3 b = box3(bx, nd);
+    if (no_work(b)) return 0;
     dim3 blk(kSX, kSY, 1), grd((unsigned)((b.len[0] + kSX - 1) / kSX), (unsigned)((b.len[1] + kSY - 1) / kSY),
                                (unsigned)b.len[2]);
     if (nd == 2) {
@@ -232,6 +233,7 @@ static int velocity_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chm
     a.dts = T(dt);
     a.nu = T(nu);
     Box3 b = box3(bx, nd);
+    if (no_work(b)) return 0;
     dim3 blk(kSX, kSY, 1), grd((unsigned)((b.len[0] + kSX - 1) / kSX), (unsigned)((b.len[1] + kSY - 1) / kSY),
                                (unsigned)b.len[2]);
     if (nd == 2) {

@@ -177,6 +177,7 @@ static int residual_launch(const chmy_grid_t* g, const chmy_box_t& bx, const chm
     a.dts = T(dt);
     a.Gdt = T(G) * T(dt);
     Box3 b = box3(bx, nd);
+    if (no_work(b)) return 0;
     dim3 blk(kRX, kRY, 1), grd((unsigned)((b.len[0] + kRX - 1) / kRX), (unsigned)((b.len[1] + kRY - 1) / kRY),
                                (unsigned)b.len[2]);
     if (nd == 2) hipLaunchKernelGGL((residual_kernel<T, TO, 2>), grd, blk, 0, s, b, a);
@@ -199,6 +200,7 @@ static int accumulate_launch(const chmy_grid_t* g, const chmy_box_t& bx, const c
     a.d[a.n] = view<TO>(*Prc);
     ++a.n;
     Box3 b = box3(bx, nd);
+    if (no_work(b)) return 0;
     dim3 blk(kRX, kRY, 1), grd((unsigned)((b.len[0] + kRX - 1) / kRX), (unsigned)((b.len[1] + kRY - 1) / kRY),
                                (unsigned)b.len[2]);
     hipLaunchKernelGGL((accumulate_kernel<T, TO>), grd, blk, 0, s, b, a);

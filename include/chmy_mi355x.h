@@ -15,6 +15,7 @@
  *    Launcher's box is lo = 0, len = ncells + 2 (src/KernelLaunch.jl:41,
  *    105-126: worksize n .+ 2 at Offset(-1)).  Boxes must lie in
  *    [0, ncells+1]; halos must be >= 1 so every ±1 neighbour is in storage.
+ *    A box with no points along some dim launches nothing and returns 0.
  *  - Ownership: the caller owns all device memory; descriptors are read
  *    during the call only.
  *  - Status: 0 ok; > 0 a hipError_t; < 0 misuse (CHMY_E_*).  The message of

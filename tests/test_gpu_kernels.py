@@ -482,6 +482,29 @@ def test_thermal_step_3d_march_equals_two_kernels(arch, T, dims):
             assert np.array_equal(dev(a).view(bits), dev(b).view(bits)), f"qT box {lo} {ws}"
 
 
+def test_empty_box_is_a_no_op(arch):
+    """A box with no points along some dim launches nothing and returns 0, as
+    a KernelAbstractions launch over an empty worksize does: every field keeps
+    its bits."""
+    g = grids(3)
+    og = ogrid(g)
+    rng = np.random.default_rng(77)
+    Tf, _ = pair(arch, g, og, (Cc(),) * 3, np.float64, rng=rng)
+    Tn, _ = pair(arch, g, og, (Cc(),) * 3, np.float64, rng=rng)
+    To, _ = pair(arch, g, og, (Cc(),) * 3, np.float64, rng=rng)
+    V = [pair(arch, g, og, vloc(D, 3), np.float64, rng=rng)[0] for D in range(3)]
+    q = [pair(arch, g, og, vloc(D, 3), np.float64, rng=rng)[0] for D in range(3)]
+    fields = [Tf, Tn, To] + V + q
+    before = [dev(f).copy() for f in fields]
+    for ws in ((0, 5, 5), (5, 0, 5), (5, 5, 0)):
+        chmy.kernels.thermal_step.launch(g, (1, 1, 1), ws, Tf, Tn, To, q, V, 0.7, 1e-3, g)
+        chmy.kernels.update_thermal_flux.launch(g, (1, 1, 1), ws, q, Tf, V, 0.7, g)
+        chmy.kernels.update_thermal.launch(g, (1, 1, 1), ws, Tf, To, q, 1e-3, g)
+    chmy.synchronize()
+    for a, f in zip(before, fields):
+        assert np.array_equal(a.view(np.int64), dev(f).view(np.int64))
+
+
 @pytest.mark.parametrize("ow", [(16, 8), (3, 5)])
 def test_launcher_outer_width_matches(arch, ow):
     """Launcher(outer_width): inner box + 2N slabs with per-side bc
