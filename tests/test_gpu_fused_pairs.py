@@ -311,15 +311,16 @@ def test_iter2_first_2d_keeps_the_first_iteration(arch, exact, visc, rhog, dims)
 
 
 def test_iter2_first_rejects_what_it_does_not_run(arch):
-    """An FP32 pass, or an FP32 V_first beside FP64 fields, is refused before any launch."""
+    """An FP32 pass, an FP32 V_first beside FP64 fields, or V_first aliasing the
+    input V, is refused before any launch."""
     g = chmy.UniformGrid(arch, origin=(-0.5,) * 3, extent=(1.0,) * 3, dims=(8, 8, 8))
     counts, comp, conds = _batch3(GENERAL3)
     p = pt_params(g)
     scal = (1.0, 0.5, 1.0, 0.3, p["dtau_Pr"], p["dtau_r"], p["nudtau"])
 
-    def call(dt, Vf_dt=None):
+    def call(dt, Vf_dt=None, alias=False):
         a, b = _Set(arch, g, dt), _Set(arch, g, dt)
-        Vf = chmy.VectorField(arch, g, Vf_dt or dt)
+        Vf = a.V if alias else chmy.VectorField(arch, g, Vf_dt or dt)
         told = chmy.TensorField(arch, g, dt)
         return _lib.lib().chmy_stokes_iter2_first(
             C.byref(g.desc()), _arr(list(a.V)), _arr(list(a.tau)), C.byref(a.Pr.desc()), _arr(list(b.V)),
@@ -328,6 +329,7 @@ def test_iter2_first_rejects_what_it_does_not_run(arch):
 
     assert call(torch.float32) != 0
     assert call(torch.float64, torch.float32) != 0
+    assert call(torch.float64, alias=True) != 0   # V_first must not be the input V
     assert call(torch.float64) == 0
     chmy.synchronize()
 
